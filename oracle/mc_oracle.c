@@ -1,0 +1,112 @@
+/*
+ * CPU ORACLE — test infrastructure only (see mc_oracle.h). Instantiates the
+ * REAL-generic body for float and double and holds the integer CsrPattern
+ * construction.
+ */
+#include "mc_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* powf_scalar(2.0) of the reference (S13, S17) is evaluated with the C
+ * library's powf, as burn-ndarray does. */
+
+/* ---------------- float instantiation ---------------- */
+#define REAL float
+#define SUF f
+#define RPow powf
+#define RSqrt sqrtf
+#define RLog logf
+#define RExp expf
+#define RSq(x) powf((x), 2.0f)
+#include "mc_oracle_body.inc"
+#undef REAL
+#undef SUF
+#undef RPow
+#undef RSqrt
+#undef RLog
+#undef RExp
+#undef RSq
+
+/* ---------------- double instantiation ---------------- */
+#define REAL double
+#define SUF d
+#define RPow pow
+#define RSqrt sqrt
+#define RLog log
+#define RExp exp
+#define RSq(x) pow((x), 2.0)
+#include "mc_oracle_body.inc"
+#undef REAL
+#undef SUF
+#undef RPow
+#undef RSqrt
+#undef RLog
+#undef RExp
+#undef RSq
+
+/* ---------------- CsrPattern::from_sparse ---------------- */
+
+static const int* g_sort_rows;
+static const int* g_sort_cols;
+
+static int cmp_rc(const void* a, const void* b) {
+    int ka = *(const int*)a, kb = *(const int*)b;
+    if (g_sort_rows[ka] != g_sort_rows[kb]) return g_sort_rows[ka] < g_sort_rows[kb] ? -1 : 1;
+    if (g_sort_cols[ka] != g_sort_cols[kb]) return g_sort_cols[ka] < g_sort_cols[kb] ? -1 : 1;
+    return ka < kb ? -1 : (ka > kb);
+}
+
+/* CsrPattern::from_sparse (sparse/mod.rs:173-227): sort COO by (row, col),
+ * emit each row's off-diagonals in ascending column order followed by the
+ * diagonal; then build_transposed_pattern (sparse/mod.rs:276-304). */
+int orc_pattern_from_coo(int n, int nnz_off, const int* rows, const int* cols,
+                         const float* vals, int* crow, int* col, int* row_for_nnz,
+                         float* adj_values, float* diag_mask, int* trans_crow,
+                         int* trans_col, int* trans_to_orig) {
+    for (int k = 0; k < nnz_off; ++k)
+        if (!(rows[k] > cols[k]) || rows[k] >= n || cols[k] < 0) return -1;
+    int* order = (int*)malloc(sizeof(int) * (size_t)(nnz_off > 0 ? nnz_off : 1));
+    for (int k = 0; k < nnz_off; ++k) order[k] = k;
+    g_sort_rows = rows;
+    g_sort_cols = cols;
+    qsort(order, (size_t)nnz_off, sizeof(int), cmp_rc);
+    int nnz = nnz_off + n;
+    int cursor = 0, w = 0;
+    crow[0] = 0;
+    for (int i = 0; i < n; ++i) {
+        while (cursor < nnz_off && rows[order[cursor]] == i) {
+            int k = order[cursor];
+            col[w] = cols[k];
+            row_for_nnz[w] = i;
+            adj_values[w] = vals[k];
+            diag_mask[w] = 0.0f;
+            ++w;
+            ++cursor;
+        }
+        col[w] = i;
+        row_for_nnz[w] = i;
+        adj_values[w] = 0.0f;
+        diag_mask[w] = 1.0f;
+        ++w;
+        crow[i + 1] = w;
+    }
+    free(order);
+    /* transposed view */
+    int* counts = (int*)calloc((size_t)n + 1, sizeof(int));
+    for (int k = 0; k < nnz; ++k) counts[col[k] + 1] += 1;
+    trans_crow[0] = 0;
+    for (int i = 1; i <= n; ++i) trans_crow[i] = trans_crow[i - 1] + counts[i];
+    int* cur = (int*)malloc(sizeof(int) * ((size_t)n + 1));
+    memcpy(cur, trans_crow, sizeof(int) * ((size_t)n + 1));
+    for (int k = 0; k < nnz; ++k) {
+        int r = col[k];
+        int pos = cur[r]++;
+        trans_col[pos] = row_for_nnz[k];
+        trans_to_orig[pos] = k;
+    }
+    free(counts);
+    free(cur);
+    return 0;
+}
