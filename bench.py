@@ -1,0 +1,280 @@
+"""Benchmark: reach-timesteps/s of the differentiable Muskingum-Cunge window,
+forward + reverse-time adjoint, on one MI355X.
+
+Workload (default, --config c2): a 350,000-reach synthetic river tree (each
+reach drains to one drawn from the next 256, about 2.7k levels deep), 720
+hourly steps, fp32 lateral inflow LogNormal(ln 0.5, 1) with storm pulses,
+per-reach Manning n and q_spatial as leaves. One bench step = the batched
+persistent forward window, the loss gradient (MSE against a fixed target at 64
+gauge reaches) and the persistent adjoint window, producing dL/dn and
+dL/dq_spatial for every reach. Inputs are resident in HBM before timing.
+
+Also reported: the same step in the plain per-timestep-launch form, the
+roofline of the dominant kernel (the persistent adjoint window) from HIP events
+on its own stream, gradient checks, and the CPU oracle timed on a bounded
+sample of the same network.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FWD_BYTES = 40    # algorithmic bytes per reach-timestep, forward
+ADJ_BYTES = 72    # adjoint
+CONFIGS = {
+    "c1": dict(R=5000, T=240, window=256, storms=False),
+    "c2": dict(R=350000, T=720, window=256, storms=True),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=48)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(basins shard across GPUs with no exchange, see DESIGN.md)"}))
+        sys.exit(2)
+
+    import torch
+    import ddrs_amd as D
+    from ddrs_amd import _native as NAT
+    from ddrs_amd import network as NW
+    from ddrs_amd.sparse import Network, SparseAdjacency
+
+    cfg = CONFIGS[args.config]
+    R, T = cfg["R"], cfg["T"]
+    ci = 1 if args.config == "c1" else 2
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    log(f"[bench] building network R={R} T={T}")
+    rows, cols = NW.gen_tree(R, cfg["window"], 1000 + ci)
+    at = NW.gen_attributes(R, 2000 + ci)
+    adj = SparseAdjacency.from_coo(R, rows, cols, None, at["length"], at["slope"])
+    t0 = time.time()
+    net = Network(adj)
+    log(f"[bench] plan {time.time() - t0:.2f}s: {net.n_chunks} chunks, {net.n_levels} levels")
+    f = lambda x: torch.as_tensor(np.ascontiguousarray(x, np.float32), device=dev)
+    n, qsp = f(at["n"]), f(at["q_spatial"])
+    p = torch.full((R,), 21.0, dtype=torch.float32, device=dev)
+    length, slope = f(at["length"]), f(at["slope"])
+    qprime = NW.gen_forcing_device(R, T, 3000 + ci, cfg["storms"], device=dev)
+    params = D.Params()
+    cp = params.to_c()
+    ws = net.workspace(T - 1, T - 1)
+    log(f"[bench] workspace {ws.nbytes() / 2**30:.2f} GiB")
+    stream = torch.cuda.current_stream()
+    sptr = stream.cuda_stream
+    ptr = lambda x: None if x is None else x.data_ptr()
+    NAT.check(NAT.lib.ddrs_mc_bind_statics(ws.handle, cp, ptr(n), ptr(qsp), ptr(p), ptr(length),
+                                           ptr(slope), None, sptr))
+    q0 = torch.empty(R, dtype=torch.float32, device=dev)
+    hot = net.workspace()
+    NAT.check(NAT.lib.ddrs_hotstart(hot.handle, cp, ptr(qprime[0]), ptr(q0), sptr))
+    hot.check()
+    Q = torch.empty((T, R), dtype=torch.float32, device=dev)
+    gQ = torch.zeros((T, R), dtype=torch.float32, device=dev)
+    gn, gqsp = torch.empty(R, device=dev), torch.empty(R, device=dev)
+    rng = np.random.default_rng(4000 + ci)
+    gauges = torch.as_tensor(np.sort(np.concatenate([[R - 1], rng.choice(R - 1, 63, replace=False)])),
+                             device=dev, dtype=torch.long)
+    # fixed observation target: the routed discharge at the gauges scaled by 0.9
+    NAT.check(NAT.lib.ddrs_mc_window_fwd(ws.handle, cp, T, ptr(qprime), ptr(q0), ptr(Q),
+                                         NAT.FORM_PERSISTENT, sptr))
+    ws.check()
+    obs = (0.9 * Q[:, gauges]).clone()
+    scale = 2.0 / (T * gauges.numel())
+
+    def step(form):
+        NAT.check(NAT.lib.ddrs_mc_window_fwd(ws.handle, cp, T, ptr(qprime), ptr(q0), ptr(Q), form,
+                                             sptr))
+        gQ[:, gauges] = scale * (Q[:, gauges] - obs)
+        NAT.check(NAT.lib.ddrs_mc_window_adj(ws.handle, cp, T, ptr(qprime), ptr(q0), ptr(gQ),
+                                             ptr(gn), ptr(gqsp), None, None, None, form, sptr))
+
+    for _ in range(args.warmup):
+        step(NAT.FORM_PERSISTENT)
+    ws.check()
+    NAT.check(NAT.lib.ddrs_set_timing(ws.handle, 1))
+    fwd_ms = adj_ms = 0.0
+    nf = na = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(NAT.FORM_PERSISTENT)
+        a = [np.zeros(1, np.int32), np.zeros(1, np.float32), np.zeros(1, np.int32),
+             np.zeros(1, np.float32)]
+        NAT.check(NAT.lib.ddrs_get_timing(ws.handle, *[x.ctypes.data for x in a]))
+        nf += int(a[0][0]); fwd_ms += float(a[1][0]); na += int(a[2][0]); adj_ms += float(a[3][0])
+        log(f"[bench] step {k + 1}/{args.steps} done")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ws.check()
+    NAT.check(NAT.lib.ddrs_set_timing(ws.handle, 0))
+    rts = R * (T - 1)
+    value = rts * args.steps / elapsed
+    ms_per_step = 1000.0 * elapsed / args.steps
+    log(f"[bench] persistent: {ms_per_step:.2f} ms/step, {value / 1e9:.3f} G reach-ts/s; "
+        f"fwd kernel {fwd_ms / max(nf, 1):.2f} ms, adj kernel {adj_ms / max(na, 1):.2f} ms")
+    gn_pers, gqsp_pers, Q_pers = gn.clone(), gqsp.clone(), Q.clone()
+
+    plain = None
+    if not args.no_plain:
+        NAT.check(NAT.lib.ddrs_set_timing(ws.handle, 1))
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        step(NAT.FORM_PLAIN)
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t1
+        ws.check()
+        a = [np.zeros(1, np.int32), np.zeros(1, np.float32), np.zeros(1, np.int32),
+             np.zeros(1, np.float32)]
+        NAT.check(NAT.lib.ddrs_get_timing(ws.handle, *[x.ctypes.data for x in a]))
+        NAT.check(NAT.lib.ddrs_set_timing(ws.handle, 0))
+        same = bool(torch.equal(gn, gn_pers) and torch.equal(gqsp, gqsp_pers) and torch.equal(Q, Q_pers))
+        plain = dict(value=rts / tp, ms_per_step=1000 * tp, fwd_launches=int(a[0][0]),
+                     fwd_kernel_ms=float(a[1][0]), adj_launches=int(a[2][0]),
+                     adj_kernel_ms=float(a[3][0]), bit_identical_to_persistent=same)
+        log(f"[bench] plain: {1000 * tp:.1f} ms/step ({value * tp / rts:.1f}x slower), identical={same}")
+
+    adj_launch_ms = adj_ms / max(na, 1)
+    fwd_launch_ms = fwd_ms / max(nf, 1)
+    steps_per_launch = T - 1
+    roof_adj = ADJ_BYTES * R * steps_per_launch / (adj_launch_ms * 1e-3) / 1e9
+    roof_fwd = FWD_BYTES * R * steps_per_launch / (fwd_launch_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_adj_window_{args.config}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(rows, cols, at, R, args.cpu_steps, ci, cfg["storms"], net, params)
+
+    out = {
+        "metric": "reach-timesteps/s fwd+adjoint (batched vs plain, 1 GPU); grad rel err vs FD",
+        "value": value,
+        "unit": "reach-timesteps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator; no network access)",
+        "config": {"workload": f"{args.config}: {R}-reach tree (window {cfg['window']}), "
+                               f"{T} hourly steps, forward + adjoint, MSE at 64 gauges",
+                   "reaches": R, "timesteps": T, "levels": net.n_levels, "form": "persistent",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": roof_adj, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": roof_adj / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_adj_window (persistent adjoint)",
+                     "bytes_per_reach_step": ADJ_BYTES, "launch_ms": adj_launch_ms,
+                     "fwd_kernel": {"achieved": roof_fwd, "bytes_per_reach_step": FWD_BYTES,
+                                    "launch_ms": fwd_launch_ms}},
+        "plain": plain,
+        "batched_over_plain": (value / plain["value"]) if plain else None,
+        "grad_check": grad_check(torch, D, NW, Network, SparseAdjacency),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def grad_check(torch, D, NW, Network, SparseAdjacency):
+    """GPU gradients against FP64 central differences of the double oracle on a
+    small network of the same generator (2,000 reaches, 24 steps, 8 reaches)."""
+    from oracle import oracle as O
+    R, T = 2000, 24
+    rows, cols = NW.gen_tree(R, 256, 11)
+    at = NW.gen_attributes(R, 12)
+    qp = NW.gen_forcing(R, T, 13, storms=True)
+    P = O.make_params()
+    pat = O.pattern_from_coo(R, rows, cols)
+    q0 = O.hotstart(P, pat, qp[0])
+    p = np.full(R, 21.0, np.float32)
+    net = Network(SparseAdjacency.from_coo(R, rows, cols, None, at["length"], at["slope"]))
+    f = lambda x: torch.as_tensor(np.ascontiguousarray(x, np.float32), device="cuda:0")
+    w = np.random.default_rng(14).uniform(0.5, 1.5, (T, R))
+    n = f(at["n"]).requires_grad_(True)
+    qsp = f(at["q_spatial"]).requires_grad_(True)
+    Q = D.route_window(net, D.Params(), n, qsp, f(p), f(qp), f(q0), f(at["length"]), f(at["slope"]))
+    Q.backward(f(w))
+    g = {"n": n.grad.cpu().numpy(), "qsp": qsp.grad.cpu().numpy()}
+    base = {"n": at["n"].astype(np.float64), "qsp": at["q_spatial"].astype(np.float64)}
+
+    def loss(v):
+        Qd = O.route_fwd(P, pat, v["n"], v["qsp"], p, at["length"], at["slope"], qp,
+                         q0=q0.astype(np.float64), dtype=np.float64)
+        return float((w * Qd).sum())
+
+    worst = 0.0
+    for i in np.random.default_rng(15).choice(R, 8, replace=False):
+        for key in ("n", "qsp"):
+            vp = {k: v.copy() for k, v in base.items()}
+            vm = {k: v.copy() for k, v in base.items()}
+            h = 1e-6 * max(abs(base[key][i]), 1e-3)
+            vp[key][i] += h
+            vm[key][i] -= h
+            fd = (loss(vp) - loss(vm)) / (2 * h)
+            worst = max(worst, abs(g[key][i] - fd) / max(abs(fd), 1e-12))
+    return {"max_rel_err_vs_fp64_fd": worst, "reaches": 8, "params": ["n", "q_spatial"],
+            "network": "2000-reach tree, 24 steps"}
+
+
+def cpu_baseline(rows, cols, at, R, steps, ci, storms, net, params):
+    """The oracle (f32 restatement of the reference CPU path, one thread) on the
+    same network for `steps` timesteps, forward + adjoint."""
+    from oracle import oracle as O
+    from ddrs_amd import network as NW
+    T = steps + 1
+    qp = NW.gen_forcing(R, T, 3000 + ci, storms)
+    P = O.make_params()
+    pat = O.pattern_from_coo(R, rows, cols)
+    q0 = O.hotstart(P, pat, qp[0])
+    p = np.full(R, 21.0, np.float32)
+    gQ = np.zeros((T, R), np.float32)
+    gQ[:, R - 1] = 1.0
+    t0 = time.perf_counter()
+    O.route_adj(P, pat, at["n"], at["q_spatial"], p, at["length"], at["slope"], qp, gQ, q0=q0)
+    dt = time.perf_counter() - t0
+    log(f"[bench] cpu oracle: {dt:.1f}s for {R}x{steps}")
+    return {"value": R * steps / dt, "unit": "reach-timesteps/s", "cores": 1, "kind": "port",
+            "sample": f"same {R}-reach network, {steps} timesteps; oracle route_adj "
+                      "(forward, per-step recompute and adjoint), fp32, one thread",
+            "cpu_model": _cpu_model(), "host_cores": os.cpu_count()}
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,263 @@
+// Per-reach Muskingum-Cunge physics for gfx950, held in registers.
+//
+// mc_coef:  S1..S23 of forward_chain_inner (src/routing/mmc_op.rs:1113-1236).
+// mc_local_bwd: the per-reach part of timestep_backward_core B25..B1
+//           (src/routing/mmc_op.rs:340-835); the two network terms (the
+//           transposed-solve result gb and the N^T gi sum) come in as arguments.
+//
+// Every statement is one reference tensor op in the reference's evaluation
+// order; the library is compiled with -ffp-contract=off so no multiply-add is
+// fused and each op rounds to fp32 exactly once, as on the reference CPU path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace ddrs {
+
+struct Phys {
+    float dt, q_lb, depth_lb, v_lb, bw_lb;
+};
+
+// Variants of the physics (config.rs:580-672): default corrected physics,
+// legacy ddr_match (celerity 5/3 v, X = x_storage), and enforce_positivity
+// (S18' K floor, S19' X cap).
+enum : int { PH_DEFAULT = 0, PH_DDR = 1, PH_POS = 2 };
+
+constexpr float kDelta = 1e-2f;  // POSITIVITY_DELTA, mmc_op.rs:57
+
+__device__ __forceinline__ float fmaxr(float x, float lo) { return (x < lo) ? lo : x; }
+__device__ __forceinline__ float fclamp(float x, float lo, float hi) {
+    return (x < lo) ? lo : ((x > hi) ? hi : x);
+}
+__device__ __forceinline__ float fminr(float a, float b) { return (b < a) ? b : a; }
+
+struct Coef {
+    float q_eps, ratio, denominator, depth, tw, ss_raw, ss, bw_raw, bw, area, wp, hyd_r;
+    float v_un, v_cl, cel, k_musk, x_eff, denom, c1, c2, c3, c4;
+};
+
+// S1..S23. sqrt_s = sqrtf(slope) (S3 and S15 both take sqrt(slope)).
+template <int PH>
+__device__ __forceinline__ void mc_coef(const Phys& P, float nman, float qsp, float psp, float q,
+                                        float length, float slope, float sqrt_s, float xst,
+                                        Coef& o) {
+    o.q_eps = qsp + 1e-6f;                                              // S1
+    float numerator = (q * nman) * (o.q_eps + 1.0f);                    // S2
+    o.denominator = psp * sqrt_s + 1e-8f;                               // S3
+    o.ratio = numerator / o.denominator;                                // S4
+    float exponent = (1.0f / (o.q_eps * 3.0f + 5.0f)) * 3.0f;          // S5
+    o.depth = fmaxr(powf(o.ratio, exponent), P.depth_lb);               // S6
+    o.tw = psp * powf(o.depth, o.q_eps);                                // S7
+    o.ss_raw = (o.tw * o.q_eps) / (o.depth * 2.0f);                     // S8
+    o.ss = fclamp(o.ss_raw, 0.5f, 50.0f);                               // S9
+    o.bw_raw = o.tw - (o.ss * o.depth) * 2.0f;                          // S10
+    o.bw = fmaxr(o.bw_raw, P.bw_lb);                                    // S11
+    o.area = ((o.tw + o.bw) * o.depth) / 2.0f;                          // S12
+    float root = sqrtf(powf(o.ss, 2.0f) + 1.0f);
+    o.wp = o.bw + (o.depth * root) * 2.0f;                              // S13
+    o.hyd_r = o.area / o.wp;                                            // S14
+    o.v_un = ((1.0f / nman) * powf(o.hyd_r, 2.0f / 3.0f)) * sqrt_s;     // S15
+    o.v_cl = fclamp(o.v_un, P.v_lb, 15.0f);                             // S16
+    if (PH == PH_DDR) {                                                 // S17
+        o.cel = o.v_cl * (5.0f / 3.0f);
+    } else {
+        float beta = ((-(o.area * root)) / (o.tw * o.wp)) * (4.0f / 3.0f) + (5.0f / 3.0f);
+        o.cel = o.v_cl * beta;
+    }
+    float k_raw = length / o.cel;                                       // S18
+    o.k_musk = (PH == PH_POS) ? fmaxr(k_raw, P.dt * (1.0f + kDelta) / 2.0f) : k_raw;
+    if (PH == PH_DDR) {                                                 // S19
+        o.x_eff = xst;
+    } else {
+        float w = q / (((o.tw * slope) * o.cel) * length + 1e-12f);
+        float x_cunge = fclamp(((-w) + 1.0f) * 0.5f, 0.0f, 0.5f);
+        if (PH == PH_POS) {
+            float cr = (1.0f / o.k_musk) * P.dt;
+            float hi_a = cr * (0.5f * (1.0f - kDelta));
+            float hi_b = ((-cr) * 0.5f + 1.0f) * (1.0f - kDelta);
+            o.x_eff = fminr(fminr(x_cunge, hi_a), hi_b);
+        } else {
+            o.x_eff = x_cunge;
+        }
+    }
+    float one_minus_x = (-o.x_eff) + 1.0f;                              // S20..S23
+    float two_k = o.k_musk * 2.0f;
+    float two_kx = two_k * o.x_eff;
+    float two_k_1mx = two_k * one_minus_x;
+    o.denom = two_k_1mx + P.dt;
+    o.c1 = ((-two_kx) + P.dt) / o.denom;
+    o.c2 = (two_kx + P.dt) / o.denom;
+    o.c3 = (two_k_1mx - P.dt) / o.denom;
+    o.c4 = (1.0f / o.denom) * (2.0f * P.dt);
+}
+
+struct LocalGrads {
+    float gn, gqsp, gp, gq, gqp;
+};
+
+// B25..B1 for one reach. gb: this reach's transposed-solve value; gc1: the B26
+// assemble-backward value; i_t: N q_t at this reach; gq24: (N^T gi) at this
+// reach. Geometry is recomputed from the saved primitives exactly as the
+// reference backward does (wp from S13's operands, area = R wp).
+template <int PH>
+__device__ __forceinline__ LocalGrads mc_local_bwd(const Phys& P, const Coef& c, float nman,
+                                                   float psp, float q, float qp, float length,
+                                                   float slope, float sqrt_s, float i_t,
+                                                   float gb, float gc1, float gq24) {
+    LocalGrads g;
+    const float dt = P.dt;
+    // B25
+    float gc2 = gb * i_t;
+    float gc3 = gb * q;
+    float gc4 = gb * qp;
+    float gq25 = c.c3 * gb;
+    g.gqp = c.c4 * gb;
+    // B23..B20
+    float denom_sq = c.denom * c.denom;
+    float one_minus_x = (-c.x_eff) + 1.0f;
+    float two_k = c.k_musk * 2.0f;
+    float two_kx = two_k * c.x_eff;
+    float two_k_1mx = two_k * one_minus_x;
+    float num_c1 = (-two_kx) + dt;
+    float num_c2 = two_kx + dt;
+    float num_c3 = two_k_1mx - dt;
+    float gd1 = ((-gc1) * num_c1) / denom_sq;
+    float gd2 = ((-gc2) * num_c2) / denom_sq;
+    float gd3 = ((-gc3) * num_c3) / denom_sq;
+    float gd4 = ((-gc4) * (2.0f * dt)) / denom_sq;
+    float gdenom_total = ((gd1 + gd2) + gd3) + gd4;
+    float g_2kx_total = ((-gc1) / c.denom) + (gc2 / c.denom);
+    float g_2k1mx_total = (gc3 / c.denom) + gdenom_total;
+    // B19 / B19'
+    float xg_q = 0.0f, xg_tw = 0.0f, xg_c = 0.0f, gk_cap = 0.0f;
+    if (PH != PH_DDR) {
+        float gxx = two_k * (g_2kx_total - g_2k1mx_total);
+        float bscl = ((c.tw * slope) * c.cel) * length + 1e-12f;
+        float w = q / bscl;
+        float half_w = w * 0.5f;
+        float x_for_clamp = c.x_eff;
+        if (PH == PH_POS) {
+            float x_cunge = fclamp(((-w) + 1.0f) * 0.5f, 0.0f, 0.5f);
+            float cr = (1.0f / c.k_musk) * dt;
+            float hi_a = cr * (0.5f * (1.0f - kDelta));
+            float hi_b = ((-cr) * 0.5f + 1.0f) * (1.0f - kDelta);
+            bool m_cunge = (x_cunge <= hi_a) && (x_cunge <= hi_b);
+            bool m_a = (!m_cunge) && (hi_a <= hi_b);
+            bool m_b = (!m_cunge) && (!m_a);
+            float g_hi_a = m_a ? gxx : 0.0f;
+            float g_hi_b = m_b ? gxx : 0.0f;
+            float gx_cunge = m_cunge ? gxx : 0.0f;
+            float gcr = (g_hi_a - g_hi_b) * (0.5f * (1.0f - kDelta));
+            gk_cap = ((-gcr) * dt) / (c.k_musk * c.k_musk);
+            gxx = gx_cunge;
+            x_for_clamp = x_cunge;
+        }
+        bool unsat = (x_for_clamp > 0.0f) && (x_for_clamp < 0.5f);
+        gxx = unsat ? gxx : 0.0f;
+        xg_q = ((-gxx) * 0.5f) / bscl;
+        xg_tw = (gxx * half_w) / c.tw;
+        xg_c = (gxx * half_w) / c.cel;
+    }
+    float g_2k_total = (g_2kx_total * c.x_eff) + (g_2k1mx_total * one_minus_x);
+    float gk = g_2k_total * 2.0f;
+    if (PH == PH_POS) gk = gk + gk_cap;
+    // geometry recomputed from saved primitives
+    float one_plus_ss_sq = c.ss * c.ss + 1.0f;
+    float sq1 = sqrtf(one_plus_ss_sq);
+    float wp = c.bw + (c.depth * sq1) * 2.0f;
+    float area = c.hyd_r * wp;
+    // B18'
+    float gk_raw = gk;
+    if (PH == PH_POS) {
+        float k_raw = length / c.cel;
+        float k_floor = dt * (1.0f + kDelta) / 2.0f;
+        gk_raw = (k_raw > k_floor) ? gk : 0.0f;
+    }
+    // B18
+    float gcel = ((-gk_raw) * length) / (c.cel * c.cel);
+    if (PH != PH_DDR) gcel = gcel + xg_c;
+    // B17
+    float gvcl, bg_area = 0.0f, bg_tw = 0.0f, bg_wp = 0.0f, bg_ss = 0.0f;
+    if (PH == PH_DDR) {
+        gvcl = gcel * (5.0f / 3.0f);
+    } else {
+        float g_term = ((area * sq1) / (c.tw * wp)) * (4.0f / 3.0f);
+        float beta = (-g_term) + (5.0f / 3.0f);
+        float gbeta = gcel * c.v_cl;
+        bg_area = ((-gbeta) * g_term) / area;
+        bg_tw = (gbeta * g_term) / c.tw;
+        bg_wp = (gbeta * g_term) / wp;
+        bg_ss = (((-gbeta) * g_term) * c.ss) / one_plus_ss_sq;
+        gvcl = gcel * beta;
+    }
+    // B16
+    float gvun = ((c.v_un > P.v_lb) && (c.v_un < 15.0f)) ? gvcl : 0.0f;
+    // B15
+    float gn15 = gvun * ((-c.v_un) / nman);
+    float gr = (gvun * (c.v_un * (2.0f / 3.0f))) / c.hyd_r;
+    // B14
+    float garea_r = gr / wp;
+    float gwp_r = ((-gr) * area) / (wp * wp);
+    // B13
+    float gwp = (PH == PH_DDR) ? gwp_r : (gwp_r + bg_wp);
+    float gbw13 = gwp;
+    float gd13 = (gwp * sq1) * 2.0f;
+    float gss13 = (((gwp * c.depth) * 2.0f) * c.ss) / sq1;
+    // B12
+    float garea = (PH == PH_DDR) ? garea_r : (garea_r + bg_area);
+    float half_d = c.depth * 0.5f;
+    float gtw12 = garea * half_d;
+    float gbw12 = garea * half_d;
+    float gd12 = (garea * (c.tw + c.bw)) * 0.5f;
+    // B11
+    float gbw_total = gbw13 + gbw12;
+    float gbw_raw = (c.bw_raw > P.bw_lb) ? gbw_total : 0.0f;
+    // B10
+    float gtw10 = gbw_raw;
+    float gss10 = ((-gbw_raw) * 2.0f) * c.depth;
+    float gd10 = ((-gbw_raw) * 2.0f) * c.ss;
+    // B9
+    float gss_comb = (PH == PH_DDR) ? (gss13 + gss10) : ((gss13 + gss10) + bg_ss);
+    float gss_clamp = ((c.ss_raw > 0.5f) && (c.ss_raw < 50.0f)) ? gss_comb : 0.0f;
+    // B8
+    float two_d = c.depth * 2.0f;
+    float gtw8 = (gss_clamp * c.q_eps) / two_d;
+    float gqeps8 = (gss_clamp * c.tw) / two_d;
+    float gd8 = (((-gss_clamp) * c.tw) * c.q_eps) / (two_d * c.depth);
+    float gtw_total = (gtw12 + gtw10) + gtw8;
+    if (PH != PH_DDR) gtw_total = gtw_total + bg_tw;
+    if (PH != PH_DDR) gtw_total = gtw_total + xg_tw;
+    // B7
+    float gp7 = (gtw_total * c.tw) / psp;
+    float gdepth7 = ((gtw_total * c.tw) * c.q_eps) / c.depth;
+    float gqeps7 = (gtw_total * c.tw) * logf(c.depth);
+    // B6
+    float gd_total = (((gd13 + gd12) + gd10) + gd8) + gdepth7;
+    float gd_pre = (c.depth > P.depth_lb) ? gd_total : 0.0f;
+    float exponent = (1.0f / (c.q_eps * 3.0f + 5.0f)) * 3.0f;
+    float gratio6 = ((gd_pre * exponent) * c.depth) / c.ratio;
+    float gexp6 = (gd_pre * c.depth) * logf(c.ratio);
+    // B5
+    float f53 = c.q_eps * 3.0f + 5.0f;
+    float gqeps5 = ((-gexp6) * 9.0f) / (f53 * f53);
+    // B4
+    float gnum = gratio6 / c.denominator;
+    float gden = ((-gratio6) * c.ratio) / c.denominator;
+    // B3
+    float gp3 = gden * sqrt_s;
+    // B2
+    float qe1 = c.q_eps + 1.0f;
+    float gq2 = (gnum * nman) * qe1;
+    float gn2 = (gnum * q) * qe1;
+    float gqeps2 = (gnum * q) * nman;
+    // B1 and totals (mmc_op.rs:823-835)
+    g.gqsp = ((gqeps8 + gqeps7) + gqeps5) + gqeps2;
+    g.gn = gn15 + gn2;
+    g.gp = gp7 + gp3;
+    float gq = (gq25 + gq24) + gq2;
+    if (PH != PH_DDR) gq = gq + xg_q;
+    g.gq = gq;
+    return g;
+}
+
+}  // namespace ddrs

@@ -1,0 +1,84 @@
+"""Networks and workloads: seeded synthetic generators and the reference's
+index construction (topological order, batch compression), all native."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native as N
+from .sparse import SparseAdjacency, _ptr, _stream
+
+
+def gen_tree(n: int, window: int = 256, seed: int = 1001):
+    """Reach i < n-1 drains to a reach drawn uniformly from (i, min(i+window, n-1)]."""
+    rows = np.zeros(max(n - 1, 0), np.int32)
+    cols = np.zeros(max(n - 1, 0), np.int32)
+    N.check(N.lib.ddrs_gen_tree(n, window, seed, _ptr(rows), _ptr(cols)), "ddrs_gen_tree")
+    return rows, cols
+
+
+def gen_attributes(n: int, seed: int = 2001):
+    out = [np.zeros(n, np.float32) for _ in range(4)]
+    N.check(N.lib.ddrs_gen_attributes(n, seed, *[_ptr(o) for o in out]), "ddrs_gen_attributes")
+    length, slope, nman, qsp = out
+    return dict(length=length, slope=slope, n=nman, q_spatial=qsp)
+
+
+def gen_forcing(n: int, T: int, seed: int = 3001, storms: bool = False) -> np.ndarray:
+    q = np.zeros((T, n), np.float32)
+    N.check(N.lib.ddrs_gen_forcing(n, T, seed, int(storms), _ptr(q)), "ddrs_gen_forcing")
+    return q
+
+
+def gen_forcing_device(n: int, T: int, seed: int = 3001, storms: bool = False,
+                       device="cuda") -> torch.Tensor:
+    q = torch.empty((T, n), dtype=torch.float32, device=device)
+    N.check(N.lib.ddrs_gen_forcing_device(n, T, seed, int(storms), _ptr(q), _stream()),
+            "ddrs_gen_forcing_device")
+    return q
+
+
+def basin_tree(n: int, window: int = 256, seed: int = 1001) -> SparseAdjacency:
+    """A synthetic single-outlet river tree with generated length and slope."""
+    rows, cols = gen_tree(n, window, seed)
+    attrs = gen_attributes(n, seed + 1000)
+    return SparseAdjacency.from_coo(n, rows, cols, None, attrs["length"], attrs["slope"])
+
+
+def build_topology(records: np.ndarray):
+    """records [n, 5] int64 (comid, up1..up4) -> (order, rows, cols, dropped),
+    adjacency/build.rs:170-272."""
+    rec = np.ascontiguousarray(records, np.int64).reshape(-1, 5)
+    n = rec.shape[0]
+    cap = 5 * n + 1
+    order = np.zeros(cap, np.int64)
+    rows = np.zeros(cap, np.int32)
+    cols = np.zeros(cap, np.int32)
+    dropped = np.zeros(cap, np.int64)
+    cnt = [ctypes.c_int32() for _ in range(3)]
+    N.check(N.lib.ddrs_build_topology(n, _ptr(rec), _ptr(order), cap, ctypes.byref(cnt[0]),
+                                      _ptr(rows), _ptr(cols), cap, ctypes.byref(cnt[1]),
+                                      _ptr(dropped), cap, ctypes.byref(cnt[2])),
+            "ddrs_build_topology")
+    no, ne, nd = (c.value for c in cnt)
+    return order[:no], rows[:ne], cols[:ne], dropped[:nd]
+
+
+def compress(rows, cols, gauges):
+    """Unioned CONUS COO + gauge positions -> compressed positions
+    (data/collate.rs:197-349). Returns (active, rows, cols, gauge_compressed)."""
+    rows = np.ascontiguousarray(rows, np.int32)
+    cols = np.ascontiguousarray(cols, np.int32)
+    gauges = np.ascontiguousarray(gauges, np.int32)
+    cap = 2 * rows.shape[0] + gauges.shape[0] + 1
+    active = np.zeros(cap, np.int32)
+    orow = np.zeros(max(rows.shape[0], 1), np.int32)
+    ocol = np.zeros(max(rows.shape[0], 1), np.int32)
+    og = np.zeros(max(gauges.shape[0], 1), np.int32)
+    na, ne = ctypes.c_int32(), ctypes.c_int32()
+    N.check(N.lib.ddrs_compress(rows.shape[0], _ptr(rows), _ptr(cols), gauges.shape[0],
+                                _ptr(gauges), _ptr(active), cap, ctypes.byref(na), _ptr(orow),
+                                _ptr(ocol), ctypes.byref(ne), _ptr(og)), "ddrs_compress")
+    return active[:na.value], orow[:ne.value], ocol[:ne.value], og[:gauges.shape[0]]

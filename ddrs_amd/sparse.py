@@ -1,0 +1,170 @@
+"""The reference's sparse seam on the HIP library.
+
+SparseAdjacency  src/sparse/mod.rs:44-105
+Network          CsrPattern (src/sparse/mod.rs:115-304) + the device plan and
+                 workspace that replace the cuSPARSE pattern cache
+triangular_csr_solve  src/sparse/mod.rs:483-531 with the CsrSolveOp backward
+                 (425-472) as a torch.autograd.Function
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _native as N
+
+
+def _ptr(t) -> Optional[int]:
+    if t is None:
+        return None
+    if isinstance(t, np.ndarray):
+        return t.ctypes.data
+    return t.data_ptr()
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _dev_f32(t: torch.Tensor, n: Optional[int] = None) -> torch.Tensor:
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError("expected a CUDA (HIP) tensor")
+    t = t.detach().to(torch.float32).contiguous()
+    if n is not None and t.numel() != n:
+        raise ValueError(f"expected {n} elements, got {t.numel()}")
+    return t
+
+
+@dataclass
+class SparseAdjacency:
+    """COO of the routing network N: rows = downstream, cols = upstream,
+    rows > cols (topological order), plus per-reach length [m] and slope."""
+
+    n: int
+    rows: np.ndarray
+    cols: np.ndarray
+    values: np.ndarray
+    length_m: np.ndarray
+    slope: np.ndarray
+
+    @classmethod
+    def from_coo(cls, n, rows, cols, values=None, length_m=None, slope=None):
+        rows = np.ascontiguousarray(rows, np.int32)
+        cols = np.ascontiguousarray(cols, np.int32)
+        values = np.ones(rows.shape[0], np.float32) if values is None else np.ascontiguousarray(values, np.float32)
+        length_m = np.ones(n, np.float32) if length_m is None else np.ascontiguousarray(length_m, np.float32)
+        slope = np.ones(n, np.float32) if slope is None else np.ascontiguousarray(slope, np.float32)
+        return cls(int(n), rows, cols, values, length_m, slope)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.rows.shape[0])
+
+
+class Network:
+    """A routing network on the GPU: the bit-exact CsrPattern on the host plus
+    the level-major device schedule (ddrs_plan_create)."""
+
+    def __init__(self, adj: SparseAdjacency):
+        self.adj = adj
+        h = ctypes.c_void_p()
+        N.check(N.lib.ddrs_plan_create(adj.n, adj.nnz, _ptr(adj.rows), _ptr(adj.cols),
+                                       _ptr(adj.values), ctypes.byref(h)), "ddrs_plan_create")
+        self.handle = h
+        info = [ctypes.c_int32() for _ in range(4)]
+        N.check(N.lib.ddrs_plan_info(h, *[ctypes.byref(x) for x in info]))
+        self.n, self.nnz, self.n_chunks, self.n_levels = (int(x.value) for x in info)
+        self._ws = {}
+
+    def __del__(self):
+        for ws in getattr(self, "_ws", {}).values():
+            ws.close()
+        if getattr(self, "handle", None):
+            N.lib.ddrs_plan_destroy(self.handle)
+            self.handle = None
+
+    def csr(self) -> dict:
+        n, nnz = self.n, self.nnz
+        out = dict(crow=np.zeros(n + 1, np.int32), col=np.zeros(nnz, np.int32),
+                   row_for_nnz=np.zeros(nnz, np.int32), adj_values=np.zeros(nnz, np.float32),
+                   diag_mask=np.zeros(nnz, np.float32), trans_crow=np.zeros(n + 1, np.int32),
+                   trans_col=np.zeros(nnz, np.int32), trans_to_orig=np.zeros(nnz, np.int32))
+        N.check(N.lib.ddrs_plan_get_csr(self.handle, *[_ptr(v) for v in out.values()]))
+        return out
+
+    def schedule(self):
+        perm = np.zeros(self.n, np.int32)
+        level = np.zeros(self.n, np.int32)
+        N.check(N.lib.ddrs_plan_get_schedule(self.handle, _ptr(perm), _ptr(level)))
+        return perm, level
+
+    def workspace(self, window: int = 1, max_steps: int = 1) -> "Workspace":
+        key = (int(window), int(max_steps))
+        for (w, m), ws in self._ws.items():
+            if w >= key[0] and m >= key[1]:
+                return ws
+        ws = Workspace(self, *key)
+        self._ws[key] = ws
+        return ws
+
+
+class Workspace:
+    def __init__(self, net: Network, window: int, max_steps: int):
+        self.net = net
+        self.window = window
+        self.max_steps = max_steps
+        h = ctypes.c_void_p()
+        N.check(N.lib.ddrs_workspace_create(net.handle, window, max_steps, ctypes.byref(h)),
+                "ddrs_workspace_create")
+        self.handle = h
+        self.fwd_token = None
+
+    def close(self):
+        if self.handle:
+            N.lib.ddrs_workspace_destroy(self.handle)
+            self.handle = None
+
+    def nbytes(self) -> int:
+        b = ctypes.c_size_t()
+        N.check(N.lib.ddrs_workspace_bytes(self.handle, ctypes.byref(b)))
+        return int(b.value)
+
+    def check(self):
+        N.check(N.lib.ddrs_workspace_check(self.handle, _stream()), "ddrs_workspace_check")
+
+
+class _TriSolve(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, a_values, b, net):
+        ws = net.workspace()
+        a = _dev_f32(a_values, net.nnz)
+        bb = _dev_f32(b, net.n)
+        x = torch.empty(net.n, dtype=torch.float32, device=bb.device)
+        N.check(N.lib.ddrs_tri_solve_fwd(ws.handle, _ptr(a), _ptr(bb), _ptr(x), _stream()),
+                "ddrs_tri_solve_fwd")
+        ws.check()
+        ctx.net = net
+        ctx.save_for_backward(a, x)
+        return x
+
+    @staticmethod
+    def backward(ctx, gx):
+        a, x = ctx.saved_tensors
+        net = ctx.net
+        ws = net.workspace()
+        g = _dev_f32(gx, net.n)
+        gb = torch.empty_like(g)
+        ga = torch.empty(net.nnz, dtype=torch.float32, device=g.device) if ctx.needs_input_grad[0] else None
+        N.check(N.lib.ddrs_tri_solve_adj(ws.handle, _ptr(a), _ptr(x), _ptr(g), _ptr(gb), _ptr(ga),
+                                         _stream()), "ddrs_tri_solve_adj")
+        ws.check()
+        return ga, gb, None
+
+
+def triangular_csr_solve(net: Network, a_values: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    """Solve A x = b for A in CsrPattern order (sparse/mod.rs:483-531)."""
+    return _TriSolve.apply(a_values, b, net)

@@ -1,0 +1,241 @@
+/*
+ * ddrs_hip.h — C ABI of the MI355X (gfx950) Muskingum-Cunge routing library.
+ *
+ * Drop-in boundary for the reference ddrs path (taddyb/ddrs):
+ *   - the BURN custom op TimestepOp : Backward<I,5> (src/routing/mmc_op.rs:850-893,
+ *     timestep_forward 1704-1857), parents [n, q_spatial, p_spatial, q_t, q'_t];
+ *   - the src/sparse seam it calls (src/sparse/dispatch.rs:57-244;
+ *     triangular_csr_solve src/sparse/mod.rs:483-531, CsrSolveOp 425-472);
+ *   - the time loop of MuskingumCunge::forward (src/routing/mmc.rs:519-632) and the
+ *     reverse walk its autograd tape performs, as one window forward + adjoint.
+ *
+ * Conventions
+ *   - Plain pointers and sizes only. Pointers named d_* are device pointers owned
+ *     by the caller; pointers named h_* are host pointers.
+ *   - Per-reach vectors are in the caller's (reference) reach order, length R.
+ *     Time series are time-major [T][R] (row t holds all reaches at step t).
+ *   - Every call that launches work takes a hipStream_t (passed as void*; NULL is
+ *     the default stream) and returns ddrs_status; on failure ddrs_last_error()
+ *     describes it (thread-local).
+ *   - Nothing is allocated inside a step: workspaces are created up front.
+ *   - Thread-compatible: one stream per workspace at a time.
+ *   - All arithmetic is fp32, indices int32.
+ */
+#ifndef DDRS_HIP_H
+#define DDRS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    DDRS_OK = 0,
+    DDRS_ERR_INVALID = 1,   /* bad argument / malformed adjacency */
+    DDRS_ERR_HIP = 2,       /* HIP runtime failure */
+    DDRS_ERR_CAPACITY = 3,  /* a size limit of this build was exceeded */
+    DDRS_ERR_NOMEM = 4
+} ddrs_status;
+
+typedef struct ddrs_plan* ddrs_plan_t;
+typedef struct ddrs_workspace* ddrs_workspace_t;
+
+/* Scalar routing parameters (reference: config.rs AttributeMinimums 426-446,
+ * Params 580-672, mmc.rs DT_SECONDS 33). */
+typedef struct {
+    float dt;               /* 3600 */
+    float discharge_lb;     /* 1e-4 */
+    float depth_lb;         /* 0.01 */
+    float velocity_lb;      /* 0.01 */
+    float bottom_width_lb;  /* 0.01 */
+    float slope_lb;         /* 1e-3, applied by ddrs_mc_bind_statics (mmc.rs:259-264) */
+    int32_t ddr_match;          /* 0 (default physics) or 1 (legacy 5/3 celerity, X = x_storage) */
+    int32_t enforce_positivity; /* S18'/S19' clamp; ignored when ddr_match */
+} ddrs_params;
+
+/* Execution form of a window (multi-step) call. */
+typedef enum {
+    DDRS_FORM_PLAIN = 0,      /* one kernel launch per timestep */
+    DDRS_FORM_PERSISTENT = 1  /* one persistent, time-skewed launch per window */
+} ddrs_form;
+
+const char* ddrs_last_error(void);
+const char* ddrs_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Plan: the network. Replaces CsrPattern::from_sparse + build_transposed_pattern
+ * (sparse/mod.rs:173-304) and the cuSPARSE pattern cache
+ * (sparse/cusparse.rs:1141-1850). COO with rows = downstream, cols = upstream,
+ * rows > cols (SparseAdjacency, sparse/mod.rs:44-105), host arrays. Builds the
+ * bit-exact CSR + transposed view on the host and a level-major device schedule.
+ * ------------------------------------------------------------------------- */
+ddrs_status ddrs_plan_create(int32_t n, int32_t nnz_off, const int32_t* h_rows,
+                             const int32_t* h_cols, const float* h_vals,
+                             ddrs_plan_t* out);
+void ddrs_plan_destroy(ddrs_plan_t plan);
+/* n, nnz (= nnz_off + n), number of 64-reach chunks, number of levels. */
+ddrs_status ddrs_plan_info(ddrs_plan_t plan, int32_t* n, int32_t* nnz, int32_t* n_chunks,
+                           int32_t* n_levels);
+/* Copy the CsrPattern arrays (sparse/mod.rs:115-146) to host buffers sized
+ * n+1 / nnz. Any pointer may be NULL. */
+ddrs_status ddrs_plan_get_csr(ddrs_plan_t plan, int32_t* h_crow, int32_t* h_col,
+                              int32_t* h_row_for_nnz, float* h_adj_values,
+                              float* h_diag_mask, int32_t* h_trans_crow,
+                              int32_t* h_trans_col, int32_t* h_trans_to_orig);
+/* Copy the schedule: h_perm[p] = reach at schedule position p, h_level[i] = level
+ * (longest upstream path) of reach i. */
+ddrs_status ddrs_plan_get_schedule(ddrs_plan_t plan, int32_t* h_perm, int32_t* h_level);
+
+/* ---------------------------------------------------------------------------
+ * Workspace: device state for windows of up to `window` timesteps (the
+ * activation-checkpoint interval k: forward history, adjoint messages and
+ * checkpoints are sized for it). max_steps bounds the total steps T-1 of one
+ * window call (checkpoint storage). 288 GB MI355X: C2 (350k x 720) with
+ * window 720 uses about 6.1 GB.
+ * ------------------------------------------------------------------------- */
+ddrs_status ddrs_workspace_create(ddrs_plan_t plan, int32_t window, int32_t max_steps,
+                                  ddrs_workspace_t* out);
+void ddrs_workspace_destroy(ddrs_workspace_t ws);
+/* Bytes of device memory the workspace holds. */
+ddrs_status ddrs_workspace_bytes(ddrs_workspace_t ws, size_t* bytes);
+/* Synchronise `stream` and report whether a persistent kernel of this
+ * workspace stopped on its progress watchdog since the last check (the
+ * replacement for the reference's expect()-panics on solver failure,
+ * sparse/cusparse.rs). */
+ddrs_status ddrs_workspace_check(ddrs_workspace_t ws, void* stream);
+
+/* Bind the per-reach static inputs of every following step/window call
+ * (device pointers, reference order, length n): Manning n, q_spatial,
+ * p_spatial (physical units, i.e. after denormalize, routing/utils.rs:32-41),
+ * channel length [m], raw slope (clamped here to params.slope_lb) and
+ * x_storage (read only when ddr_match). Copies into schedule order. */
+ddrs_status ddrs_mc_bind_statics(ddrs_workspace_t ws, const ddrs_params* params,
+                                 const float* d_n, const float* d_q_spatial,
+                                 const float* d_p_spatial, const float* d_length,
+                                 const float* d_slope, const float* d_x_storage,
+                                 void* stream);
+
+/* ---------------------------------------------------------------------------
+ * One timestep — the TimestepOp forward (timestep_forward, mmc_op.rs:1704-1857;
+ * forward_chain_inner S1..S28, 1061-1336) with the statics bound above.
+ * q'_t must already be clamped by the caller (mmc.rs:526). Writes
+ * q_next = max(x_sol, discharge_lb) and, if d_x_sol != NULL, the pre-clamp solve.
+ * ------------------------------------------------------------------------- */
+ddrs_status ddrs_mc_step_fwd(ddrs_workspace_t ws, const ddrs_params* params,
+                             const float* d_q_t, const float* d_q_prime_t, float* d_q_next,
+                             float* d_x_sol, void* stream);
+/* TimestepOp::backward (mmc_op.rs:850-893 / timestep_backward_core 219-848):
+ * from the step's inputs, its pre-clamp solve and dL/dQ_{t+1}, writes the five
+ * parent gradients (any output may be NULL). Intermediates are recomputed, not
+ * saved. */
+ddrs_status ddrs_mc_step_bwd(ddrs_workspace_t ws, const ddrs_params* params,
+                             const float* d_q_t, const float* d_q_prime_t,
+                             const float* d_x_sol, const float* d_grad_q_next,
+                             float* d_grad_n, float* d_grad_q_spatial,
+                             float* d_grad_p_spatial, float* d_grad_q_t,
+                             float* d_grad_q_prime_t, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Window — MuskingumCunge::forward's time loop (mmc.rs:519-632) after
+ * setup_inputs: Q[0] = max(Q0, lb); Q[t] = step(Q[t-1], max(q'[t-1], lb)) for
+ * t = 1..T-1 (the last forcing row is unused, as in the reference).
+ *   d_q_prime  [T][n] raw lateral inflow
+ *   d_q0       [n]    initial state (e.g. from ddrs_hotstart)
+ *   d_Q_out    [T][n] routed discharge (NULL: not written)
+ * Checkpoints Q every `window` steps; keeps the last window's solve history for
+ * the adjoint.
+ * ------------------------------------------------------------------------- */
+ddrs_status ddrs_mc_window_fwd(ddrs_workspace_t ws, const ddrs_params* params, int32_t T,
+                               const float* d_q_prime, const float* d_q0, float* d_Q_out,
+                               ddrs_form form, void* stream);
+/* Reverse-time adjoint of the last ddrs_mc_window_fwd for the loss whose
+ * gradient with respect to Q is d_grad_Q [T][n] (NULL: zero). Recomputes each
+ * earlier checkpoint segment's forward before its adjoint. Writes
+ *   d_grad_n, d_grad_q_spatial, d_grad_p_spatial [n]  (summed over all steps)
+ *   d_grad_q_prime [T][n]  gradient w.r.t. the raw q' through its clamp (NULL: skipped)
+ *   d_grad_q0 [n]          dL/dQ0 through the initial clamp (NULL: skipped)
+ * Any output may be NULL. */
+ddrs_status ddrs_mc_window_adj(ddrs_workspace_t ws, const ddrs_params* params, int32_t T,
+                               const float* d_q_prime, const float* d_q0,
+                               const float* d_grad_Q, float* d_grad_n,
+                               float* d_grad_q_spatial, float* d_grad_p_spatial,
+                               float* d_grad_q_prime, float* d_grad_q0, ddrs_form form,
+                               void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Sparse seam: triangular_csr_solve (sparse/mod.rs:483-531) forward and the
+ * CsrSolveOp backward (425-472; dispatch.rs:102-138, 214-244) for arbitrary
+ * A values in CsrPattern order (nnz).
+ * ------------------------------------------------------------------------- */
+ddrs_status ddrs_tri_solve_fwd(ddrs_workspace_t ws, const float* d_a_values, const float* d_b,
+                               float* d_x, void* stream);
+/* grad_b = A^-T grad_x; grad_a[k] = -grad_b[row k] * x[col k] (NULL: skipped). */
+ddrs_status ddrs_tri_solve_adj(ddrs_workspace_t ws, const float* d_a_values, const float* d_x,
+                               const float* d_grad_x, float* d_grad_b, float* d_grad_a,
+                               void* stream);
+/* Cold start (mmc.rs:326-361): Q0 = max(solve(I - N, q'_0), discharge_lb). */
+ddrs_status ddrs_hotstart(ddrs_workspace_t ws, const ddrs_params* params,
+                          const float* d_q_prime0, float* d_q0, void* stream);
+/* Its backward (the CsrSolveOp of the cold-start solve through the clamp):
+ * grad_q'_0 = (I - N)^-T (grad_Q0 masked where Q0 <= discharge_lb). */
+ddrs_status ddrs_hotstart_adj(ddrs_workspace_t ws, const ddrs_params* params,
+                              const float* d_q0, const float* d_grad_q0,
+                              float* d_grad_q_prime0, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Timing of the dominant kernels of the last window call on its own stream
+ * (HIP events around each launch): number of launches and summed milliseconds
+ * of the forward-window and adjoint-window kernels. Enabled by
+ * ddrs_set_timing(ws, 1).
+ * ------------------------------------------------------------------------- */
+ddrs_status ddrs_set_timing(ddrs_workspace_t ws, int32_t enable);
+ddrs_status ddrs_get_timing(ddrs_workspace_t ws, int32_t* fwd_launches, float* fwd_ms,
+                            int32_t* adj_launches, float* adj_ms);
+
+/* ---------------------------------------------------------------------------
+ * Index construction (host, bit-exact): the reference's topological order
+ * (adjacency/build.rs:170-272; rustworkx/petgraph DFS finish-time rule) and
+ * per-batch compression (data/collate.rs:197-349).
+ * ------------------------------------------------------------------------- */
+/* records: n_records rows of (comid, up1, up2, up3, up4), zero = no upstream.
+ * h_order gets the COMIDs in topological order (capacity order_cap), h_rows /
+ * h_cols the lower-triangular COO (capacity edge_cap). Counts are returned. */
+ddrs_status ddrs_build_topology(int32_t n_records, const int64_t* h_records,
+                                int64_t* h_order, int32_t order_cap, int32_t* n_order,
+                                int32_t* h_rows, int32_t* h_cols, int32_t edge_cap,
+                                int32_t* n_edges, int64_t* h_dropped, int32_t dropped_cap,
+                                int32_t* n_dropped);
+/* Unioned COO (CONUS positions) + gauge positions -> compressed positions.
+ * h_active (capacity active_cap) gets the sorted active set; h_rows/h_cols the
+ * compressed edges (n_edges entries, in sorted (row, col) order after dedup);
+ * h_gauge the compressed gauge positions. */
+ddrs_status ddrs_compress(int32_t n_edges_in, const int32_t* h_rows_in,
+                          const int32_t* h_cols_in, int32_t n_gauges,
+                          const int32_t* h_gauges, int32_t* h_active, int32_t active_cap,
+                          int32_t* n_active, int32_t* h_rows, int32_t* h_cols,
+                          int32_t* n_edges, int32_t* h_gauge);
+
+/* ---------------------------------------------------------------------------
+ * Synthetic workloads (host), seeded and portable.
+ * ------------------------------------------------------------------------- */
+/* Reach i < n-1 drains to a reach drawn uniformly from (i, min(i+window, n-1)];
+ * writes n-1 edges (rows = downstream, cols = upstream). */
+ddrs_status ddrs_gen_tree(int32_t n, int32_t window, uint64_t seed, int32_t* h_rows,
+                          int32_t* h_cols);
+/* length U[1,10] km (m), slope log-U[1e-4, 5e-2], n U[0.015, 0.25], q_spatial U[0,1]. */
+ddrs_status ddrs_gen_attributes(int32_t n, uint64_t seed, float* h_length, float* h_slope,
+                                float* h_n, float* h_q_spatial);
+/* q'[T][n] ~ LogNormal(ln 0.5, 1.0); if storms != 0, times (1 + m_t) where m_t
+ * is a sum of Poisson(1/72 h) Gamma(2, 2) pulses decaying with e^{-1/24} per h.
+ * Device variant writes a device buffer on `stream`; host variant a host buffer. */
+ddrs_status ddrs_gen_forcing(int32_t n, int32_t T, uint64_t seed, int32_t storms,
+                             float* h_q_prime);
+ddrs_status ddrs_gen_forcing_device(int32_t n, int32_t T, uint64_t seed, int32_t storms,
+                                    float* d_q_prime, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

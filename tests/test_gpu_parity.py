@@ -1,0 +1,398 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle, golden
+fixtures and FP64 finite differences. Needs an MI355X.
+
+Tolerances (stated per test): routed discharge within 1e-4 max relative error
+of the f32 oracle; gradients within 1e-3 relative (rtol 1e-3 with an absolute
+floor of 1e-5 x the vector's max magnitude) of the f32 oracle; every result
+also reports its error against the FP64 oracle; integer/index work bit-exact.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tests.nets import (GOLDEN_X_SOL_BITS, POS_LENGTH, POS_QT, chain, mock_params,
+                        random_tree, stress_params)
+
+pytestmark = pytest.mark.gpu
+
+if not torch.cuda.is_available():  # collected on the CPU too, skipped there
+    pytest.skip("needs a GPU", allow_module_level=True)
+
+import ddrs_amd as D  # noqa: E402
+from ddrs_amd import _native as NAT  # noqa: E402
+from ddrs_amd import network as NW  # noqa: E402
+from ddrs_amd.sparse import Network, SparseAdjacency  # noqa: E402
+
+DEV = "cuda:0"
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def to_params(P: O.OrcParams) -> D.Params:
+    p = D.Params()
+    m = p.attribute_minimums
+    m.discharge, m.slope, m.velocity = P.discharge_lb, P.slope_lb, P.velocity_lb
+    m.depth, m.bottom_width = P.depth_lb, P.bottom_width_lb
+    p.ddr_match, p.enforce_positivity, p.dt = bool(P.ddr_match), bool(P.enforce_positivity), P.dt
+    return p
+
+
+def t(x):
+    return torch.as_tensor(np.ascontiguousarray(x, np.float32), device=DEV)
+
+
+def relerr(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-30)))
+
+
+def grad_close(name, got, want, rtol=1e-3):
+    got = np.asarray(got, np.float64)
+    want = np.asarray(want, np.float64)
+    scale = max(float(np.abs(want).max()), 1e-30)
+    np.testing.assert_allclose(got, want, rtol=rtol, atol=1e-5 * scale, err_msg=name)
+
+
+class Case:
+    """A seeded routing case: network, statics, forcing, parameters."""
+
+    def __init__(self, R, T, seed, window=256, storms=False, params=None, rows=None, cols=None,
+                 vals=None):
+        if rows is None:
+            rows, cols = NW.gen_tree(R, window, 1000 + seed)
+        self.R, self.T = R, T
+        self.rows, self.cols = rows, cols
+        self.vals = np.ones(rows.shape[0], np.float32) if vals is None else vals
+        at = NW.gen_attributes(R, 2000 + seed)
+        self.length, self.slope_raw = at["length"], at["slope"]
+        self.n, self.qsp = at["n"], at["q_spatial"]
+        self.p = np.full(R, 21.0, np.float32)
+        self.qprime = NW.gen_forcing(R, T, 3000 + seed, storms)
+        self.P = params if params is not None else O.make_params()
+        self.pat = O.pattern_from_coo(R, rows, cols, self.vals)
+        self.net = Network(SparseAdjacency.from_coo(R, rows, cols, self.vals, self.length,
+                                                     self.slope_raw))
+
+    def q0(self):
+        return O.hotstart(self.P, self.pat, self.qprime[0])
+
+    def oracle_Q(self, dtype=np.float32):
+        return O.route_fwd(self.P, self.pat, self.n, self.qsp, self.p, self.length,
+                           self.slope_raw, self.qprime, q0=self.q0(), dtype=dtype)
+
+    def gpu_window(self, form, window=None, gQ=None, need_grad=True):
+        params = to_params(self.P)
+        n, qsp, p = (t(v).requires_grad_(need_grad) for v in (self.n, self.qsp, self.p))
+        qp = t(self.qprime).requires_grad_(need_grad)
+        q0 = t(self.q0()).requires_grad_(need_grad)
+        Q = D.route_window(self.net, params, n, qsp, p, qp, q0, t(self.length),
+                           t(self.slope_raw), None, form=form, window=window)
+        out = {"Q": Q.detach().cpu().numpy()}
+        if need_grad:
+            g = t(gQ) if gQ is not None else torch.ones_like(Q)
+            Q.backward(g)
+            out.update(n=n.grad.cpu().numpy(), q_spatial=qsp.grad.cpu().numpy(),
+                       p_spatial=p.grad.cpu().numpy(), q_prime=qp.grad.cpu().numpy(),
+                       q0=q0.grad.cpu().numpy())
+        torch.cuda.synchronize()
+        return out
+
+
+# ---------------------------------------------------------------------------
+
+
+def test_hotstart_fixture_gpu(golden_dir):
+    """tests/training_step_layer_b.rs:242-350 on the GPU (within 1e-5)."""
+    say("hotstart npz 77006074")
+    sg = np.load(os.path.join(golden_dir, "subgraph_77006074.npz"), allow_pickle=False)
+    hs = np.load(os.path.join(golden_dir, "hotstart_77006074.npz"), allow_pickle=False)
+    n = hs["hotstart"].shape[0]
+    net = Network(SparseAdjacency.from_coo(n, sg["rows"], sg["cols"], sg["vals"]))
+    q0 = D.hotstart(net, D.Params(), t(hs["q_prime_t0"])).cpu().numpy()
+    np.testing.assert_allclose(q0, hs["hotstart"], rtol=1e-5, atol=1e-5)
+    pat = O.pattern_from_coo(n, sg["rows"], sg["cols"], sg["vals"])
+    np.testing.assert_array_equal(q0, O.hotstart(O.make_params(), pat, hs["q_prime_t0"]))
+
+
+def test_step_golden_x_sol():
+    """The 10-reach stress fixture (tests/positivity_clamp.rs:59-70): the GPU
+    step's raw solve against GOLDEN_X_SOL_BITS within 1e-5 relative; reports how
+    many of the 10 values are bit-identical."""
+    say("golden x_sol 10-reach chain")
+    R = 10
+    rows, cols = chain(R)
+    net = Network(SparseAdjacency.from_coo(R, rows, cols))
+    params = to_params(stress_params(O, False))
+    q_next = D.timestep_forward(params, net, t(np.full(R, 0.035)), t(np.full(R, 0.4)),
+                                t(np.full(R, 20.0)), t(POS_QT), t(np.full(R, 0.01)),
+                                t(POS_LENGTH), t(np.full(R, 0.001)))
+    ws = net.workspace()
+    x = torch.empty(R, dtype=torch.float32, device=DEV)
+    qn = torch.empty_like(x)
+    qt_d = t(POS_QT)
+    qp_d = t(np.full(R, 0.01))
+    NAT.check(NAT.lib.ddrs_mc_step_fwd(ws.handle, params.to_c(), qt_d.data_ptr(),
+                                       qp_d.data_ptr(), qn.data_ptr(), x.data_ptr(),
+                                       torch.cuda.current_stream().cuda_stream))
+    ws.check()
+    xs = x.cpu().numpy()
+    gold = np.array(GOLDEN_X_SOL_BITS, np.uint32).view(np.float32)
+    exact = int((xs.view(np.uint32) == gold.view(np.uint32)).sum())
+    print(f"golden x_sol: {exact}/10 bit-identical, max rel err {relerr(xs, gold):.3e}")
+    np.testing.assert_allclose(xs, gold, rtol=1e-5)
+    np.testing.assert_allclose(q_next.cpu().numpy(), np.maximum(gold, 1e-4), rtol=1e-5)
+
+
+@pytest.mark.parametrize("variant", ["default", "ddr_match", "positivity"])
+def test_step_fwd_bwd_vs_oracle(variant):
+    """One TimestepOp forward + backward on a 3000-reach tree against the oracle
+    (discharge 1e-4 rel, gradients 1e-3 rel)."""
+    say("step", variant)
+    rng = np.random.default_rng(11)
+    R = 3000
+    rows, cols = random_tree(R, 128, rng)
+    P = O.make_params(ddr_match=variant == "ddr_match",
+                      enforce_positivity=variant == "positivity")
+    params = to_params(P)
+    net = Network(SparseAdjacency.from_coo(R, rows, cols))
+    pat = O.pattern_from_coo(R, rows, cols)
+    n = rng.uniform(0.015, 0.25, R).astype(np.float32)
+    qsp = rng.uniform(0, 1, R).astype(np.float32)
+    p = np.full(R, 21.0, np.float32)
+    q = rng.lognormal(np.log(5.0), 1.5, R).astype(np.float32)
+    qp = np.maximum(rng.lognormal(np.log(0.5), 1.0, R), 1e-4).astype(np.float32)
+    L = rng.uniform(1000, 10000, R).astype(np.float32)
+    S = np.maximum(np.exp(rng.uniform(np.log(1e-4), np.log(5e-2), R)), 1e-3).astype(np.float32)
+    xst = np.full(R, 0.3, np.float32)
+    ins = [t(v).requires_grad_(True) for v in (n, qsp, p, q, qp)]
+    out = D.timestep_forward(params, net, *ins, t(L), t(S), t(xst))
+    w = rng.uniform(0.5, 1.5, R).astype(np.float32)
+    out.backward(t(w))
+    qn_ref, saved, a = O.step_fwd(P, pat, n, qsp, p, q, qp, L, S, xst)
+    qn64, saved64, a64 = O.step_fwd(P, pat, n, qsp, p, q, qp, L, S, xst, dtype=np.float64)
+    got = out.detach().cpu().numpy()
+    print(f"step fwd rel err vs f32 oracle {relerr(got, qn_ref):.3e}, vs f64 {relerr(got, qn64):.3e}")
+    assert relerr(got, qn_ref) <= 1e-4
+    g = O.step_bwd(P, pat, n, p, q, qp, L, S, saved, a, w)
+    g64 = O.step_bwd(P, pat, n, p, q, qp, L, S, saved64, a64, w, dtype=np.float64)
+    for name, tin in zip(["n", "q_spatial", "p_spatial", "q_t", "q_prime_t"], ins):
+        gg = tin.grad.cpu().numpy()
+        e32 = np.abs(gg - g[name]).max() / max(np.abs(g[name]).max(), 1e-30)
+        e64 = np.abs(gg - g64[name]).max() / max(np.abs(g64[name]).max(), 1e-30)
+        print(f"  grad {name}: max err / max|g| vs f32 {e32:.3e}, vs f64 {e64:.3e}")
+        grad_close(name, gg, g[name])
+
+
+def test_window_c1_forward_vs_oracle():
+    """C1: 5,000-reach tree (window 256), T = 240, lognormal q'. Discharge within
+    1e-4 max relative error of the f32 oracle, both forms bit-identical."""
+    say("window C1 5000 x 240")
+    c = Case(5000, 240, seed=1)
+    ref = c.oracle_Q()
+    ref64 = c.oracle_Q(np.float64)
+    pers = c.gpu_window(D.FORM_PERSISTENT, need_grad=False)["Q"]
+    plain = c.gpu_window(D.FORM_PLAIN, need_grad=False)["Q"]
+    e32, e64 = relerr(pers, ref), relerr(pers, ref64)
+    print(f"C1 discharge max rel err vs f32 oracle {e32:.3e}, vs f64 oracle {e64:.3e}; "
+          f"oracle f32 vs f64 {relerr(ref, ref64):.3e}")
+    assert e32 <= 1e-4
+    np.testing.assert_array_equal(pers, plain)
+
+
+def test_window_adjoint_vs_oracle():
+    """Forward + reverse-time adjoint on a 2,000-reach tree over 48 steps with
+    storm forcing and a random dL/dQ: gradients w.r.t. n, q_spatial, p_spatial,
+    q' and Q0 within 1e-3 of the f32 oracle."""
+    say("window adjoint 2000 x 48")
+    c = Case(2000, 48, seed=2, storms=True)
+    rng = np.random.default_rng(3)
+    gQ = rng.uniform(-1, 1, (c.T, c.R)).astype(np.float32)
+    got = c.gpu_window(D.FORM_PERSISTENT, gQ=gQ)
+    q0 = c.q0()
+    ref = O.route_adj(c.P, c.pat, c.n, c.qsp, c.p, c.length, c.slope_raw, c.qprime, gQ, q0=q0)
+    ref64 = O.route_adj(c.P, c.pat, c.n, c.qsp, c.p, c.length, c.slope_raw, c.qprime, gQ, q0=q0,
+                        dtype=np.float64)
+    for name in ["n", "q_spatial", "p_spatial", "q_prime", "q0"]:
+        a, b, b64 = got[name], ref[name], ref64[name]
+        s = max(np.abs(b).max(), 1e-30)
+        print(f"  grad {name}: max err/max|g| vs f32 {np.abs(a - b).max() / s:.3e}, "
+              f"vs f64 {np.abs(a - b64).max() / max(np.abs(b64).max(), 1e-30):.3e}")
+        grad_close(name, a, b)
+
+
+def test_plain_persistent_and_checkpointed_are_identical():
+    """The plain per-step form, the persistent form and a checkpointed adjoint
+    (window 7 < T-1, forward recomputed per segment) give identical bits."""
+    say("forms identical 1500 x 30")
+    c = Case(1500, 30, seed=4, storms=True)
+    gQ = np.random.default_rng(5).uniform(0, 1, (c.T, c.R)).astype(np.float32)
+    a = c.gpu_window(D.FORM_PERSISTENT, gQ=gQ)
+    b = c.gpu_window(D.FORM_PLAIN, gQ=gQ)
+    k = c.gpu_window(D.FORM_PERSISTENT, window=7, gQ=gQ)
+    for name in a:
+        np.testing.assert_array_equal(a[name], b[name], err_msg=f"plain {name}")
+        np.testing.assert_array_equal(a[name], k[name], err_msg=f"checkpointed {name}")
+
+
+def test_gradients_vs_fp64_finite_differences():
+    """GPU adjoint against FP64 central differences of the double oracle on
+    seeded reaches, for loss = sum(w * Q), within 1e-3 relative."""
+    say("fd fp64 600 x 24")
+    c = Case(600, 24, seed=6)
+    w = np.random.default_rng(7).uniform(0.5, 1.5, (c.T, c.R))
+    got = c.gpu_window(D.FORM_PERSISTENT, gQ=w)
+    q0 = c.q0().astype(np.float64)
+    base = dict(n=c.n.astype(np.float64), qsp=c.qsp.astype(np.float64), p=c.p.astype(np.float64))
+
+    def loss(v):
+        Q = O.route_fwd(c.P, c.pat, v["n"], v["qsp"], v["p"], c.length, c.slope_raw, c.qprime,
+                        q0=q0, dtype=np.float64)
+        return float((w * Q).sum())
+
+    rng = np.random.default_rng(8)
+    reaches = rng.choice(c.R, 16, replace=False)
+    for key, gkey in [("n", "n"), ("qsp", "q_spatial")]:
+        for i in reaches:
+            vp = {k: v.copy() for k, v in base.items()}
+            vm = {k: v.copy() for k, v in base.items()}
+            h = 1e-6 * max(abs(base[key][i]), 1e-3)
+            vp[key][i] += h
+            vm[key][i] -= h
+            fd = (loss(vp) - loss(vm)) / (2 * h)
+            g = got[gkey][i]
+            scale = max(np.abs(got[gkey]).max(), 1e-30)
+            assert abs(g - fd) <= 1e-3 * abs(fd) + 1e-5 * scale, (key, i, g, fd)
+
+
+@pytest.mark.parametrize("kind", ["forest", "long_chain", "weights", "star", "ragged"])
+def test_structures(kind):
+    """Edge structures: a forest of uneven basins, a 3,000-reach chain (one
+    sub-level per reach inside every chunk), non-unit edge weights, a reach
+    with 40 upstream reaches, and sizes that leave a ragged last chunk."""
+    say("structure", kind)
+    rng = np.random.default_rng(12)
+    vals = None
+    if kind == "forest":
+        rows, cols, off = [], [], 0
+        for size in [1, 2, 17, 300, 64, 65, 900, 5]:
+            r, cl = random_tree(size, 16, rng) if size > 1 else (np.zeros(0, np.int32),) * 2
+            rows.append(r + off)
+            cols.append(cl + off)
+            off += size
+        R = off
+        rows, cols = np.concatenate(rows).astype(np.int32), np.concatenate(cols).astype(np.int32)
+    elif kind == "long_chain":
+        R = 3000
+        rows, cols = chain(R)
+    elif kind == "weights":
+        R = 1200
+        rows, cols = random_tree(R, 32, rng)
+        vals = rng.uniform(0.5, 1.5, rows.shape[0]).astype(np.float32)
+    elif kind == "star":
+        R = 41
+        rows, cols = np.full(40, 40, np.int32), np.arange(40, dtype=np.int32)
+    else:
+        R = 129
+        rows, cols = random_tree(R, 8, rng)
+    c = Case(R, 20, seed=13, rows=rows, cols=cols, vals=vals)
+    gQ = rng.uniform(0, 1, (c.T, c.R)).astype(np.float32)
+    got = c.gpu_window(D.FORM_PERSISTENT, gQ=gQ)
+    ref = c.oracle_Q()
+    assert relerr(got["Q"], ref) <= 1e-4
+    radj = O.route_adj(c.P, c.pat, c.n, c.qsp, c.p, c.length, c.slope_raw, c.qprime, gQ,
+                       q0=c.q0())
+    for name in ["n", "q_spatial", "q_prime", "q0"]:
+        grad_close(name, got[name], radj[name])
+
+
+@pytest.mark.parametrize("variant", ["ddr_match", "positivity"])
+def test_window_physics_variants(variant):
+    say("window variant", variant)
+    P = O.make_params(ddr_match=variant == "ddr_match",
+                      enforce_positivity=variant == "positivity")
+    c = Case(1000, 30, seed=14, params=P, storms=True)
+    gQ = np.random.default_rng(15).uniform(0, 1, (c.T, c.R)).astype(np.float32)
+    got = c.gpu_window(D.FORM_PERSISTENT, gQ=gQ)
+    assert relerr(got["Q"], c.oracle_Q()) <= 1e-4
+    ref = O.route_adj(c.P, c.pat, c.n, c.qsp, c.p, c.length, c.slope_raw, c.qprime, gQ,
+                      q0=c.q0())
+    for name in ["n", "q_spatial", "p_spatial", "q_prime"]:
+        grad_close(name, got[name], ref[name])
+
+
+def test_tri_solve_vs_oracle():
+    """triangular_csr_solve forward and CsrSolveOp backward (sparse/mod.rs:425-531)
+    with arbitrary A values (non-unit diagonal)."""
+    say("tri solve 2500")
+    rng = np.random.default_rng(16)
+    R = 2500
+    rows, cols = random_tree(R, 64, rng)
+    net = Network(SparseAdjacency.from_coo(R, rows, cols))
+    pat = O.pattern_from_coo(R, rows, cols)
+    a = rng.uniform(-0.9, -0.1, pat.nnz).astype(np.float32)
+    a[pat.diag_mask == 1] = rng.uniform(0.8, 1.5, R)
+    b = rng.uniform(0, 2, R).astype(np.float32)
+    at, bt = t(a).requires_grad_(True), t(b).requires_grad_(True)
+    x = D.triangular_csr_solve(net, at, bt)
+    g = rng.uniform(-1, 1, R).astype(np.float32)
+    x.backward(t(g))
+    xr = O.forward_sub_lower(pat, a, b)
+    np.testing.assert_array_equal(x.detach().cpu().numpy(), xr)
+    gb = O.back_sub_upper_transposed(pat, a, g)
+    np.testing.assert_array_equal(bt.grad.cpu().numpy(), gb)
+    ga = (-gb[pat.row_for_nnz]) * xr[pat.col]
+    np.testing.assert_array_equal(at.grad.cpu().numpy(), ga)
+
+
+def test_determinism_and_edge_sizes():
+    """Two runs are bit-identical; a single reach and T = 1 work."""
+    say("determinism + edge sizes")
+    c = Case(3000, 40, seed=17, storms=True)
+    a = c.gpu_window(D.FORM_PERSISTENT)
+    b = c.gpu_window(D.FORM_PERSISTENT)
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k])
+    one = Case(1, 10, seed=18, rows=np.zeros(0, np.int32), cols=np.zeros(0, np.int32))
+    got = one.gpu_window(D.FORM_PERSISTENT)
+    assert relerr(got["Q"], one.oracle_Q()) <= 1e-4
+    t1 = Case(50, 1, seed=19)
+    got = t1.gpu_window(D.FORM_PERSISTENT)
+    np.testing.assert_array_equal(got["Q"][0], np.maximum(t1.q0(), 1e-4))
+    assert np.all(got["n"] == 0)
+
+
+def test_muskingum_cunge_api_end_to_end():
+    """MuskingumCunge.setup_inputs + forward (mmc.rs:239-632) with [0,1]
+    parameters, cold start and autograd to the normalized parameters; against
+    the oracle with the denormalize chain rule applied."""
+    say("MuskingumCunge API 800 x 36")
+    c = Case(800, 36, seed=20)
+    rng = np.random.default_rng(21)
+    un = rng.uniform(0, 1, c.R).astype(np.float32)
+    uq = rng.uniform(0, 1, c.R).astype(np.float32)
+    params = D.Params()
+    mc = D.MuskingumCunge(params)
+    adj = SparseAdjacency.from_coo(c.R, c.rows, c.cols, None, c.length, c.slope_raw)
+    pn, pq = t(un).requires_grad_(True), t(uq).requires_grad_(True)
+    qp = t(c.qprime).requires_grad_(True)
+    mc.setup_inputs(D.RoutingInputs(adj), qp, D.SpatialParameters(pn, pq))
+    out = mc.forward()
+    assert tuple(out.shape) == (c.R, c.T)
+    out.sum().backward()
+    nphys = O.denormalize(un, 0.015, 0.25, False)
+    qphys = O.denormalize(uq, 0.0, 1.0, False)
+    Qr = O.route_fwd(c.P, c.pat, nphys, qphys, c.p, c.length, c.slope_raw, c.qprime)
+    assert relerr(out.detach().cpu().numpy().T, Qr) <= 1e-4
+    g = O.route_adj(c.P, c.pat, nphys, qphys, c.p, c.length, c.slope_raw, c.qprime,
+                    np.ones((c.T, c.R), np.float32))
+    grad_close("n", pn.grad.cpu().numpy(), g["n"] * (0.25 - 0.015))
+    grad_close("q_spatial", pq.grad.cpu().numpy(), g["q_spatial"])
+    grad_close("q_prime", qp.grad.cpu().numpy(), g["q_prime"])
