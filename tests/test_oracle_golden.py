@@ -94,3 +94,24 @@ def test_pattern_layout():
     assert pat.trans_col.tolist() == [0, 2, 1, 3, 2, 3, 3, 4, 4]
     with pytest.raises(ValueError):
         O.pattern_from_coo(3, np.array([0], np.int32), np.array([1], np.int32))
+
+
+def test_compress_invariants():
+    """collate::compress (data/collate.rs:197-349): the active set is sorted and
+    unique; every compressed edge and gauge maps back to its CONUS position;
+    lower-triangular is kept; ddr_match reads the gauge's upstream columns."""
+    rng = np.random.default_rng(3)
+    R = 400
+    down = np.array([i + 1 + rng.integers(0, min(8, R - 1 - i)) for i in range(R - 1)])
+    edges = [(int(down[i]), i) for i in range(R - 1) if rng.random() < 0.6]
+    gauges = sorted(int(g) for g in rng.choice(R, 7, replace=False))
+    active, rows, cols, gc, out = topo.compress(edges + edges[:5], gauges)
+    assert active == sorted(set(active))
+    assert sorted(set(edges)) == [(active[r], active[c]) for r, c in zip(rows, cols)]
+    assert [active[g] for g in gc] == gauges
+    assert all(r >= c for r, c in zip(rows, cols))
+    assert out == [[g] for g in gc]
+    _, rows2, cols2, gc2, out2 = topo.compress(edges, gauges, ddr_match=True)
+    for g, o in zip(gc2, out2):
+        ups = [c for r, c in zip(rows2, cols2) if r == g]
+        assert o == (ups if ups else [g])
