@@ -70,6 +70,7 @@ struct ddrs_workspace {
     Statics st{};
     float *st_q = nullptr, *st_i = nullptr, *st_lam = nullptr;
     float *acc_n = nullptr, *acc_qsp = nullptr, *acc_p = nullptr;
+    float *sc_q = nullptr, *sc_i = nullptr, *sc_qp = nullptr;
     float* ck = nullptr;  // [max_seg][2][R]
     uint64_t *hist = nullptr, *amsg = nullptr, *slot = nullptr;
     int* err = nullptr;
@@ -440,6 +441,9 @@ ddrs_status ddrs_workspace_create(ddrs_plan_t plan, int32_t window, int32_t max_
     F(&ws->acc_n, R);
     F(&ws->acc_qsp, R);
     F(&ws->acc_p, R);
+    F(&ws->sc_q, R);
+    F(&ws->sc_i, R);
+    F(&ws->sc_qp, R);
     F(&ws->ck, static_cast<size_t>(ws->max_seg) * 2 * R);
     if (st == DDRS_OK)
         st = dalloc(ws, reinterpret_cast<void**>(&ws->hist), static_cast<size_t>(window) * R * 8);
@@ -558,6 +562,9 @@ static AdjArgs adj_args(ddrs_workspace* ws, const ddrs_params* p) {
     a.acc_n = ws->acc_n;
     a.acc_qsp = ws->acc_qsp;
     a.acc_p = ws->acc_p;
+    a.sc_q = ws->sc_q;
+    a.sc_i = ws->sc_i;
+    a.sc_qp = ws->sc_qp;
     a.err = ws->err;
     return a;
 }

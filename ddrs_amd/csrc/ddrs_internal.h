@@ -13,7 +13,7 @@
 namespace ddrs {
 
 // Upstream (downstream) entries of one 64-reach chunk are staged in LDS.
-constexpr int kStageMax = 512;
+constexpr int kStageMax = 256;
 // Chunks one persistent workgroup may own (C <= kMaxChunksPerWave * grid).
 constexpr int kMaxChunksPerWave = 128;
 
@@ -82,6 +82,9 @@ struct AdjArgs {
     float* acc_n;
     float* acc_qsp;
     float* acc_p;
+    float* sc_q;           // [R] out[t] of the step being processed (prepare -> solve)
+    float* sc_i;           // [R] N out[t]
+    float* sc_qp;          // [R] raw q' of the step
     int* err;
 };
 

@@ -63,15 +63,28 @@ constexpr uint64_t kWatchdogTicks = 200000000ull;  // 2 s at 100 MHz
 // ---------------------------------------------------------------------------
 // Persistent driver shared by all solves. Dir = +1 walks steps forward and
 // chunks in increasing order, Dir = -1 walks steps backward (step index
-// nsteps-1 first) and chunks in decreasing order. ready(c, s) polls the
-// cross-chunk inputs of chunk c at window step s and stages them; process(c, s)
-// computes and publishes it. Both are called by the whole wave.
-template <int Dir, class Ready, class Process>
-__device__ __forceinline__ void persistent_loop(int C, int nsteps, int* tcur, int* err,
-                                                Ready ready, Process process) {
+// nsteps-1 first) and chunks in decreasing order. For chunk c at window step s:
+//   prepare(c, s, slot)  everything that does not depend on other chunks at
+//                        step s (coefficients, right-hand side), kept in LDS
+//                        slot `slot`;
+//   ready(c, s)          polls the cross-chunk inputs of step s and stages them,
+//                        with each entry's weight and kind, in LDS;
+//   solve(c, s, slot)    the dependent part: the level-by-level solve inside the
+//                        chunk, publishing, and the step's epilogue.
+// A wave's first kPrepSlots chunks keep a prepared step in LDS, so their
+// coefficients are computed while the inputs are still in flight; other
+// chunks prepare just in time in the scratch slot kPrepSlots. All three are
+// called by the whole wave.
+constexpr int kPrepSlots = 2;
+
+template <int Dir, class Prepare, class Ready, class Solve>
+__device__ __forceinline__ void persistent_loop(int C, int nsteps, int* tcur, int* pstep,
+                                                int* err, Prepare prepare, Ready ready,
+                                                Solve solve) {
     const int W = gridDim.x, w = blockIdx.x;
     const int m = (C - w + W - 1) / W;
     for (int k = threadIdx.x; k < m; k += 64) tcur[k] = 0;
+    if (threadIdx.x < kPrepSlots) pstep[threadIdx.x] = -1;
     __syncthreads();
     int remaining = m;
     uint64_t last = wall_ticks();
@@ -82,11 +95,21 @@ __device__ __forceinline__ void persistent_loop(int C, int nsteps, int* tcur, in
             int done = uni(tcur[k]);
             if (done >= nsteps) continue;
             const int c = w + k * W;
+            const int slot = (k < kPrepSlots) ? k : kPrepSlots;
             int burst = 0;
+            const bool jit = slot == kPrepSlots;
             while (done < nsteps && burst < 8) {
                 const int s = (Dir > 0) ? done : (nsteps - 1 - done);
-                if (!ready(c, s)) break;
-                process(c, s);
+                // slotted chunks prepare before polling, the scratch slot after
+                if (jit && !ready(c, s)) break;
+                if (jit || uni(pstep[slot]) != s) {
+                    prepare(c, s, slot);
+                    __syncthreads();
+                    if (!jit && threadIdx.x == 0) pstep[slot] = s;
+                    __syncthreads();
+                }
+                if (!jit && !ready(c, s)) break;
+                solve(c, s, slot);
                 ++done;
                 ++burst;
             }
@@ -114,13 +137,42 @@ __device__ __forceinline__ void persistent_loop(int C, int nsteps, int* tcur, in
 // forward window
 
 template <int PH>
-__global__ __launch_bounds__(64) void k_fwd_window(FwdArgs a) {
+__global__ __launch_bounds__(64, 5) void k_fwd_window(FwdArgs a) {
     __shared__ float lds_x[64];
-    __shared__ float stage[kStageMax];
+    __shared__ float sval[kStageMax], sadj[kStageMax];
+    __shared__ short skind[kStageMax];  // >= 0: lane of an intra-chunk upstream; -1: staged
+    __shared__ float pb[kPrepSlots + 1][64], pnc1[kPrepSlots + 1][64];
+    __shared__ int pe0[kPrepSlots + 1][64], pe1[kPrepSlots + 1][64], porig[kPrepSlots + 1][64];
+    __shared__ int psub[kPrepSlots + 1][64];
+    __shared__ int pstep[kPrepSlots];
     __shared__ int tcur[kMaxChunksPerWave];
     const int lane = threadIdx.x;
     const Sched& sc = a.sc;
     const int R = sc.R;
+
+    // S1..S25 for chunk c at window step s: coefficients and right-hand side.
+    auto prepare = [&](int c, int s, int slot) {
+        const int p = c * 64 + lane;
+        const int base = sc.up_ptr[c * 64];
+        if (p < R) {
+            const int t = a.t0 + a.s_base + s;
+            const int orig = sc.perm[p];
+            const float q = a.st_q[p];
+            const float it = a.st_i[p];
+            const float qp = fmaxr(a.qprime[static_cast<size_t>(t) * R + orig], a.P.q_lb);
+            Coef co;
+            mc_coef<PH>(a.P, a.st.n[p], a.st.qsp[p], a.st.p[p], q, a.st.len[p], a.st.slope[p],
+                        a.st.sqrt_s[p], a.st.xst[p], co);
+            pb[slot][lane] = ((co.c2 * it) + (co.c3 * q)) + (co.c4 * qp);        // S25
+            pnc1[slot][lane] = -co.c1;
+            pe0[slot][lane] = sc.up_ptr[p] - base;
+            pe1[slot][lane] = sc.up_ptr[p + 1] - base;
+            porig[slot][lane] = orig;
+            psub[slot][lane] = sc.sub[p];
+        } else {
+            psub[slot][lane] = -1;
+        }
+    };
 
     auto ready = [&](int c, int s) -> bool {
         const int p0 = c * 64;
@@ -130,10 +182,14 @@ __global__ __launch_bounds__(64) void k_fwd_window(FwdArgs a) {
         bool ok = true;
         for (int e = base + lane; e < end; e += 64) {
             const int u = sc.up_pos[e];
-            if ((u >> 6) != c) {
+            sadj[e - base] = sc.up_adj[e];
+            if ((u >> 6) == c) {
+                skind[e - base] = static_cast<short>(u & 63);
+            } else {
+                skind[e - base] = -1;
                 const uint64_t wv = poll(row + u);
                 if (!word_ready(wv, a.epoch)) ok = false;
-                else stage[e - base] = word_value(wv);
+                else sval[e - base] = word_value(wv);
             }
         }
         const bool all = __all(ok);
@@ -141,37 +197,27 @@ __global__ __launch_bounds__(64) void k_fwd_window(FwdArgs a) {
         return all;
     };
 
-    auto process = [&](int c, int s) {
+    // S26..S28 and the next step's S24 (N Q_{t+1}) from LDS only.
+    auto solve = [&](int c, int s, int slot) {
         const int p = c * 64 + lane;
-        const bool valid = p < R;
-        const int sg = a.s_base + s;           // window-local step index
-        const int t = a.t0 + sg;               // producing out[t+1] from out[t]
-        const int base = sc.up_ptr[c * 64];
+        const int sg = a.s_base + s;
+        const int t = a.t0 + sg;
         uint64_t* row = a.hist + static_cast<size_t>(sg) * R;
-        Coef co;
-        float q = 0.f, b = 0.f, x = 0.f, inext = 0.f;
-        int orig = 0, my = -1;
-        if (valid) {
-            orig = sc.perm[p];
-            q = a.st_q[p];
-            const float it = a.st_i[p];
-            const float qp = fmaxr(a.qprime[static_cast<size_t>(t) * R + orig], a.P.q_lb);
-            mc_coef<PH>(a.P, a.st.n[p], a.st.qsp[p], a.st.p[p], q, a.st.len[p], a.st.slope[p],
-                        a.st.sqrt_s[p], a.st.xst[p], co);
-            b = ((co.c2 * it) + (co.c3 * q)) + (co.c4 * qp);             // S25
-            my = sc.sub[p];
-        }
+        const int my = psub[slot][lane];
+        const float b = pb[slot][lane];
+        const float nc1 = pnc1[slot][lane];
+        const int e0 = pe0[slot][lane], e1 = pe1[slot][lane];
+        float x = 0.f, inext = 0.f;
         const int nsub = uni(sc.nsub[c]);
         for (int l = 0; l < nsub; ++l) {
-            if (my == l) {                                               // S26 + S27
+            if (my == l) {
                 float sum = 0.0f, in = 0.0f;
-                const float nc1 = -co.c1;
-                for (int e = sc.up_ptr[p]; e < sc.up_ptr[p + 1]; ++e) {
-                    const int u = sc.up_pos[e];
-                    const float v = ((u >> 6) == c) ? lds_x[u & 63] : stage[e - base];
-                    const float adj = sc.up_adj[e];
+                for (int e = e0; e < e1; ++e) {
+                    const int kd = skind[e];
+                    const float v = (kd >= 0) ? lds_x[kd] : sval[e];
+                    const float adj = sadj[e];
                     sum = sum + (nc1 * adj) * v;
-                    in = in + fmaxr(v, a.P.q_lb) * adj;                  // next step's S24
+                    in = in + fmaxr(v, a.P.q_lb) * adj;
                 }
                 const float ad = 1.0f + nc1 * 0.0f;
                 x = (b - sum) / ad;
@@ -181,8 +227,9 @@ __global__ __launch_bounds__(64) void k_fwd_window(FwdArgs a) {
             }
             __syncthreads();
         }
-        if (valid) {                                                     // S28
+        if (my >= 0) {
             const float qn = fmaxr(x, a.P.q_lb);
+            const int orig = porig[slot][lane];
             a.st_q[p] = qn;
             a.st_i[p] = inext;
             if (a.Q_out) a.Q_out[static_cast<size_t>(t + 1) * R + orig] = qn;
@@ -190,20 +237,64 @@ __global__ __launch_bounds__(64) void k_fwd_window(FwdArgs a) {
         }
     };
 
-    persistent_loop<+1>(sc.C, a.nsteps, tcur, a.err, ready, process);
+    persistent_loop<+1>(sc.C, a.nsteps, tcur, pstep, a.err, prepare, ready, solve);
 }
 
 // ---------------------------------------------------------------------------
 // adjoint window
 
 template <int PH>
-__global__ __launch_bounds__(64) void k_adj_window(AdjArgs a) {
+__global__ __launch_bounds__(64, 4) void k_adj_window(AdjArgs a) {
     __shared__ float ldsP[64], ldsG[64];
-    __shared__ float stP[kStageMax], stG[kStageMax];
+    __shared__ float stP[kStageMax], stG[kStageMax], sadj[kStageMax];
+    __shared__ short skind[kStageMax];
+    __shared__ float pgx[kPrepSlots + 1][64], pnc1[kPrepSlots + 1][64], pc2[kPrepSlots + 1][64];
+    __shared__ int pe0[kPrepSlots + 1][64], pe1[kPrepSlots + 1][64], psub[kPrepSlots + 1][64];
+    __shared__ int pstep[kPrepSlots];
     __shared__ int tcur[kMaxChunksPerWave];
     const int lane = threadIdx.x;
     const Sched& sc = a.sc;
     const int R = sc.R;
+
+    // Everything of reverse step s that does not wait for downstream messages:
+    // out[t], N out[t] and q' of the step, the coefficients, and B28.
+    auto prepare = [&](int c, int s, int slot) {
+        const int p = c * 64 + lane;
+        const int base = sc.dn_ptr[c * 64];
+        if (p < R) {
+            const int sg = a.s_base + s;
+            const int t = a.t0 + sg;
+            const int orig = sc.perm[p];
+            const float xs = word_value(a.hist[static_cast<size_t>(sg) * R + p]);
+            float q, it;
+            if (sg > 0) {
+                const uint64_t* hq = a.hist + static_cast<size_t>(sg - 1) * R;
+                q = fmaxr(word_value(hq[p]), a.P.q_lb);
+                float acc = 0.0f;
+                for (int e = sc.up_ptr[p]; e < sc.up_ptr[p + 1]; ++e)
+                    acc = acc + fmaxr(word_value(hq[sc.up_pos[e]]), a.P.q_lb) * sc.up_adj[e];
+                it = acc;
+            } else {
+                q = a.ck_q[p];
+                it = a.ck_i[p];
+            }
+            const float qp_raw = a.qprime[static_cast<size_t>(t) * R + orig];
+            Coef co;
+            mc_coef<PH>(a.P, a.st.n[p], a.st.qsp[p], a.st.p[p], q, a.st.len[p], a.st.slope[p],
+                        a.st.sqrt_s[p], a.st.xst[p], co);
+            pgx[slot][lane] = (xs > a.P.q_lb) ? a.st_lam[p] : 0.0f;            // B28
+            pnc1[slot][lane] = -co.c1;
+            pc2[slot][lane] = co.c2;
+            pe0[slot][lane] = sc.dn_ptr[p] - base;
+            pe1[slot][lane] = sc.dn_ptr[p + 1] - base;
+            psub[slot][lane] = sc.subr[p];
+            a.sc_q[p] = q;
+            a.sc_i[p] = it;
+            a.sc_qp[p] = qp_raw;
+        } else {
+            psub[slot][lane] = -1;
+        }
+    };
 
     auto ready = [&](int c, int s) -> bool {
         const int p0 = c * 64;
@@ -213,7 +304,11 @@ __global__ __launch_bounds__(64) void k_adj_window(AdjArgs a) {
         bool ok = true;
         for (int e = base + lane; e < end; e += 64) {
             const int d = sc.dn_pos[e];
-            if ((d >> 6) != c) {
+            sadj[e - base] = sc.dn_adj[e];
+            if ((d >> 6) == c) {
+                skind[e - base] = static_cast<short>(d & 63);
+            } else {
+                skind[e - base] = -1;
                 const uint64_t w0 = poll(row + 2 * d);
                 const uint64_t w1 = poll(row + 2 * d + 1);
                 if (!word_ready(w0, a.epoch) || !word_ready(w1, a.epoch)) {
@@ -229,58 +324,34 @@ __global__ __launch_bounds__(64) void k_adj_window(AdjArgs a) {
         return all;
     };
 
-    auto process = [&](int c, int s) {
+    auto solve = [&](int c, int s, int slot) {
         const int p = c * 64 + lane;
-        const bool valid = p < R;
         const int sg = a.s_base + s;
-        const int t = a.t0 + sg;  // forward step out[t] -> out[t+1]
-        const int base = sc.dn_ptr[c * 64];
-        const uint64_t* hx = a.hist + static_cast<size_t>(sg) * R;         // x of out[t+1]
-        const uint64_t* hq = (sg > 0) ? a.hist + static_cast<size_t>(sg - 1) * R : nullptr;
+        const int t = a.t0 + sg;
         uint64_t* mrow = a.amsg + static_cast<size_t>(sg) * R * 2;
-        Coef co;
-        float q = 0.f, it = 0.f, qp = 0.f, qp_raw = 0.f, gx = 0.f, gb = 0.f, gq24 = 0.f;
-        int orig = 0, my = -1;
-        if (valid) {
-            orig = sc.perm[p];
-            const float xs = word_value(hx[p]);
-            if (hq) {
-                q = fmaxr(word_value(hq[p]), a.P.q_lb);
-                float acc = 0.0f;
-                for (int e = sc.up_ptr[p]; e < sc.up_ptr[p + 1]; ++e)
-                    acc = acc + fmaxr(word_value(hq[sc.up_pos[e]]), a.P.q_lb) * sc.up_adj[e];
-                it = acc;
-            } else {
-                q = a.ck_q[p];
-                it = a.ck_i[p];
-            }
-            qp_raw = a.qprime[static_cast<size_t>(t) * R + orig];
-            qp = fmaxr(qp_raw, a.P.q_lb);
-            mc_coef<PH>(a.P, a.st.n[p], a.st.qsp[p], a.st.p[p], q, a.st.len[p], a.st.slope[p],
-                        a.st.sqrt_s[p], a.st.xst[p], co);
-            const float lam = a.st_lam[p];
-            gx = (xs > a.P.q_lb) ? lam : 0.0f;                           // B28
-            my = sc.subr[p];
-        }
+        const int my = psub[slot][lane];
+        const float gx = pgx[slot][lane];
+        const float nc1 = pnc1[slot][lane];
+        const float c2 = pc2[slot][lane];
+        const int e0 = pe0[slot][lane], e1 = pe1[slot][lane];
+        float gb = 0.f, gq24 = 0.f;
         const int nsub = uni(sc.nsubr[c]);
         for (int l = 0; l < nsub; ++l) {
             if (my == l) {                                               // B27 transposed solve
                 float sumP = 0.0f, sumG = 0.0f;
-                for (int e = sc.dn_ptr[p]; e < sc.dn_ptr[p + 1]; ++e) {
-                    const int d = sc.dn_pos[e];
-                    const bool intra = (d >> 6) == c;
-                    const float Pd = intra ? ldsP[d & 63] : stP[e - base];
-                    const float Gd = intra ? ldsG[d & 63] : stG[e - base];
-                    const float adj = sc.dn_adj[e];
+                for (int e = e0; e < e1; ++e) {
+                    const int kd = skind[e];
+                    const float Pd = (kd >= 0) ? ldsP[kd] : stP[e];
+                    const float Gd = (kd >= 0) ? ldsG[kd] : stG[e];
+                    const float adj = sadj[e];
                     sumP = sumP + adj * Pd;
                     sumG = sumG + Gd * adj;                              // B24 N^T gi
                 }
-                const float nc1 = -co.c1;
                 const float ad = 1.0f + nc1 * 0.0f;
                 gb = (gx - sumP) / ad;
                 gq24 = sumG;
                 const float Pm = nc1 * gb;
-                const float Gm = co.c2 * gb;
+                const float Gm = c2 * gb;
                 ldsP[lane] = Pm;
                 ldsG[lane] = Gm;
                 publish(mrow + 2 * p, Pm, a.epoch);
@@ -288,7 +359,8 @@ __global__ __launch_bounds__(64) void k_adj_window(AdjArgs a) {
             }
             __syncthreads();
         }
-        if (valid) {
+        if (my >= 0) {
+            const uint64_t* hx = a.hist + static_cast<size_t>(sg) * R;
             // B26: gc1 = -sum_k adj_k * ((-gb) * x[col k]) in CSR order
             float acc = 0.0f;
             for (int e = sc.up_ptr[p]; e < sc.up_ptr[p + 1]; ++e) {
@@ -296,12 +368,18 @@ __global__ __launch_bounds__(64) void k_adj_window(AdjArgs a) {
                 acc = acc + gA * sc.up_adj[e];
             }
             const float gc1 = -acc;
+            const float q = a.sc_q[p], it = a.sc_i[p], qp_raw = a.sc_qp[p];
+            const float qp = fmaxr(qp_raw, a.P.q_lb);
+            Coef co;
+            mc_coef<PH>(a.P, a.st.n[p], a.st.qsp[p], a.st.p[p], q, a.st.len[p], a.st.slope[p],
+                        a.st.sqrt_s[p], a.st.xst[p], co);
             const LocalGrads g = mc_local_bwd<PH>(a.P, co, a.st.n[p], a.st.p[p], q, qp,
                                                   a.st.len[p], a.st.slope[p], a.st.sqrt_s[p],
                                                   it, gb, gc1, gq24);
             a.acc_n[p] = a.acc_n[p] + g.gn;
             a.acc_qsp[p] = a.acc_qsp[p] + g.gqsp;
             a.acc_p[p] = a.acc_p[p] + g.gp;
+            const int orig = sc.perm[p];
             const float direct = a.gQ ? a.gQ[static_cast<size_t>(t) * R + orig] : 0.0f;
             a.st_lam[p] = direct + g.gq;
             if (a.gqp_out)
@@ -310,7 +388,7 @@ __global__ __launch_bounds__(64) void k_adj_window(AdjArgs a) {
         }
     };
 
-    persistent_loop<-1>(sc.C, a.nsteps, tcur, a.err, ready, process);
+    persistent_loop<-1>(sc.C, a.nsteps, tcur, pstep, a.err, prepare, ready, solve);
 }
 
 // ---------------------------------------------------------------------------
@@ -364,7 +442,10 @@ __global__ __launch_bounds__(64) void k_tri_fwd(TriArgs a) {
         }
         if (valid) a.x[sc.perm[p]] = a.clamp ? fmaxr(x, a.lb) : x;
     };
-    persistent_loop<+1>(sc.C, 1, tcur, a.err, ready, process);
+    __shared__ int pstep[kPrepSlots];
+    auto prepare = [&](int, int, int) {};
+    auto solve = [&](int c, int s, int) { process(c, s); };
+    persistent_loop<+1>(sc.C, 1, tcur, pstep, a.err, prepare, ready, solve);
 }
 
 __global__ __launch_bounds__(64) void k_tri_adj(TriArgs a) {
@@ -417,7 +498,10 @@ __global__ __launch_bounds__(64) void k_tri_adj(TriArgs a) {
         }
         if (valid) a.x[sc.perm[p]] = g;
     };
-    persistent_loop<-1>(sc.C, 1, tcur, a.err, ready, process);
+    __shared__ int pstep[kPrepSlots];
+    auto prepare = [&](int, int, int) {};
+    auto solve = [&](int c, int s, int) { process(c, s); };
+    persistent_loop<-1>(sc.C, 1, tcur, pstep, a.err, prepare, ready, solve);
 }
 
 // grad_a[k] = (-gb[row k]) * x[col k] over the CSR entries (dispatch.rs:120-136)

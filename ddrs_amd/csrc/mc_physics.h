@@ -12,6 +12,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "mc_math.h"
+
 namespace ddrs {
 
 struct Phys {
@@ -25,16 +27,13 @@ enum : int { PH_DEFAULT = 0, PH_DDR = 1, PH_POS = 2 };
 
 constexpr float kDelta = 1e-2f;  // POSITIVITY_DELTA, mmc_op.rs:57
 
-// x^y and ln(x) evaluated in double and rounded once to fp32. On this path's
-// inputs this reproduces the reference CPU's libm powf on 99.94% of calls; the
-// fp32 powf of the device library agrees on only 79% (measured on gfx950), and
-// every disagreement seeds a rounding difference that the time loop amplifies.
-__device__ __forceinline__ float rpow(float x, float y) {
-    return static_cast<float>(pow(static_cast<double>(x), static_cast<double>(y)));
-}
-__device__ __forceinline__ float rlog(float x) {
-    return static_cast<float>(log(static_cast<double>(x)));
-}
+// x^y and ln(x) are evaluated in double and rounded once to fp32 (mc_math.h):
+// on this path's inputs that reproduces the reference CPU's libm powf on 99.94%
+// of calls, where the device library's fp32 powf agrees on only 79% (measured
+// on gfx950); every disagreement seeds a rounding difference that the time
+// loop amplifies.
+__device__ __forceinline__ float rpow(float x, float y) { return pow_f32(x, y); }
+__device__ __forceinline__ float rlog(float x) { return log_f32(x); }
 
 __device__ __forceinline__ float fmaxr(float x, float lo) { return (x < lo) ? lo : x; }
 __device__ __forceinline__ float fclamp(float x, float lo, float hi) {

@@ -1,0 +1,79 @@
+"""The device pow/log of ddrs_amd/csrc/mc_math.h, compiled for the host with
+g++ (host and device evaluate the same fma/IEEE operations), against libm
+powf/logf (the reference's) and against long-double powl/logl (the exact value
+to ~2^-64). CPU only."""
+import os
+import subprocess
+import tempfile
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+SRC = r'''
+#include <cmath>
+#include <cstdio>
+#include <cstdint>
+#include "mc_math.h"
+static uint64_t st = 88172645463325252ull;
+static double u() { st ^= st << 13; st ^= st >> 7; st ^= st << 17; return (st >> 11) * (1.0 / 9007199254740992.0); }
+int main() {
+  long n = 0, d_libm = 0, d_cr = 0, dl_libm = 0, dl_cr = 0;
+  double worst = 0.0;
+  for (int k = 0; k < 3000000; ++k) {
+    float x, y;
+    int m = k % 3;
+    if (m == 0) { x = (float)exp(log(1e-7) + u() * log(1e12)); float qe = (float)u() + 1e-6f; y = (1.0f / (qe * 3.0f + 5.0f)) * 3.0f; }
+    else if (m == 1) { x = (float)exp(log(0.001) + u() * log(1e5)); y = (float)u() + 1e-6f; }
+    else { x = (float)exp(log(1e-4) + u() * log(1e6)); y = (k % 6 == 2) ? 2.0f : 2.0f / 3.0f; }
+    float a = ddrs::pow_f32(x, y);
+    long double ex = powl((long double)x, (long double)y);
+    float cr = (float)ex;  /* rounding of a ~64-bit value: correctly rounded except at ties */
+    double rel = fabs((double)(((long double)a - ex) / ex));
+    if (rel > worst) worst = rel;
+    n++; d_libm += (a != powf(x, y)); d_cr += (a != cr);
+    float lx = ddrs::log_f32(x);
+    dl_libm += (lx != logf(x));
+    dl_cr += (lx != (float)logl((long double)x));
+  }
+  /* special inputs take the library path */
+  int special_ok = 1;
+  const float xs[] = {0.0f, -1.5f, INFINITY, NAN, 1.0f, 3.0f, 1e-38f, 1e-44f, 2.5e38f};
+  const float ys[] = {0.5f, 2.0f, 0.0f, -0.7f, 2.0f / 3.0f, 1e-6f, 0.375f};
+  for (float xv : xs) for (float yv : ys) {
+    float a = ddrs::pow_f32(xv, yv), b = powf(xv, yv);
+    if (!((a == b) || (std::isnan(a) && std::isnan(b)) || fabs((double)a - b) <= 1e-7 * fabs((double)b))) {
+      printf("# pow special %g %g: %g vs %g\n", xv, yv, a, b); special_ok = 0; }
+  }
+  for (float xv : xs) {
+    float a = ddrs::log_f32(xv), b = logf(xv);
+    if (!((a == b) || (std::isnan(a) && std::isnan(b)) || fabs((double)a - b) <= 1e-7 * fabs((double)b))) {
+      printf("# log special %g: %g vs %g\n", xv, a, b); special_ok = 0; }
+  }
+  printf("%ld %ld %ld %ld %.3e %d %ld\n", n, d_libm, d_cr, dl_libm, worst, special_ok, dl_cr);
+  return 0;
+}
+'''
+
+
+def test_device_pow_log_match_libm():
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "t.cpp")
+        exe = os.path.join(d, "t")
+        open(src, "w").write(SRC)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I",
+                        os.path.join(ROOT, "ddrs_amd", "csrc"), src, "-o", exe, "-lm"], check=True)
+        raw = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    print(raw)
+    out = [ln for ln in raw.splitlines() if not ln.startswith('#')][0].split()
+    n, d_libm, d_cr, dl_libm = (int(v) for v in out[:4])
+    worst, special_ok, dl_cr = float(out[4]), int(out[5]), int(out[6])
+    print(f"pow_f32 vs libm powf: {d_libm}/{n} differ; vs correctly rounded: {d_cr}/{n}; "
+          f"log_f32 vs logf: {dl_libm}/{n}, vs correctly rounded {dl_cr}/{n}; "
+          f"worst rel err of the fp32 result {worst:.3e}")
+    assert d_cr <= n * 1e-5            # correctly rounded but for near-midpoint cases
+    assert d_libm <= n * 1e-3          # libm powf is itself off on ~0.06% of these
+    assert dl_cr <= n * 1e-5           # log_f32 is correctly rounded but near midpoints
+    assert dl_libm <= n * 5e-3         # libm logf misrounds on ~0.2% of these
+    assert worst <= 2.0 ** -24 * 1.0001  # within half an ulp (+ ties)
+    assert special_ok == 1
