@@ -117,6 +117,7 @@ def main():
     NAT.check(NAT.lib.ddrs_set_timing(ws.handle, 1))
     fwd_ms = adj_ms = 0.0
     nf = na = 0
+    phase_ms = np.zeros(3)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -125,6 +126,9 @@ def main():
              np.zeros(1, np.float32)]
         NAT.check(NAT.lib.ddrs_get_timing(ws.handle, *[x.ctypes.data for x in a]))
         nf += int(a[0][0]); fwd_ms += float(a[1][0]); na += int(a[2][0]); adj_ms += float(a[3][0])
+        ph = [np.zeros(1, np.float32) for _ in range(3)]
+        NAT.check(NAT.lib.ddrs_get_adj_phase_timing(ws.handle, *[x.ctypes.data for x in ph]))
+        phase_ms += [float(x[0]) for x in ph]
         log(f"[bench] step {k + 1}/{args.steps} done")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
@@ -133,8 +137,14 @@ def main():
     rts = R * (T - 1)
     value = rts * args.steps / elapsed
     ms_per_step = 1000.0 * elapsed / args.steps
+    lay = [np.zeros(1, np.int32) for _ in range(4)]
+    NAT.check(NAT.lib.ddrs_workspace_layout(ws.handle, *[x.ctypes.data for x in lay]))
+    layout = dict(fwd_reaches_per_lane=int(lay[0][0]), fwd_workgroups=int(lay[1][0]),
+                  sweep_reaches_per_lane=int(lay[2][0]), sweep_workgroups=int(lay[3][0]))
+    phase_ms /= args.steps
     log(f"[bench] persistent: {ms_per_step:.2f} ms/step, {value / 1e9:.3f} G reach-ts/s; "
-        f"fwd kernel {fwd_ms / max(nf, 1):.2f} ms, adj kernel {adj_ms / max(na, 1):.2f} ms")
+        f"fwd kernel {fwd_ms / args.steps:.2f} ms, adjoint {adj_ms / args.steps:.2f} ms "
+        f"(lin {phase_ms[0]:.2f}, sweep {phase_ms[1]:.2f}, fin {phase_ms[2]:.2f}); {layout}")
     gn_pers, gqsp_pers, Q_pers = gn.clone(), gqsp.clone(), Q.clone()
 
     plain = None

@@ -61,6 +61,9 @@ _SIGS = {
     "ddrs_hotstart_adj": (c_int, [P, ctypes.POINTER(DdrsParams), P, P, P, P]),
     "ddrs_set_timing": (c_int, [P, c_int]),
     "ddrs_get_timing": (c_int, [P, P, P, P, P]),
+    "ddrs_get_adj_phase_timing": (c_int, [P, P, P, P]),
+    "ddrs_workspace_layout": (c_int, [P, P, P, P, P]),
+    "ddrs_workspace_set_chunk_width": (c_int, [P, c_int, c_int]),
     "ddrs_build_topology": (c_int, [c_int, P, P, c_int, P, P, P, c_int, P, P, c_int, P]),
     "ddrs_compress": (c_int, [c_int, P, P, c_int, P, P, c_int, P, P, P, P, P]),
     "ddrs_gen_tree": (c_int, [c_int, c_int, c_u64, P, P]),

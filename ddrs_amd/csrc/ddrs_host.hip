@@ -43,19 +43,28 @@ inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 }  // namespace
 
+// Chunking of the schedule at one width (64 * RPL positions per chunk).
+struct ChunkHost {
+    int C = 0;
+    bool ok = false;  // every chunk's links fit the LDS staging capacity
+    std::vector<unsigned char> sub, subr;
+    std::vector<int> nsub, nsubr;
+    Chunks dev{};
+};
+
 struct ddrs_plan {
-    int R = 0, E = 0, nnz = 0, C = 0, nlevels = 0;
+    int R = 0, P = 0, E = 0, nnz = 0, C = 0, nlevels = 0;
     // CsrPattern (sparse/mod.rs:115-146)
     std::vector<int> crow, col, row_for_nnz, trans_crow, trans_col, trans_to_orig;
     std::vector<float> adj_values, diag_mask;
-    // schedule
-    std::vector<int> level, perm;
+    // schedule: order = reaches in position order; perm = position -> reach
+    // (-1 for an empty position)
+    std::vector<int> level, order, perm;
     std::vector<void*> dev;  // every device allocation, freed on destroy
     // schedule in host memory (uploaded when a workspace is created)
-    std::vector<int> h_up_ptr, h_up_pos, h_up_k, h_dn_ptr, h_dn_pos, h_dn_k, h_diag_k, h_nsub,
-        h_nsubr;
+    std::vector<int> h_up_ptr, h_up_pos, h_up_k, h_dn_ptr, h_dn_pos, h_dn_k, h_diag_k;
     std::vector<float> h_up_adj, h_dn_adj;
-    std::vector<unsigned char> h_sub, h_subr;
+    ChunkHost ch[kNumWidths];  // RPL = 1, 2
     bool on_device = false;
     Sched sc{};
     int* d_row_for_nnz = nullptr;
@@ -67,22 +76,28 @@ struct ddrs_workspace {
     int window = 0, max_steps = 0, max_seg = 0;
     size_t bytes = 0;
     std::vector<void*> dev;
-    Statics st{};
+    float4* st = nullptr;  // [P][3] reach statics (position-indexed buffers have P entries)
     float *st_q = nullptr, *st_i = nullptr, *st_lam = nullptr;
     float *acc_n = nullptr, *acc_qsp = nullptr, *acc_p = nullptr;
-    float *sc_q = nullptr, *sc_i = nullptr, *sc_qp = nullptr;
-    float* ck = nullptr;  // [max_seg][2][R]
+    float* ck = nullptr;  // [max_seg][2][P]
     uint64_t *hist = nullptr, *amsg = nullptr, *slot = nullptr;
+    unsigned long long* claim_ctr = nullptr;  // chunk claims of persistent launches
+    unsigned long long claims = 0;            // claims made so far
+    float4 *rec = nullptr, *recj = nullptr;  // [window][P]
+    unsigned char* recm = nullptr;           // [window][P]
+    float* gbuf = nullptr;                   // [window][P]
     int* err = nullptr;
     uint32_t epoch = 0;
-    int grid_fwd[3] = {0, 0, 0}, grid_adj[3] = {0, 0, 0}, grid_tri_fwd = 0, grid_tri_adj = 0;
+    // persistent launches: grid and chunk width (reaches per lane)
+    int grid_fwd[3] = {0, 0, 0}, rpl_fwd[3] = {1, 1, 1};
+    int grid_sweep = 0, rpl_sweep = 1, grid_tri_fwd = 0, grid_tri_adj = 0;
     bool statics_bound = false;
     // last forward window
     int last_T = -1, hist_seg = -1;
-    // timing
+    // timing: pairs of events around each launch of a kind
     bool timing = false;
-    std::vector<hipEvent_t> ev_fwd, ev_adj;
-    int n_fwd = 0, n_adj = 0;
+    std::vector<hipEvent_t> ev[4];  // forward window, lin, sweep, fin
+    int n_ev[4] = {0, 0, 0, 0};
 };
 
 namespace {
@@ -115,8 +130,12 @@ ddrs_status check_params(const ddrs_params* p) {
     return DDRS_OK;
 }
 
-void record(ddrs_workspace* ws, std::vector<hipEvent_t>& pool, int& n, hipStream_t s) {
+enum TimedKind : int { TK_FWD = 0, TK_LIN = 1, TK_SWEEP = 2, TK_FIN = 3 };
+
+void record(ddrs_workspace* ws, int kind, hipStream_t s) {
     if (!ws->timing) return;
+    std::vector<hipEvent_t>& pool = ws->ev[kind];
+    int& n = ws->n_ev[kind];
     if (static_cast<int>(pool.size()) <= n) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return;
@@ -221,71 +240,116 @@ ddrs_status ddrs_plan_create(int32_t n, int32_t nnz_off, const int32_t* h_rows,
     std::vector<int> lc(maxlev + 2, 0);
     for (int i = 0; i < R; ++i) lc[pl->level[i] + 1]++;
     for (int l = 0; l <= maxlev; ++l) lc[l + 1] += lc[l];
-    pl->perm.resize(R);
-    std::vector<int> inv(R);
-    for (int i = 0; i < R; ++i) {
-        int p = lc[pl->level[i]]++;
-        pl->perm[p] = i;
-        inv[i] = p;
+    pl->order.resize(R);
+    for (int i = 0; i < R; ++i) pl->order[lc[pl->level[i]]++] = i;
+    // Positions: the level-major order cut into 64-position chunks of at most
+    // 64 reaches and at most kMaxSubLevels sub-levels (the longest chain of
+    // links inside the chunk). A chunk is closed early when the next reach
+    // would deepen it past the cap, and its remaining positions stay empty, so
+    // a thin chain is spread over several chunks that pipeline over time
+    // steps instead of one chunk walking the whole chain every step.
+    std::vector<int> inv(R, -1), subl(R, 0);
+    {
+        int cur = 0, fill = 0;
+        for (int k = 0; k < R; ++k) {
+            const int i = pl->order[k];
+            int sl = 0;
+            for (int e = pl->crow[i]; e < pl->crow[i + 1]; ++e) {
+                const int u = pl->col[e];
+                if (u != i && (inv[u] >> 6) == cur) sl = std::max(sl, subl[u] + 1);
+            }
+            if (fill == 64 || sl >= kMaxSubLevels) {
+                ++cur;
+                fill = 0;
+                sl = 0;
+            }
+            inv[i] = cur * 64 + fill;
+            subl[i] = sl;
+            ++fill;
+        }
+        pl->C = cur + 1;
+        pl->P = 128 * ((pl->C + 1) / 2);  // whole chunks at both widths
     }
-    const int C = (R + 63) / 64;
-    pl->C = C;
-    std::vector<int> up_ptr(R + 1, 0), dn_ptr(R + 1, 0), diag_k(R, 0);
+    const int P = pl->P;
+    pl->perm.assign(P, -1);
+    for (int i = 0; i < R; ++i) pl->perm[inv[i]] = i;
+    std::vector<int> up_ptr(P + 1, 0), dn_ptr(P + 1, 0), diag_k(P, 0);
     std::vector<int> up_pos, up_k, dn_pos, dn_k;
     std::vector<float> up_adj, dn_adj;
     up_pos.reserve(nnz_off);
     dn_pos.reserve(nnz_off);
-    for (int p = 0; p < R; ++p) {
+    for (int p = 0; p < P; ++p) {
         const int i = pl->perm[p];
-        for (int k = pl->crow[i]; k < pl->crow[i + 1]; ++k) {
-            if (pl->col[k] == i) {
-                diag_k[p] = k;
-            } else {
-                up_pos.push_back(inv[pl->col[k]]);
-                up_adj.push_back(pl->adj_values[k]);
-                up_k.push_back(k);
+        if (i >= 0) {
+            for (int k = pl->crow[i]; k < pl->crow[i + 1]; ++k) {
+                if (pl->col[k] == i) {
+                    diag_k[p] = k;
+                } else {
+                    up_pos.push_back(inv[pl->col[k]]);
+                    up_adj.push_back(pl->adj_values[k]);
+                    up_k.push_back(k);
+                }
+            }
+            for (int k = pl->trans_crow[i]; k < pl->trans_crow[i + 1]; ++k) {
+                const int r = pl->trans_col[k];
+                if (r == i) continue;
+                const int ko = pl->trans_to_orig[k];
+                dn_pos.push_back(inv[r]);
+                dn_adj.push_back(pl->adj_values[ko]);
+                dn_k.push_back(ko);
             }
         }
         up_ptr[p + 1] = static_cast<int>(up_pos.size());
-        for (int k = pl->trans_crow[i]; k < pl->trans_crow[i + 1]; ++k) {
-            const int r = pl->trans_col[k];
-            if (r == i) continue;
-            const int ko = pl->trans_to_orig[k];
-            dn_pos.push_back(inv[r]);
-            dn_adj.push_back(pl->adj_values[ko]);
-            dn_k.push_back(ko);
-        }
         dn_ptr[p + 1] = static_cast<int>(dn_pos.size());
     }
-    // sub-levels inside each chunk and staging capacity
-    std::vector<unsigned char> sub(R, 0), subr(R, 0);
-    std::vector<int> nsub(C, 1), nsubr(C, 1);
-    for (int c = 0; c < C; ++c) {
-        const int p0 = c * 64, p1 = std::min(p0 + 64, R);
-        if (up_ptr[p1] - up_ptr[p0] > kStageMax || dn_ptr[p1] - dn_ptr[p0] > kStageMax) {
-            delete pl;
-            return fail(DDRS_ERR_CAPACITY, "a 64-reach chunk has more than " +
-                                               std::to_string(kStageMax) +
-                                               " upstream or downstream links");
+    // chunkings: sub-levels inside each chunk and the LDS staging capacity
+    for (int wi = 0; wi < kNumWidths; ++wi) {
+        const int rpl = 1 << wi, cw = 64 * rpl;
+        ChunkHost& ch = pl->ch[wi];
+        const int C = (wi == 0) ? pl->C : P / cw;
+        ch.C = C;
+        ch.ok = true;
+        for (int c = 0; c < C && ch.ok; ++c) {
+            const int p0 = c * cw, p1 = p0 + cw;
+            if (up_ptr[p1] - up_ptr[p0] > kStageUpBase * rpl ||
+                dn_ptr[p1] - dn_ptr[p0] > kStageDnBase * rpl)
+                ch.ok = false;
         }
-        int mx = 0;
-        for (int p = p0; p < p1; ++p) {
-            int s = 0;
-            for (int e = up_ptr[p]; e < up_ptr[p + 1]; ++e)
-                if (up_pos[e] >= p0) s = std::max(s, sub[up_pos[e]] + 1);
-            sub[p] = static_cast<unsigned char>(s);
-            mx = std::max(mx, s);
+        if (!ch.ok) {
+            if (wi == 0) {
+                delete pl;
+                return fail(DDRS_ERR_CAPACITY,
+                            "a 64-reach chunk has more than " + std::to_string(kStageUpBase) +
+                                " upstream or " + std::to_string(kStageDnBase) +
+                                " downstream links");
+            }
+            continue;
         }
-        nsub[c] = mx + 1;
-        mx = 0;
-        for (int p = p1 - 1; p >= p0; --p) {
-            int s = 0;
-            for (int e = dn_ptr[p]; e < dn_ptr[p + 1]; ++e)
-                if (dn_pos[e] < p1) s = std::max(s, subr[dn_pos[e]] + 1);
-            subr[p] = static_cast<unsigned char>(s);
-            mx = std::max(mx, s);
+        ch.sub.assign(P, 0);
+        ch.subr.assign(P, 0);
+        ch.nsub.assign(C, 1);
+        ch.nsubr.assign(C, 1);
+        for (int c = 0; c < C; ++c) {
+            const int p0 = c * cw, p1 = p0 + cw;
+            int mx = 0;
+            for (int p = p0; p < p1; ++p) {
+                int sl = 0;
+                for (int e = up_ptr[p]; e < up_ptr[p + 1]; ++e)
+                    if (up_pos[e] >= p0) sl = std::max(sl, ch.sub[up_pos[e]] + 1);
+                ch.sub[p] = static_cast<unsigned char>(sl);
+                mx = std::max(mx, sl);
+            }
+            ch.nsub[c] = mx + 1;
+            mx = 0;
+            for (int p = p1 - 1; p >= p0; --p) {
+                int sl = 0;
+                for (int e = dn_ptr[p]; e < dn_ptr[p + 1]; ++e)
+                    if (dn_pos[e] < p1) sl = std::max(sl, ch.subr[dn_pos[e]] + 1);
+                ch.subr[p] = static_cast<unsigned char>(sl);
+                mx = std::max(mx, sl);
+            }
+            ch.nsubr[c] = mx + 1;
         }
-        nsubr[c] = mx + 1;
     }
 
     pl->h_up_ptr.swap(up_ptr);
@@ -297,10 +361,6 @@ ddrs_status ddrs_plan_create(int32_t n, int32_t nnz_off, const int32_t* h_rows,
     pl->h_dn_adj.swap(dn_adj);
     pl->h_dn_k.swap(dn_k);
     pl->h_diag_k.swap(diag_k);
-    pl->h_sub.swap(sub);
-    pl->h_subr.swap(subr);
-    pl->h_nsub.swap(nsub);
-    pl->h_nsubr.swap(nsubr);
     *out = pl;
     return DDRS_OK;
 }
@@ -341,7 +401,7 @@ ddrs_status ddrs_plan_get_csr(ddrs_plan_t plan, int32_t* crow, int32_t* col, int
 
 ddrs_status ddrs_plan_get_schedule(ddrs_plan_t plan, int32_t* h_perm, int32_t* h_level) {
     if (!plan) return fail(DDRS_ERR_INVALID, "plan is NULL");
-    if (h_perm) std::memcpy(h_perm, plan->perm.data(), plan->perm.size() * sizeof(int));
+    if (h_perm) std::memcpy(h_perm, plan->order.data(), plan->order.size() * sizeof(int));
     if (h_level) std::memcpy(h_level, plan->level.data(), plan->level.size() * sizeof(int));
     return DDRS_OK;
 }
@@ -356,11 +416,9 @@ ddrs_status plan_to_device(ddrs_plan* pl) {
     if (pl->on_device) return DDRS_OK;
     Sched& sc = pl->sc;
     sc.R = pl->R;
-    sc.C = pl->C;
-    int *d_perm, *d_up_ptr, *d_up_pos, *d_up_k, *d_dn_ptr, *d_dn_pos, *d_dn_k, *d_diag, *d_nsub,
-        *d_nsubr;
+    sc.P = pl->P;
+    int *d_perm, *d_up_ptr, *d_up_pos, *d_up_k, *d_dn_ptr, *d_dn_pos, *d_dn_k, *d_diag;
     float *d_up_adj, *d_dn_adj;
-    unsigned char *d_sub, *d_subr;
     hipError_t e = hipSuccess;
 #define UP(dp, hv)                               \
     if (e == hipSuccess) {                       \
@@ -377,12 +435,24 @@ ddrs_status plan_to_device(ddrs_plan* pl) {
     UP(d_dn_adj, pl->h_dn_adj);
     UP(d_dn_k, pl->h_dn_k);
     UP(d_diag, pl->h_diag_k);
-    UP(d_sub, pl->h_sub);
-    UP(d_subr, pl->h_subr);
-    UP(d_nsub, pl->h_nsub);
-    UP(d_nsubr, pl->h_nsubr);
     UP(pl->d_row_for_nnz, pl->row_for_nnz);
     UP(pl->d_col, pl->col);
+    for (int wi = 0; wi < kNumWidths; ++wi) {
+        ChunkHost& ch = pl->ch[wi];
+        if (!ch.ok) continue;
+        unsigned char *d_sub = nullptr, *d_subr = nullptr;
+        int *d_nsub = nullptr, *d_nsubr = nullptr;
+        UP(d_sub, ch.sub);
+        UP(d_subr, ch.subr);
+        UP(d_nsub, ch.nsub);
+        UP(d_nsubr, ch.nsubr);
+        ch.dev.C = ch.C;
+        ch.dev.shift = 6 + wi;
+        ch.dev.sub = d_sub;
+        ch.dev.subr = d_subr;
+        ch.dev.nsub = d_nsub;
+        ch.dev.nsubr = d_nsubr;
+    }
 #undef UP
     if (e != hipSuccess) return fail(DDRS_ERR_HIP, std::string("plan upload: ") + hipGetErrorString(e));
     sc.perm = d_perm;
@@ -395,10 +465,6 @@ ddrs_status plan_to_device(ddrs_plan* pl) {
     sc.dn_adj = d_dn_adj;
     sc.dn_k = d_dn_k;
     sc.diag_k = d_diag;
-    sc.sub = d_sub;
-    sc.subr = d_subr;
-    sc.nsub = d_nsub;
-    sc.nsubr = d_nsubr;
     pl->on_device = true;
     return DDRS_OK;
 }
@@ -416,6 +482,7 @@ ddrs_status ddrs_workspace_create(ddrs_plan_t plan, int32_t window, int32_t max_
     *out = nullptr;
     if (!plan) return fail(DDRS_ERR_INVALID, "plan is NULL");
     if (window < 1 || max_steps < 0) return fail(DDRS_ERR_INVALID, "window must be >= 1");
+    if (window > 65535) return fail(DDRS_ERR_INVALID, "window must be <= 65535");
     ddrs_status pst = plan_to_device(plan);
     if (pst != DDRS_OK) return pst;
     auto* ws = new ddrs_workspace();
@@ -423,42 +490,38 @@ ddrs_status ddrs_workspace_create(ddrs_plan_t plan, int32_t window, int32_t max_
     ws->window = window;
     ws->max_steps = max_steps;
     ws->max_seg = std::max(1, (max_steps + window - 1) / window);
-    const size_t R = plan->R;
+    const size_t R = plan->P;  // position-indexed buffers
+    const size_t WR = static_cast<size_t>(window) * R;
     ddrs_status st = DDRS_OK;
-    auto F = [&](float** p, size_t n) {
-        if (st == DDRS_OK) st = dalloc(ws, reinterpret_cast<void**>(p), n * sizeof(float));
+    auto A = [&](auto** p, size_t bytes) {
+        if (st == DDRS_OK) st = dalloc(ws, reinterpret_cast<void**>(p), bytes);
     };
-    F(&ws->st.n, R);
-    F(&ws->st.qsp, R);
-    F(&ws->st.p, R);
-    F(&ws->st.len, R);
-    F(&ws->st.slope, R);
-    F(&ws->st.sqrt_s, R);
-    F(&ws->st.xst, R);
-    F(&ws->st_q, R);
-    F(&ws->st_i, R);
-    F(&ws->st_lam, R);
-    F(&ws->acc_n, R);
-    F(&ws->acc_qsp, R);
-    F(&ws->acc_p, R);
-    F(&ws->sc_q, R);
-    F(&ws->sc_i, R);
-    F(&ws->sc_qp, R);
-    F(&ws->ck, static_cast<size_t>(ws->max_seg) * 2 * R);
-    if (st == DDRS_OK)
-        st = dalloc(ws, reinterpret_cast<void**>(&ws->hist), static_cast<size_t>(window) * R * 8);
-    if (st == DDRS_OK)
-        st = dalloc(ws, reinterpret_cast<void**>(&ws->amsg), static_cast<size_t>(window) * R * 16);
-    if (st == DDRS_OK) st = dalloc(ws, reinterpret_cast<void**>(&ws->slot), R * 8);
-    if (st == DDRS_OK) st = dalloc(ws, reinterpret_cast<void**>(&ws->err), sizeof(int));
+    A(&ws->st, R * 3 * sizeof(float4));
+    A(&ws->st_q, R * 4);
+    A(&ws->st_i, R * 4);
+    A(&ws->st_lam, R * 4);
+    A(&ws->acc_n, R * 4);
+    A(&ws->acc_qsp, R * 4);
+    A(&ws->acc_p, R * 4);
+    A(&ws->ck, static_cast<size_t>(ws->max_seg) * 2 * R * 4);
+    A(&ws->hist, WR * 8);
+    A(&ws->amsg, WR * 16);
+    A(&ws->rec, WR * sizeof(float4));
+    A(&ws->recj, WR * sizeof(float4));
+    A(&ws->recm, WR);
+    A(&ws->gbuf, WR * 4);
+    A(&ws->slot, R * 8);
+    A(&ws->claim_ctr, 8);
+    A(&ws->err, sizeof(int));
     if (st != DDRS_OK) {
         ddrs_workspace_destroy(ws);
         return st;
     }
     // tags start at epoch 0 = "never written"
-    hipError_t e = hipMemset(ws->hist, 0, static_cast<size_t>(window) * R * 8);
-    if (e == hipSuccess) e = hipMemset(ws->amsg, 0, static_cast<size_t>(window) * R * 16);
+    hipError_t e = hipMemset(ws->hist, 0, WR * 8);
+    if (e == hipSuccess) e = hipMemset(ws->amsg, 0, WR * 16);
     if (e == hipSuccess) e = hipMemset(ws->slot, 0, R * 8);
+    if (e == hipSuccess) e = hipMemset(ws->claim_ctr, 0, 8);
     if (e == hipSuccess) e = hipMemset(ws->err, 0, sizeof(int));
     if (e != hipSuccess) {
         ddrs_workspace_destroy(ws);
@@ -467,27 +530,30 @@ ddrs_status ddrs_workspace_create(ddrs_plan_t plan, int32_t window, int32_t max_
     int dev = 0, ncu = 0;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    auto grid = [&](int which) -> int {
-        int nb = occupancy_blocks(which);
-        long cap = static_cast<long>(nb) * ncu;
-        return static_cast<int>(std::min<long>(cap, plan->C));
+    // Chunk width of a persistent kernel: the narrowest whose chunks can all be
+    // resident at once; otherwise the one with the least work per resident
+    // workgroup. Any width is correct (chunks are claimed in dependency order);
+    // this only decides how much of the launch runs concurrently.
+    auto pick = [&](int kernel, int variant, int* grid, int* rpl) {
+        long best = -1;
+        for (int wi = 0; wi < kNumWidths; ++wi) {
+            const ChunkHost& ch = plan->ch[wi];
+            if (!ch.ok) continue;
+            const int r = 1 << wi;
+            const long cap = std::max(1L, static_cast<long>(occupancy_blocks(kernel, variant, r)) * ncu);
+            const long m = (ch.C + cap - 1) / cap;
+            const long cost = m * r;
+            if (best < 0 || cost < best || m == 1) {
+                best = cost;
+                *rpl = r;
+                *grid = ch.C;
+            }
+            if (m == 1) break;
+        }
     };
-    for (int v = 0; v < 3; ++v) {
-        ws->grid_fwd[v] = grid(v);
-        ws->grid_adj[v] = grid(3 + v);
-    }
-    ws->grid_tri_fwd = grid(6);
-    ws->grid_tri_adj = grid(7);
-    int gmin = std::min({ws->grid_fwd[0], ws->grid_fwd[1], ws->grid_fwd[2], ws->grid_adj[0],
-                         ws->grid_adj[1], ws->grid_adj[2], ws->grid_tri_fwd, ws->grid_tri_adj});
-    if (gmin <= 0) {
-        ddrs_workspace_destroy(ws);
-        return fail(DDRS_ERR_HIP, "occupancy query returned no resident workgroups");
-    }
-    if ((plan->C + gmin - 1) / gmin > kMaxChunksPerWave) {
-        ddrs_workspace_destroy(ws);
-        return fail(DDRS_ERR_CAPACITY, "network too large for one GPU's resident workgroups");
-    }
+    for (int v = 0; v < 3; ++v) pick(OCC_FWD, v, &ws->grid_fwd[v], &ws->rpl_fwd[v]);
+    pick(OCC_SWEEP, 0, &ws->grid_sweep, &ws->rpl_sweep);
+    ws->grid_tri_fwd = ws->grid_tri_adj = plan->ch[0].C;
     *out = ws;
     return DDRS_OK;
 }
@@ -495,8 +561,8 @@ ddrs_status ddrs_workspace_create(ddrs_plan_t plan, int32_t window, int32_t max_
 void ddrs_workspace_destroy(ddrs_workspace_t ws) {
     if (!ws) return;
     for (void* p : ws->dev) hipFree(p);
-    for (auto e : ws->ev_fwd) hipEventDestroy(e);
-    for (auto e : ws->ev_adj) hipEventDestroy(e);
+    for (auto& pool : ws->ev)
+        for (auto e : pool) hipEventDestroy(e);
     delete ws;
 }
 
@@ -529,8 +595,8 @@ ddrs_status ddrs_mc_bind_statics(ddrs_workspace_t ws, const ddrs_params* params,
     if (st != DDRS_OK) return st;
     if (!d_n || !d_q_spatial || !d_p_spatial || !d_length || !d_slope)
         return fail(DDRS_ERR_INVALID, "statics must not be NULL (x_storage may be)");
-    launch_bind_statics(ws->plan->R, ws->plan->sc.perm, d_n, d_q_spatial, d_p_spatial, d_length,
-                        d_slope, d_x_storage, params->slope_lb, ws->st, S(stream));
+    launch_bind_statics(ws->plan->sc, d_n, d_q_spatial, d_p_spatial, d_length, d_slope,
+                        d_x_storage, params->slope_lb, ws->st, S(stream));
     HIPCHK(hipGetLastError());
     ws->statics_bound = true;
     return DDRS_OK;
@@ -541,7 +607,9 @@ ddrs_status ddrs_mc_bind_statics(ddrs_workspace_t ws, const ddrs_params* params,
 
 static FwdArgs fwd_args(ddrs_workspace* ws, const ddrs_params* p) {
     FwdArgs a{};
+    const int v = variant(p);
     a.sc = ws->plan->sc;
+    a.ch = ws->plan->ch[ws->rpl_fwd[v] == 2 ? 1 : 0].dev;
     a.st = ws->st;
     a.P = to_phys(p);
     a.hist = ws->hist;
@@ -551,22 +619,90 @@ static FwdArgs fwd_args(ddrs_workspace* ws, const ddrs_params* p) {
     return a;
 }
 
-static AdjArgs adj_args(ddrs_workspace* ws, const ddrs_params* p) {
-    AdjArgs a{};
-    a.sc = ws->plan->sc;
-    a.st = ws->st;
-    a.P = to_phys(p);
-    a.hist = ws->hist;
-    a.amsg = ws->amsg;
-    a.st_lam = ws->st_lam;
-    a.acc_n = ws->acc_n;
-    a.acc_qsp = ws->acc_qsp;
-    a.acc_p = ws->acc_p;
-    a.sc_q = ws->sc_q;
-    a.sc_i = ws->sc_i;
-    a.sc_qp = ws->sc_qp;
-    a.err = ws->err;
-    return a;
+// Claims of the next persistent launch of `grid` workgroups.
+static Claim next_claim(ddrs_workspace* ws, int grid) {
+    Claim c{ws->claim_ctr, ws->claims};
+    ws->claims += static_cast<unsigned long long>(grid);
+    return c;
+}
+
+static void launch_fwd_ws(ddrs_workspace* ws, FwdArgs a, const ddrs_params* p, hipStream_t s) {
+    const int v = variant(p);
+    a.claim = next_claim(ws, ws->grid_fwd[v]);
+    launch_fwd(a, v, ws->rpl_fwd[v], ws->grid_fwd[v], s);
+}
+
+// The three adjoint phases over window steps [s_base, s_base + nsteps) of the
+// segment starting at t0, in one launch each (plain form: per step).
+struct AdjSegment {
+    int t0 = 0, len = 0;
+    const float* qprime = nullptr;
+    const float* gQ = nullptr;
+    float* gqp_out = nullptr;
+    int mask_qp = 1, mask_lam = 1;
+    const float* ck_q = nullptr;
+    const float* ck_i = nullptr;
+};
+
+static void run_adj_segment(ddrs_workspace* ws, const ddrs_params* p, const AdjSegment& g,
+                            ddrs_form form, hipStream_t s) {
+    const Sched& sc = ws->plan->sc;
+    LinArgs L{};
+    L.sc = sc;
+    L.st = ws->st;
+    L.P = to_phys(p);
+    L.t0 = g.t0;
+    L.qprime = g.qprime;
+    L.gQ = g.gQ;
+    L.mask_qp = g.mask_qp;
+    L.mask_lam = g.mask_lam;
+    L.hist = ws->hist;
+    L.ck_q = g.ck_q;
+    L.ck_i = g.ck_i;
+    L.rec = ws->rec;
+    L.recm = ws->recm;
+    L.recj = ws->recj;
+    SweepArgs W{};
+    W.sc = sc;
+    W.ch = ws->plan->ch[ws->rpl_sweep == 2 ? 1 : 0].dev;
+    W.rec = ws->rec;
+    W.recm = ws->recm;
+    W.amsg = ws->amsg;
+    W.st_lam = ws->st_lam;
+    W.gbuf = ws->gbuf;
+    W.err = ws->err;
+    FinArgs F{};
+    F.R = sc.R;
+    F.P = sc.P;
+    F.perm = sc.perm;
+    F.t0 = g.t0;
+    F.recj = ws->recj;
+    F.gbuf = ws->gbuf;
+    F.acc_n = ws->acc_n;
+    F.acc_qsp = ws->acc_qsp;
+    F.acc_p = ws->acc_p;
+    F.gqp_out = g.gqp_out;
+    const int v = variant(p);
+    auto phases = [&](int s_base, int n) {
+        L.s_base = W.s_base = F.s_base = s_base;
+        L.nsteps = W.nsteps = F.nsteps = n;
+        W.epoch = ++ws->epoch;
+        W.claim = next_claim(ws, ws->grid_sweep);
+        record(ws, TK_LIN, s);
+        launch_lin(L, v, s);
+        record(ws, TK_LIN, s);
+        record(ws, TK_SWEEP, s);
+        launch_sweep(W, ws->rpl_sweep, ws->grid_sweep, s);
+        record(ws, TK_SWEEP, s);
+        record(ws, TK_FIN, s);
+        launch_fin(F, s);
+        record(ws, TK_FIN, s);
+    };
+    if (form == DDRS_FORM_PERSISTENT) {
+        phases(0, g.len);
+    } else {
+        for (int k = g.len - 1; k >= 0; --k) phases(k, 1);
+    }
 }
 
 ddrs_status ddrs_mc_step_fwd(ddrs_workspace_t ws, const ddrs_params* params, const float* d_q_t,
@@ -587,9 +723,9 @@ ddrs_status ddrs_mc_step_fwd(ddrs_workspace_t ws, const ddrs_params* params, con
     a.qprime = d_q_prime_t;
     a.X_out = d_x_sol;
     a.Q_out = nullptr;
-    launch_fwd(a, params, ws->grid_fwd[variant(params)], s);
+    launch_fwd_ws(ws, a, params, s);
     HIPCHK(hipGetLastError());
-    launch_unpermute(ws->plan->R, ws->plan->sc.perm, ws->st_q, d_q_next, s);
+    launch_unpermute(ws->plan->sc, ws->st_q, d_q_next, s);
     HIPCHK(hipGetLastError());
     ws->last_T = -1;
     return DDRS_OK;
@@ -608,27 +744,25 @@ ddrs_status ddrs_mc_step_bwd(ddrs_workspace_t ws, const ddrs_params* params, con
         return fail(DDRS_ERR_INVALID, "NULL input");
     hipStream_t s = S(stream);
     const Sched& sc = ws->plan->sc;
-    const int R = sc.R;
     launch_state(sc, d_q_t, 0, params->discharge_lb, ws->st_q, ws->st_i, nullptr, s);
-    launch_hist_from_x(R, sc.perm, d_x_sol, ws->hist, s);
-    launch_lam_init(R, sc.perm, d_grad_q_next, ws->st_lam, ws->acc_n, ws->acc_qsp, ws->acc_p, s);
-    AdjArgs a = adj_args(ws, params);
-    a.t0 = 0;
-    a.s_base = 0;
-    a.nsteps = 1;
-    a.epoch = ++ws->epoch;
-    a.qprime = d_q_prime_t;
-    a.gQ = nullptr;
-    a.gqp_out = d_grad_q_prime_t;  // row 0, c4 * gb without the q' clamp mask
-    a.mask_qp = 0;
-    a.ck_q = ws->st_q;
-    a.ck_i = ws->st_i;
-    launch_adj(a, params, ws->grid_adj[variant(params)], s);
+    launch_hist_from_x(sc, d_x_sol, ws->hist, s);
+    launch_lam_init(sc, d_grad_q_next, ws->hist, params->discharge_lb, ws->st_lam, ws->acc_n,
+                    ws->acc_qsp, ws->acc_p, s);
+    AdjSegment g;
+    g.t0 = 0;
+    g.len = 1;
+    g.qprime = d_q_prime_t;
+    g.gqp_out = d_grad_q_prime_t;  // row 0: c4 * gb without the q' clamp mask
+    g.mask_qp = 0;
+    g.mask_lam = 0;                // grad_q_t is the gradient of the raw q_t input
+    g.ck_q = ws->st_q;
+    g.ck_i = ws->st_i;
+    run_adj_segment(ws, params, g, DDRS_FORM_PERSISTENT, s);
     HIPCHK(hipGetLastError());
-    if (d_grad_n) launch_unpermute(R, sc.perm, ws->acc_n, d_grad_n, s);
-    if (d_grad_q_spatial) launch_unpermute(R, sc.perm, ws->acc_qsp, d_grad_q_spatial, s);
-    if (d_grad_p_spatial) launch_unpermute(R, sc.perm, ws->acc_p, d_grad_p_spatial, s);
-    if (d_grad_q_t) launch_unpermute(R, sc.perm, ws->st_lam, d_grad_q_t, s);
+    if (d_grad_n) launch_unpermute(sc, ws->acc_n, d_grad_n, s);
+    if (d_grad_q_spatial) launch_unpermute(sc, ws->acc_qsp, d_grad_q_spatial, s);
+    if (d_grad_p_spatial) launch_unpermute(sc, ws->acc_p, d_grad_p_spatial, s);
+    if (d_grad_q_t) launch_unpermute(sc, ws->st_lam, d_grad_q_t, s);
     HIPCHK(hipGetLastError());
     ws->last_T = -1;
     return DDRS_OK;
@@ -641,34 +775,36 @@ ddrs_status ddrs_mc_step_bwd(ddrs_workspace_t ws, const ddrs_params* params, con
 
 namespace {
 
+int seg_len(ddrs_workspace* ws, int T, int seg) {
+    return std::min(ws->window, (T - 1) - seg * ws->window);
+}
+
 // Forward of checkpoint segment `seg` (steps [seg*window, ...)) from the state
 // currently in st_q / st_i; writes the segment's solve history.
 ddrs_status run_fwd_segment(ddrs_workspace* ws, const ddrs_params* params, int T, int seg,
                             const float* qprime, float* Q_out, ddrs_form form, hipStream_t s,
                             bool timed) {
-    const int steps = T - 1;
     const int t0 = seg * ws->window;
-    const int len = std::min(ws->window, steps - t0);
+    const int len = seg_len(ws, T, seg);
     FwdArgs a = fwd_args(ws, params);
     a.t0 = t0;
     a.qprime = qprime;
     a.Q_out = Q_out;
-    const int grid = ws->grid_fwd[variant(params)];
     if (form == DDRS_FORM_PERSISTENT) {
         a.s_base = 0;
         a.nsteps = len;
         a.epoch = ++ws->epoch;
-        if (timed) record(ws, ws->ev_fwd, ws->n_fwd, s);
-        launch_fwd(a, params, grid, s);
-        if (timed) record(ws, ws->ev_fwd, ws->n_fwd, s);
+        if (timed) record(ws, TK_FWD, s);
+        launch_fwd_ws(ws, a, params, s);
+        if (timed) record(ws, TK_FWD, s);
     } else {
         for (int k = 0; k < len; ++k) {
             a.s_base = k;
             a.nsteps = 1;
             a.epoch = ++ws->epoch;
-            if (timed) record(ws, ws->ev_fwd, ws->n_fwd, s);
-            launch_fwd(a, params, grid, s);
-            if (timed) record(ws, ws->ev_fwd, ws->n_fwd, s);
+            if (timed) record(ws, TK_FWD, s);
+            launch_fwd_ws(ws, a, params, s);
+            if (timed) record(ws, TK_FWD, s);
         }
     }
     HIPCHK(hipGetLastError());
@@ -676,8 +812,8 @@ ddrs_status run_fwd_segment(ddrs_workspace* ws, const ddrs_params* params, int T
     return DDRS_OK;
 }
 
-float* ck_q(ddrs_workspace* ws, int seg) { return ws->ck + static_cast<size_t>(seg) * 2 * ws->plan->R; }
-float* ck_i(ddrs_workspace* ws, int seg) { return ck_q(ws, seg) + ws->plan->R; }
+float* ck_q(ddrs_workspace* ws, int seg) { return ws->ck + static_cast<size_t>(seg) * 2 * ws->plan->P; }
+float* ck_i(ddrs_workspace* ws, int seg) { return ck_q(ws, seg) + ws->plan->P; }
 
 }  // namespace
 
@@ -697,14 +833,14 @@ ddrs_status ddrs_mc_window_fwd(ddrs_workspace_t ws, const ddrs_params* params, i
     if (form != DDRS_FORM_PLAIN && form != DDRS_FORM_PERSISTENT)
         return fail(DDRS_ERR_INVALID, "unknown form");
     hipStream_t s = S(stream);
-    const int R = ws->plan->R;
-    ws->n_fwd = 0;
+    const int P = ws->plan->P;
+    ws->n_ev[TK_FWD] = 0;
     launch_state(ws->plan->sc, d_q0, 1, params->discharge_lb, ws->st_q, ws->st_i, d_Q_out, s);
     const int steps = T - 1;
     const int nseg = (steps + ws->window - 1) / ws->window;
     for (int j = 0; j < nseg; ++j) {
-        HIPCHK(hipMemcpyAsync(ck_q(ws, j), ws->st_q, R * sizeof(float), hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipMemcpyAsync(ck_i(ws, j), ws->st_i, R * sizeof(float), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(ck_q(ws, j), ws->st_q, P * sizeof(float), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(ck_i(ws, j), ws->st_i, P * sizeof(float), hipMemcpyDeviceToDevice, s));
         st = run_fwd_segment(ws, params, T, j, d_q_prime, d_Q_out, form, s, true);
         if (st != DDRS_OK) return st;
     }
@@ -728,54 +864,47 @@ ddrs_status ddrs_mc_window_adj(ddrs_workspace_t ws, const ddrs_params* params, i
         return fail(DDRS_ERR_INVALID, "unknown form");
     hipStream_t s = S(stream);
     const Sched& sc = ws->plan->sc;
-    const int R = sc.R;
-    ws->n_adj = 0;
-    const float* g_last = d_grad_Q ? d_grad_Q + static_cast<size_t>(T - 1) * R : nullptr;
-    launch_lam_init(R, sc.perm, g_last, ws->st_lam, ws->acc_n, ws->acc_qsp, ws->acc_p, s);
-    if (d_grad_q_prime) launch_zero(static_cast<size_t>(R), d_grad_q_prime + static_cast<size_t>(T - 1) * R, s);
+    const int R = sc.R, P = sc.P;
+    ws->n_ev[TK_LIN] = ws->n_ev[TK_SWEEP] = ws->n_ev[TK_FIN] = 0;
     const int steps = T - 1;
     const int nseg = (steps + ws->window - 1) / ws->window;
-    const int grid = ws->grid_adj[variant(params)];
+    const float* g_last = d_grad_Q ? d_grad_Q + static_cast<size_t>(T - 1) * R : nullptr;
+    if (d_grad_q_prime) launch_zero(static_cast<size_t>(R), d_grad_q_prime + static_cast<size_t>(T - 1) * R, s);
+    if (nseg == 0) {
+        // T == 1: out[0] = max(Q0, lb) only
+        launch_lam_init(sc, g_last, nullptr, params->discharge_lb, ws->st_lam, ws->acc_n,
+                        ws->acc_qsp, ws->acc_p, s);
+    }
     for (int j = nseg - 1; j >= 0; --j) {
         if (ws->hist_seg != j) {
-            HIPCHK(hipMemcpyAsync(ws->st_q, ck_q(ws, j), R * sizeof(float), hipMemcpyDeviceToDevice, s));
-            HIPCHK(hipMemcpyAsync(ws->st_i, ck_i(ws, j), R * sizeof(float), hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipMemcpyAsync(ws->st_q, ck_q(ws, j), P * sizeof(float), hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipMemcpyAsync(ws->st_i, ck_i(ws, j), P * sizeof(float), hipMemcpyDeviceToDevice, s));
             st = run_fwd_segment(ws, params, T, j, d_q_prime, nullptr, DDRS_FORM_PERSISTENT, s, false);
             if (st != DDRS_OK) return st;
         }
-        const int t0 = j * ws->window;
-        const int len = std::min(ws->window, steps - t0);
-        AdjArgs a = adj_args(ws, params);
-        a.t0 = t0;
-        a.qprime = d_q_prime;
-        a.gQ = d_grad_Q;
-        a.gqp_out = d_grad_q_prime;
-        a.mask_qp = 1;
-        a.ck_q = ck_q(ws, j);
-        a.ck_i = ck_i(ws, j);
-        if (form == DDRS_FORM_PERSISTENT) {
-            a.s_base = 0;
-            a.nsteps = len;
-            a.epoch = ++ws->epoch;
-            record(ws, ws->ev_adj, ws->n_adj, s);
-            launch_adj(a, params, grid, s);
-            record(ws, ws->ev_adj, ws->n_adj, s);
-        } else {
-            for (int k = len - 1; k >= 0; --k) {
-                a.s_base = k;
-                a.nsteps = 1;
-                a.epoch = ++ws->epoch;
-                record(ws, ws->ev_adj, ws->n_adj, s);
-                launch_adj(a, params, grid, s);
-                record(ws, ws->ev_adj, ws->n_adj, s);
-            }
+        const int len = seg_len(ws, T, j);
+        if (j == nseg - 1) {
+            // dL/d x of the last step through out[T-1] = max(x, lb)
+            launch_lam_init(sc, g_last, ws->hist + static_cast<size_t>(len - 1) * P,
+                            params->discharge_lb, ws->st_lam, ws->acc_n, ws->acc_qsp, ws->acc_p, s);
         }
+        AdjSegment g;
+        g.t0 = j * ws->window;
+        g.len = len;
+        g.qprime = d_q_prime;
+        g.gQ = d_grad_Q;
+        g.gqp_out = d_grad_q_prime;
+        g.mask_qp = 1;
+        g.mask_lam = 1;
+        g.ck_q = ck_q(ws, j);
+        g.ck_i = ck_i(ws, j);
+        run_adj_segment(ws, params, g, form, s);
         HIPCHK(hipGetLastError());
     }
-    if (d_grad_q0) launch_grad_q0(R, sc.perm, ws->st_lam, d_q0, params->discharge_lb, d_grad_q0, s);
-    if (d_grad_n) launch_unpermute(R, sc.perm, ws->acc_n, d_grad_n, s);
-    if (d_grad_q_spatial) launch_unpermute(R, sc.perm, ws->acc_qsp, d_grad_q_spatial, s);
-    if (d_grad_p_spatial) launch_unpermute(R, sc.perm, ws->acc_p, d_grad_p_spatial, s);
+    if (d_grad_q0) launch_grad_q0(sc, ws->st_lam, d_q0, params->discharge_lb, d_grad_q0, s);
+    if (d_grad_n) launch_unpermute(sc, ws->acc_n, d_grad_n, s);
+    if (d_grad_q_spatial) launch_unpermute(sc, ws->acc_qsp, d_grad_q_spatial, s);
+    if (d_grad_p_spatial) launch_unpermute(sc, ws->acc_p, d_grad_p_spatial, s);
     HIPCHK(hipGetLastError());
     return DDRS_OK;
 }
@@ -789,6 +918,8 @@ ddrs_status ddrs_tri_solve_fwd(ddrs_workspace_t ws, const float* d_a_values, con
     if (!d_a_values || !d_b || !d_x) return fail(DDRS_ERR_INVALID, "NULL argument");
     TriArgs a{};
     a.sc = ws->plan->sc;
+    a.ch = ws->plan->ch[0].dev;
+    a.claim = next_claim(ws, ws->plan->ch[0].C);
     a.epoch = ++ws->epoch;
     a.a_values = d_a_values;
     a.b = d_b;
@@ -809,6 +940,8 @@ ddrs_status ddrs_tri_solve_adj(ddrs_workspace_t ws, const float* d_a_values, con
     hipStream_t s = S(stream);
     TriArgs a{};
     a.sc = ws->plan->sc;
+    a.ch = ws->plan->ch[0].dev;
+    a.claim = next_claim(ws, ws->plan->ch[0].C);
     a.epoch = ++ws->epoch;
     a.a_values = d_a_values;
     a.b = d_grad_x;
@@ -833,6 +966,8 @@ ddrs_status ddrs_hotstart(ddrs_workspace_t ws, const ddrs_params* params,
     if (!d_q_prime0 || !d_q0) return fail(DDRS_ERR_INVALID, "NULL argument");
     TriArgs a{};
     a.sc = ws->plan->sc;
+    a.ch = ws->plan->ch[0].dev;
+    a.claim = next_claim(ws, ws->plan->ch[0].C);
     a.epoch = ++ws->epoch;
     a.a_values = nullptr;  // A = I - N
     a.b = d_q_prime0;
@@ -854,6 +989,8 @@ ddrs_status ddrs_hotstart_adj(ddrs_workspace_t ws, const ddrs_params* params, co
     if (!d_q0 || !d_grad_q0 || !d_grad_q_prime0) return fail(DDRS_ERR_INVALID, "NULL argument");
     TriArgs a{};
     a.sc = ws->plan->sc;
+    a.ch = ws->plan->ch[0].dev;
+    a.claim = next_claim(ws, ws->plan->ch[0].C);
     a.epoch = ++ws->epoch;
     a.a_values = nullptr;
     a.b = d_grad_q0;
@@ -873,30 +1010,84 @@ ddrs_status ddrs_hotstart_adj(ddrs_workspace_t ws, const ddrs_params* params, co
 ddrs_status ddrs_set_timing(ddrs_workspace_t ws, int32_t enable) {
     if (!ws) return fail(DDRS_ERR_INVALID, "workspace is NULL");
     ws->timing = enable != 0;
-    ws->n_fwd = ws->n_adj = 0;
+    for (int& n : ws->n_ev) n = 0;
     return DDRS_OK;
 }
+
+namespace {
+ddrs_status sum_pairs(ddrs_workspace* ws, int kind, int32_t* launches, float* ms) {
+    std::vector<hipEvent_t>& pool = ws->ev[kind];
+    const int n = ws->n_ev[kind];
+    float tot = 0.0f;
+    if (n > 0) HIPCHK(hipEventSynchronize(pool[n - 1]));
+    for (int k = 0; k + 1 < n; k += 2) {
+        float v = 0.0f;
+        HIPCHK(hipEventElapsedTime(&v, pool[k], pool[k + 1]));
+        tot += v;
+    }
+    if (ms) *ms = tot;
+    if (launches) *launches = n / 2;
+    return DDRS_OK;
+}
+}  // namespace
 
 ddrs_status ddrs_get_timing(ddrs_workspace_t ws, int32_t* fwd_launches, float* fwd_ms,
                             int32_t* adj_launches, float* adj_ms) {
     if (!ws) return fail(DDRS_ERR_INVALID, "workspace is NULL");
-    auto sum = [&](std::vector<hipEvent_t>& pool, int n, float* ms) -> ddrs_status {
-        float tot = 0.0f;
-        if (n > 0) HIPCHK(hipEventSynchronize(pool[n - 1]));
-        for (int k = 0; k + 1 < n; k += 2) {
-            float v = 0.0f;
-            HIPCHK(hipEventElapsedTime(&v, pool[k], pool[k + 1]));
-            tot += v;
-        }
-        if (ms) *ms = tot;
-        return DDRS_OK;
+    ddrs_status st = sum_pairs(ws, TK_FWD, fwd_launches, fwd_ms);
+    float tot = 0.0f;
+    int32_t nl = 0;
+    for (int k = TK_LIN; k <= TK_FIN && st == DDRS_OK; ++k) {
+        float v = 0.0f;
+        int32_t n = 0;
+        st = sum_pairs(ws, k, &n, &v);
+        tot += v;
+        nl += n;
+    }
+    if (st != DDRS_OK) return st;
+    if (adj_ms) *adj_ms = tot;
+    if (adj_launches) *adj_launches = nl;
+    return DDRS_OK;
+}
+
+ddrs_status ddrs_get_adj_phase_timing(ddrs_workspace_t ws, float* lin_ms, float* sweep_ms,
+                                      float* fin_ms) {
+    if (!ws) return fail(DDRS_ERR_INVALID, "workspace is NULL");
+    ddrs_status st = sum_pairs(ws, TK_LIN, nullptr, lin_ms);
+    if (st == DDRS_OK) st = sum_pairs(ws, TK_SWEEP, nullptr, sweep_ms);
+    if (st == DDRS_OK) st = sum_pairs(ws, TK_FIN, nullptr, fin_ms);
+    return st;
+}
+
+ddrs_status ddrs_workspace_set_chunk_width(ddrs_workspace_t ws, int32_t rpl_fwd,
+                                           int32_t rpl_sweep) {
+    if (!ws) return fail(DDRS_ERR_INVALID, "workspace is NULL");
+    auto ok = [&](int r) {
+        return r == 0 || ((r == 1 || r == 2) && ws->plan->ch[r == 2 ? 1 : 0].ok);
     };
-    ddrs_status st = sum(ws->ev_fwd, ws->n_fwd, fwd_ms);
-    if (st != DDRS_OK) return st;
-    st = sum(ws->ev_adj, ws->n_adj, adj_ms);
-    if (st != DDRS_OK) return st;
-    if (fwd_launches) *fwd_launches = ws->n_fwd / 2;
-    if (adj_launches) *adj_launches = ws->n_adj / 2;
+    if (!ok(rpl_fwd) || !ok(rpl_sweep))
+        return fail(DDRS_ERR_INVALID, "chunk width must be 0 (keep), 1 or 2 reaches per lane, "
+                                      "and available for this network");
+    if (rpl_fwd) {
+        for (int v = 0; v < 3; ++v) {
+            ws->rpl_fwd[v] = rpl_fwd;
+            ws->grid_fwd[v] = ws->plan->ch[rpl_fwd == 2 ? 1 : 0].C;
+        }
+    }
+    if (rpl_sweep) {
+        ws->rpl_sweep = rpl_sweep;
+        ws->grid_sweep = ws->plan->ch[rpl_sweep == 2 ? 1 : 0].C;
+    }
+    return DDRS_OK;
+}
+
+ddrs_status ddrs_workspace_layout(ddrs_workspace_t ws, int32_t* rpl_fwd, int32_t* grid_fwd,
+                                  int32_t* rpl_sweep, int32_t* grid_sweep) {
+    if (!ws) return fail(DDRS_ERR_INVALID, "workspace is NULL");
+    if (rpl_fwd) *rpl_fwd = ws->rpl_fwd[0];
+    if (grid_fwd) *grid_fwd = ws->grid_fwd[0];
+    if (rpl_sweep) *rpl_sweep = ws->rpl_sweep;
+    if (grid_sweep) *grid_sweep = ws->grid_sweep;
     return DDRS_OK;
 }
 

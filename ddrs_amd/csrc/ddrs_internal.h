@@ -12,44 +12,54 @@
 
 namespace ddrs {
 
-// Upstream (downstream) entries of one 64-reach chunk are staged in LDS.
-constexpr int kStageMax = 256;
-// Chunks one persistent workgroup may own (C <= kMaxChunksPerWave * grid).
-constexpr int kMaxChunksPerWave = 128;
+// Links of a chunk staged in LDS, per 64 reaches of chunk width: upstream
+// links (forward) and downstream links (adjoint; one per reach in a tree).
+constexpr int kStageUpBase = 192;
+constexpr int kStageDnBase = 128;
+// Chunk widths: 64 * RPL positions for RPL (reaches per lane) in {1, 2}.
+constexpr int kNumWidths = 2;
+// Longest chain of links inside a 64-position chunk (its sub-levels): every
+// time step walks it serially, so it bounds the step rate of the chunk.
+constexpr int kMaxSubLevels = 8;
 
-// Level-major schedule, device side. Position p in [0, R).
+// Level-major schedule, device side. Position p in [0, P); reach-indexed
+// (reference order) arrays have R entries.
 struct Sched {
-    int R, C;
-    const int* perm;          // [R] position -> reach id
-    const int* up_ptr;        // [R+1] upstream entries of position p
+    int R, P;
+    const int* perm;          // [P] position -> reach id, -1 for an empty position
+    const int* up_ptr;        // [P+1] upstream entries of position p
     const int* up_pos;        // [E] upstream position, reference CSR order
     const float* up_adj;      // [E] N value
     const int* up_k;          // [E] CsrPattern nonzero index
-    const int* dn_ptr;        // [R+1] downstream entries of position p
+    const int* dn_ptr;        // [P+1] downstream entries of position p
     const int* dn_pos;        // [E] downstream position, transposed-CSR order
     const float* dn_adj;      // [E]
     const int* dn_k;          // [E] CsrPattern nonzero index
-    const int* diag_k;        // [R] CsrPattern index of the diagonal
-    const unsigned char* sub;   // [R] forward sub-level inside the chunk
-    const unsigned char* subr;  // [R] reverse sub-level inside the chunk
-    const int* nsub;          // [C]
-    const int* nsubr;         // [C]
+    const int* diag_k;        // [P] CsrPattern index of the diagonal
 };
 
-// Per-reach statics in schedule order.
-struct Statics {
-    float* n;
-    float* qsp;
-    float* p;
-    float* len;
-    float* slope;   // clamped
-    float* sqrt_s;
-    float* xst;
+// The schedule cut into chunks of 64 << (shift - 6) consecutive positions; a
+// chunk is one workgroup of one wavefront.
+struct Chunks {
+    int C, shift;
+    const unsigned char* sub;   // [P] forward sub-level inside the chunk
+    const unsigned char* subr;  // [P] reverse sub-level inside the chunk
+    const int* nsub;            // [C]
+    const int* nsubr;           // [C]
+};
+
+// Chunk claims of a persistent launch: the workgroup that increments *ctr to
+// base + k takes the k-th chunk in dependency order.
+struct Claim {
+    unsigned long long* ctr;
+    unsigned long long base;
 };
 
 struct FwdArgs {
     Sched sc;
-    Statics st;
+    Chunks ch;
+    Claim claim;
+    const float4* st;      // [P][3] reach statics, schedule order
     Phys P;
     int t0;          // out[t0] is the window's initial state
     int s_base;      // window-local index of this launch's first step
@@ -58,68 +68,98 @@ struct FwdArgs {
     const float* qprime;   // [T][R] reference order
     float* Q_out;          // [T][R] reference order or null
     float* X_out;          // [R] pre-clamp solve (single step) or null
-    uint64_t* hist;        // [window][R] tagged x, schedule order
-    float* st_q;           // [R] out[t] of the current step
-    float* st_i;           // [R] N out[t]
+    uint64_t* hist;        // [window][P] tagged x, schedule order
+    float* st_q;           // [P] out[t] of the current step
+    float* st_i;           // [P] N out[t]
     int* err;
 };
 
-struct AdjArgs {
+// Adjoint, phase 1 (every reach-step of a segment independently): the step's
+// coefficients and its backward for gb = 1, from the forward history.
+struct LinArgs {
     Sched sc;
-    Statics st;
+    const float4* st;
     Phys P;
     int t0, s_base, nsteps;
-    uint32_t epoch;
     const float* qprime;   // [T][R]
     const float* gQ;       // [T][R] or null
-    float* gqp_out;        // [T][R] or null
     int mask_qp;           // 1: gradient w.r.t. raw q' through max(q', lb)
-    const uint64_t* hist;  // forward history of the window
-    const float* ck_q;     // out[t0] (schedule order)
-    const float* ck_i;     // N out[t0]
-    uint64_t* amsg;        // [window][R][2] tagged (P, G)
-    float* st_lam;         // [R] dL/d out[t+1] on entry of step t
+    int mask_lam;          // 1: dL/d out[t] is passed on through out[t] = max(x, lb)
+    const uint64_t* hist;  // forward history of the segment
+    const float* ck_q;     // [P] out[t0] (schedule order)
+    const float* ck_i;     // [P] N out[t0]
+    float4* rec;           // [window][P] {-c1, c2, d gq / d gb, dL/d out[t] direct}
+    unsigned char* recm;   // [window][P] mask of dL/d out[t]
+    float4* recj;          // [window][P] d(gn, gqsp, gp, gq') / d gb
+};
+
+// Adjoint, phase 2 (persistent, reverse time): the transposed solves.
+struct SweepArgs {
+    Sched sc;
+    Chunks ch;
+    Claim claim;
+    int s_base, nsteps;
+    uint32_t epoch;
+    const float4* rec;
+    const unsigned char* recm;
+    uint64_t* amsg;        // [window][P][2] tagged (P, G)
+    float* st_lam;         // [P] gradient into x of the step (masked dL/d out[t+1])
+    float* gbuf;           // [window][P] gb of every reach-step
+    int* err;
+};
+
+// Adjoint, phase 3: parameter and q' gradients from gb.
+struct FinArgs {
+    int R, P;
+    const int* perm;
+    int t0, s_base, nsteps;
+    const float4* recj;
+    const float* gbuf;
     float* acc_n;
     float* acc_qsp;
     float* acc_p;
-    float* sc_q;           // [R] out[t] of the step being processed (prepare -> solve)
-    float* sc_i;           // [R] N out[t]
-    float* sc_qp;          // [R] raw q' of the step
-    int* err;
+    float* gqp_out;        // [T][R] reference order or null
 };
 
 struct TriArgs {
     Sched sc;
+    Chunks ch;             // 64-wide
+    Claim claim;
     uint32_t epoch;
     const float* a_values;  // [nnz] or null (A = I - N)
     const float* b;         // [R] reference order
     const float* mask_x;    // [R] or null: gradient masked where mask_x <= lb
     float* x;               // [R] reference order
-    uint64_t* slot;         // [R]
+    uint64_t* slot;         // [P]
     int clamp;
     float lb;
     int* err;
 };
 
-void launch_fwd(const FwdArgs& a, const ddrs_params* p, int grid, hipStream_t s);
-void launch_adj(const AdjArgs& a, const ddrs_params* p, int grid, hipStream_t s);
+// Persistent launches run one workgroup per chunk (grid = Chunks::C).
+void launch_fwd(const FwdArgs& a, int variant, int rpl, int grid, hipStream_t s);
+void launch_lin(const LinArgs& a, int variant, hipStream_t s);
+void launch_sweep(const SweepArgs& a, int rpl, int grid, hipStream_t s);
+void launch_fin(const FinArgs& a, hipStream_t s);
 void launch_tri_fwd(const TriArgs& a, int grid, hipStream_t s);
 void launch_tri_adj(const TriArgs& a, int grid, hipStream_t s);
 void launch_grad_a(int nnz, const int* row, const int* col, const float* gb, const float* x,
                    float* ga, hipStream_t s);
-void launch_bind_statics(int R, const int* perm, const float* n, const float* qsp,
-                         const float* psp, const float* len, const float* slope,
-                         const float* xst, float slope_lb, const Statics& st, hipStream_t s);
+void launch_bind_statics(const Sched& sc, const float* n, const float* qsp, const float* psp,
+                         const float* len, const float* slope, const float* xst, float slope_lb,
+                         float4* st, hipStream_t s);
 void launch_state(const Sched& sc, const float* q, int clamp, float lb, float* st_q,
                   float* st_i, float* Q_row0, hipStream_t s);
-void launch_hist_from_x(int R, const int* perm, const float* x, uint64_t* hist, hipStream_t s);
-void launch_lam_init(int R, const int* perm, const float* g, float* lam, float* a0, float* a1,
-                     float* a2, hipStream_t s);
-void launch_unpermute(int R, const int* perm, const float* src, float* dst, hipStream_t s);
-void launch_grad_q0(int R, const int* perm, const float* lam, const float* q0, float lb,
-                    float* g, hipStream_t s);
+void launch_hist_from_x(const Sched& sc, const float* x, uint64_t* hist, hipStream_t s);
+void launch_lam_init(const Sched& sc, const float* g, const uint64_t* hist_row, float lb,
+                     float* lam, float* a0, float* a1, float* a2, hipStream_t s);
+void launch_unpermute(const Sched& sc, const float* src, float* dst, hipStream_t s);
+void launch_grad_q0(const Sched& sc, const float* lam, const float* q0, float lb, float* g,
+                    hipStream_t s);
 void launch_zero(size_t n, float* x, hipStream_t s);
-// which: 0..2 forward (default, ddr, pos), 3..5 adjoint, 6 tri fwd, 7 tri adj
-int occupancy_blocks(int which);
+
+// Resident workgroups per CU of a persistent kernel.
+enum OccKernel : int { OCC_FWD = 0, OCC_SWEEP = 1, OCC_TRI_FWD = 2, OCC_TRI_ADJ = 3 };
+int occupancy_blocks(int kernel, int variant, int rpl);
 
 }  // namespace ddrs

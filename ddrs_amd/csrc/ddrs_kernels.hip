@@ -2,10 +2,12 @@
 //
 // Layout. Reaches are renumbered into a level-major schedule: position p
 // orders reaches by (level, reach id), where level is the longest upstream
-// path. Every upstream reach of p therefore sits at a smaller position, and a
-// 64-reach chunk (one wavefront, one workgroup) only depends on chunks at lower
-// levels plus, where a level holds fewer than 64 reaches, on earlier sub-levels
-// of itself. Per-reach statics and state live in schedule order in HBM.
+// path. Every upstream reach of p therefore sits at a smaller position. The
+// schedule is cut into chunks of 64 * RPL consecutive positions (RPL reaches
+// per lane, one wavefront per workgroup); a chunk depends only on chunks with
+// smaller indices plus, where a level holds fewer reaches than the chunk, on
+// earlier sub-levels of itself. Per-reach statics and state live in schedule
+// order in HBM.
 //
 // Synchronisation. A chunk publishes each solved value as a tagged 64-bit word
 // (fp32 bits | launch epoch << 32) with one relaxed agent-scope atomic store;
@@ -14,14 +16,16 @@
 // and readiness together, so no release/acquire fence (and no L2 writeback) is
 // needed per hop. Within a chunk, values pass through LDS level by level.
 //
-// Time skew. A persistent launch owns chunks w, w+W, ... and advances each one
-// through the window's timesteps independently, polling without blocking, so
-// chunk c can be at step t+5 while a downstream chunk is still at step t: the
+// Time skew. A persistent launch has one workgroup per chunk, and each
+// workgroup advances its chunk through all of the launch's timesteps, so chunk
+// c can be at step t+5 while a downstream chunk is still at step t: the
 // critical path becomes (chunk levels + T) hops instead of (chunk levels x T).
-// The grid never exceeds the number of co-resident workgroups, which together
-// with non-blocking polling guarantees progress: the least-advanced, lowest
-// chunk always has its inputs. A wall-clock watchdog (s_memrealtime) ends every
-// wave with an error flag if no chunk of the wave progresses for two seconds.
+// Workgroups take their chunk from an atomic counter when they start, in
+// dependency order (upstream first for the forward, downstream first for the
+// adjoint), so every chunk a running workgroup waits on belongs to a
+// workgroup that has already started: progress does not depend on how many
+// workgroups the GPU holds at once. A wall-clock watchdog (s_memrealtime) ends
+// a workgroup with an error flag if its inputs do not arrive for two seconds.
 //
 // Arithmetic follows the reference op for op (see mc_physics.h); upstream sums
 // are taken in the reference CSR order, so results do not depend on scheduling.
@@ -61,135 +65,144 @@ __device__ __forceinline__ uint64_t wall_ticks() { return __builtin_amdgcn_s_mem
 constexpr uint64_t kWatchdogTicks = 200000000ull;  // 2 s at 100 MHz
 
 // ---------------------------------------------------------------------------
-// Persistent driver shared by all solves. Dir = +1 walks steps forward and
-// chunks in increasing order, Dir = -1 walks steps backward (step index
-// nsteps-1 first) and chunks in decreasing order. For chunk c at window step s:
-//   prepare(c, s, slot)  everything that does not depend on other chunks at
-//                        step s (coefficients, right-hand side), kept in LDS
-//                        slot `slot`;
-//   ready(c, s)          polls the cross-chunk inputs of step s and stages them,
-//                        with each entry's weight and kind, in LDS;
-//   solve(c, s, slot)    the dependent part: the level-by-level solve inside the
-//                        chunk, publishing, and the step's epilogue.
-// A wave's first kPrepSlots chunks keep a prepared step in LDS, so their
-// coefficients are computed while the inputs are still in flight; other
-// chunks prepare just in time in the scratch slot kPrepSlots. All three are
-// called by the whole wave.
-constexpr int kPrepSlots = 2;
+// One chunk per workgroup, for all steps of the launch.
 
-template <int Dir, class Prepare, class Ready, class Solve>
-__device__ __forceinline__ void persistent_loop(int C, int nsteps, int* tcur, int* pstep,
-                                                int* err, Prepare prepare, Ready ready,
-                                                Solve solve) {
-    const int W = gridDim.x, w = blockIdx.x;
-    const int m = (C - w + W - 1) / W;
-    for (int k = threadIdx.x; k < m; k += 64) tcur[k] = 0;
-    if (threadIdx.x < kPrepSlots) pstep[threadIdx.x] = -1;
+// Chunk of this workgroup: the claim counter only grows (claims of earlier
+// launches are subtracted through `base`), so it is never reset. Dir = +1
+// hands out chunks 0, 1, ..., Dir = -1 hands out C-1, C-2, ...
+template <int Dir>
+__device__ __forceinline__ int claim_chunk(const Claim& cl, int C, int* lds) {
+    if (threadIdx.x == 0) {
+        const unsigned long long k = atomicAdd(cl.ctr, 1ull) - cl.base;
+        *lds = static_cast<int>(k);
+    }
     __syncthreads();
-    int remaining = m;
-    uint64_t last = wall_ticks();
-    while (remaining > 0) {
-        bool progress = false;
-        for (int kk = 0; kk < m; ++kk) {
-            const int k = (Dir > 0) ? kk : (m - 1 - kk);
-            int done = uni(tcur[k]);
-            if (done >= nsteps) continue;
-            const int c = w + k * W;
-            const int slot = (k < kPrepSlots) ? k : kPrepSlots;
-            int burst = 0;
-            const bool jit = slot == kPrepSlots;
-            while (done < nsteps && burst < 8) {
-                const int s = (Dir > 0) ? done : (nsteps - 1 - done);
-                // slotted chunks prepare before polling, the scratch slot after
-                if (jit && !ready(c, s)) break;
-                if (jit || uni(pstep[slot]) != s) {
-                    prepare(c, s, slot);
-                    __syncthreads();
-                    if (!jit && threadIdx.x == 0) pstep[slot] = s;
-                    __syncthreads();
+    const int k = uni(*lds);
+    return (Dir > 0) ? k : (C - 1 - k);
+}
+
+// For the launch's steps in time order (Dir = +1) or reverse (Dir = -1):
+//   prepare(s, first)  everything of step s that does not depend on other
+//                      chunks (coefficients, right-hand side, into LDS);
+//                      `first` asks for the chunk's step-invariant data too;
+//   ready(s)           polls the cross-chunk inputs of step s into LDS;
+//   solve(s, last)     the level-by-level solve inside the chunk, publishing,
+//                      and the step's epilogue; `last` is the launch's final step.
+// All three are called by the whole wave. Returns early (error flag set) when
+// the inputs of a step do not arrive within the watchdog time or another
+// workgroup has raised the flag.
+template <int Dir, class Prepare, class Ready, class Solve>
+__device__ __forceinline__ void chunk_loop(int nsteps, int* err, Prepare prepare, Ready ready,
+                                           Solve solve) {
+    for (int done = 0; done < nsteps; ++done) {
+        const int s = (Dir > 0) ? done : (nsteps - 1 - done);
+        prepare(s, done == 0);
+        __syncthreads();
+        if (!ready(s)) {
+            const uint64_t t0 = wall_ticks();
+            do {
+                __builtin_amdgcn_s_sleep(2);
+                const bool give_up =
+                    (wall_ticks() - t0 > kWatchdogTicks) ||
+                    __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+                if (give_up) {
+                    if (threadIdx.x == 0) atomicExch(err, 1);
+                    return;
                 }
-                if (!jit && !ready(c, s)) break;
-                solve(c, s, slot);
-                ++done;
-                ++burst;
-            }
-            if (burst > 0) {
-                progress = true;
-                __syncthreads();
-                if (threadIdx.x == 0) tcur[k] = done;
-                __syncthreads();
-                if (done >= nsteps) --remaining;
-            }
+            } while (!ready(s));
         }
-        if (progress) {
-            last = wall_ticks();
-        } else {
-            __builtin_amdgcn_s_sleep(4);
-            if (wall_ticks() - last > kWatchdogTicks) {
-                if (threadIdx.x == 0) atomicExch(err, 1);
-                return;
-            }
-        }
+        solve(s, done == nsteps - 1);
+    }
+}
+
+template <int RPL>
+struct Width {
+    static constexpr int CW = 64 * RPL;                  // positions per chunk
+    static constexpr int SHIFT = (RPL == 1) ? 6 : 7;
+    static constexpr int kStageUp = kStageUpBase * RPL;  // staged upstream links
+    static constexpr int kStageDn = kStageDnBase * RPL;  // staged downstream links
+};
+
+// Stage the links [base, end) of chunk c into LDS: link position, weight, and
+// kind (>= 0: chunk-local index of a link inside the chunk, -1: polled from
+// another chunk).
+template <int RPL>
+__device__ __forceinline__ void stage_links(int c, int base, int end, const int* pos,
+                                            const float* w, int* spos, float* sadj,
+                                            short* skind) {
+    for (int e = base + threadIdx.x; e < end; e += 64) {
+        const int u = pos[e];
+        spos[e - base] = u;
+        sadj[e - base] = w[e];
+        skind[e - base] = ((u >> Width<RPL>::SHIFT) == c)
+                              ? static_cast<short>(u & (Width<RPL>::CW - 1))
+                              : static_cast<short>(-1);
     }
 }
 
 // ---------------------------------------------------------------------------
 // forward window
 
-template <int PH>
-__global__ __launch_bounds__(64, 5) void k_fwd_window(FwdArgs a) {
-    __shared__ float lds_x[64];
-    __shared__ float sval[kStageMax], sadj[kStageMax];
-    __shared__ short skind[kStageMax];  // >= 0: lane of an intra-chunk upstream; -1: staged
-    __shared__ float pb[kPrepSlots + 1][64], pnc1[kPrepSlots + 1][64];
-    __shared__ int pe0[kPrepSlots + 1][64], pe1[kPrepSlots + 1][64], porig[kPrepSlots + 1][64];
-    __shared__ int psub[kPrepSlots + 1][64];
-    __shared__ int pstep[kPrepSlots];
-    __shared__ int tcur[kMaxChunksPerWave];
+template <int PH, int RPL>
+__global__ __launch_bounds__(64) void k_fwd_window(FwdArgs a) {
+    using Wd = Width<RPL>;
+    constexpr int CW = Wd::CW;
+    __shared__ float lds_x[CW];
+    __shared__ float sval[Wd::kStageUp], sadj[Wd::kStageUp];
+    __shared__ int spos[Wd::kStageUp];
+    __shared__ short skind[Wd::kStageUp];
+    __shared__ float pb[CW], pnc1[CW], sq[CW], si[CW];
+    __shared__ short pe0[CW], pe1[CW], psub[CW];
+    __shared__ int porig[CW];
+    __shared__ int claimed;
     const int lane = threadIdx.x;
     const Sched& sc = a.sc;
-    const int R = sc.R;
+    const int R = sc.R, P = sc.P;
+    const int c = claim_chunk<+1>(a.claim, a.ch.C, &claimed);
+    const int p0 = c * CW;
+    const int base = sc.up_ptr[p0];
+    const int nlinks = sc.up_ptr[p0 + CW] - base;
+    const int nsub = a.ch.nsub[c];
 
-    // S1..S25 for chunk c at window step s: coefficients and right-hand side.
-    auto prepare = [&](int c, int s, int slot) {
-        const int p = c * 64 + lane;
-        const int base = sc.up_ptr[c * 64];
-        if (p < R) {
-            const int t = a.t0 + a.s_base + s;
-            const int orig = sc.perm[p];
-            const float q = a.st_q[p];
-            const float it = a.st_i[p];
-            const float qp = fmaxr(a.qprime[static_cast<size_t>(t) * R + orig], a.P.q_lb);
-            Coef co;
-            mc_coef<PH>(a.P, a.st.n[p], a.st.qsp[p], a.st.p[p], q, a.st.len[p], a.st.slope[p],
-                        a.st.sqrt_s[p], a.st.xst[p], co);
-            pb[slot][lane] = ((co.c2 * it) + (co.c3 * q)) + (co.c4 * qp);        // S25
-            pnc1[slot][lane] = -co.c1;
-            pe0[slot][lane] = sc.up_ptr[p] - base;
-            pe1[slot][lane] = sc.up_ptr[p + 1] - base;
-            porig[slot][lane] = orig;
-            psub[slot][lane] = sc.sub[p];
-        } else {
-            psub[slot][lane] = -1;
+    // S1..S25 for window step s: coefficients and right-hand side.
+    auto prepare = [&](int s, bool first) {
+        if (first) stage_links<RPL>(c, base, base + nlinks, sc.up_pos, sc.up_adj, spos, sadj, skind);
+        const int t = a.t0 + a.s_base + s;
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+            const int j = r * 64 + lane;
+            const int p = p0 + j;
+            if (first) porig[j] = sc.perm[p];
+            if (porig[j] >= 0) {
+                if (first) {
+                    pe0[j] = static_cast<short>(sc.up_ptr[p] - base);
+                    pe1[j] = static_cast<short>(sc.up_ptr[p + 1] - base);
+                    psub[j] = a.ch.sub[p];
+                    sq[j] = a.st_q[p];
+                    si[j] = a.st_i[p];
+                }
+                const float q = sq[j];
+                const float it = si[j];
+                const float qp = fmaxr(a.qprime[static_cast<size_t>(t) * R + porig[j]], a.P.q_lb);
+                const Reach rc = load_reach(a.st, p);
+                Coef co;
+                mc_coef<PH>(a.P, rc, q, co);
+                pb[j] = ((co.c2 * it) + (co.c3 * q)) + (co.c4 * qp);             // S25
+                pnc1[j] = -co.c1;
+            } else if (first) {
+                psub[j] = -1;
+            }
         }
     };
 
-    auto ready = [&](int c, int s) -> bool {
-        const int p0 = c * 64;
-        const int p1 = min(p0 + 64, R);
-        const int base = sc.up_ptr[p0], end = sc.up_ptr[p1];
-        const uint64_t* row = a.hist + static_cast<size_t>(a.s_base + s) * R;
+    auto ready = [&](int s) -> bool {
+        const uint64_t* row = a.hist + static_cast<size_t>(a.s_base + s) * P;
         bool ok = true;
-        for (int e = base + lane; e < end; e += 64) {
-            const int u = sc.up_pos[e];
-            sadj[e - base] = sc.up_adj[e];
-            if ((u >> 6) == c) {
-                skind[e - base] = static_cast<short>(u & 63);
-            } else {
-                skind[e - base] = -1;
-                const uint64_t wv = poll(row + u);
+        for (int e = lane; e < nlinks; e += 64) {
+            if (skind[e] < 0) {
+                const uint64_t wv = poll(row + spos[e]);
                 if (!word_ready(wv, a.epoch)) ok = false;
-                else sval[e - base] = word_value(wv);
+                else sval[e] = word_value(wv);
             }
         }
         const bool all = __all(ok);
@@ -198,124 +211,179 @@ __global__ __launch_bounds__(64, 5) void k_fwd_window(FwdArgs a) {
     };
 
     // S26..S28 and the next step's S24 (N Q_{t+1}) from LDS only.
-    auto solve = [&](int c, int s, int slot) {
-        const int p = c * 64 + lane;
+    auto solve = [&](int s, bool last) {
         const int sg = a.s_base + s;
         const int t = a.t0 + sg;
-        uint64_t* row = a.hist + static_cast<size_t>(sg) * R;
-        const int my = psub[slot][lane];
-        const float b = pb[slot][lane];
-        const float nc1 = pnc1[slot][lane];
-        const int e0 = pe0[slot][lane], e1 = pe1[slot][lane];
-        float x = 0.f, inext = 0.f;
-        const int nsub = uni(sc.nsub[c]);
+        uint64_t* row = a.hist + static_cast<size_t>(sg) * P;
+        int my[RPL];
+        float x[RPL], inext[RPL];
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+            my[r] = psub[r * 64 + lane];
+            x[r] = 0.0f;
+            inext[r] = 0.0f;
+        }
         for (int l = 0; l < nsub; ++l) {
-            if (my == l) {
-                float sum = 0.0f, in = 0.0f;
-                for (int e = e0; e < e1; ++e) {
-                    const int kd = skind[e];
-                    const float v = (kd >= 0) ? lds_x[kd] : sval[e];
-                    const float adj = sadj[e];
-                    sum = sum + (nc1 * adj) * v;
-                    in = in + fmaxr(v, a.P.q_lb) * adj;
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                if (my[r] == l) {
+                    const int j = r * 64 + lane;
+                    const float b = pb[j];
+                    const float nc1 = pnc1[j];
+                    const int e1 = pe1[j];
+                    float sum = 0.0f, in = 0.0f;
+                    for (int e = pe0[j]; e < e1; ++e) {
+                        const int kd = skind[e];
+                        const float v = (kd >= 0) ? lds_x[kd] : sval[e];
+                        const float adj = sadj[e];
+                        sum = sum + (nc1 * adj) * v;
+                        in = in + fmaxr(v, a.P.q_lb) * adj;
+                    }
+                    const float ad = 1.0f + nc1 * 0.0f;
+                    x[r] = (b - sum) / ad;
+                    inext[r] = in;
+                    lds_x[j] = x[r];
+                    publish(row + p0 + j, x[r], a.epoch);
                 }
-                const float ad = 1.0f + nc1 * 0.0f;
-                x = (b - sum) / ad;
-                inext = in;
-                lds_x[lane] = x;
-                publish(row + p, x, a.epoch);
             }
             __syncthreads();
         }
-        if (my >= 0) {
-            const float qn = fmaxr(x, a.P.q_lb);
-            const int orig = porig[slot][lane];
-            a.st_q[p] = qn;
-            a.st_i[p] = inext;
-            if (a.Q_out) a.Q_out[static_cast<size_t>(t + 1) * R + orig] = qn;
-            if (a.X_out) a.X_out[orig] = x;
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+            if (my[r] >= 0) {
+                const int j = r * 64 + lane;
+                const float qn = fmaxr(x[r], a.P.q_lb);
+                sq[j] = qn;
+                si[j] = inext[r];
+                if (last) {
+                    a.st_q[p0 + j] = qn;
+                    a.st_i[p0 + j] = inext[r];
+                }
+                if (a.Q_out) a.Q_out[static_cast<size_t>(t + 1) * R + porig[j]] = qn;
+                if (a.X_out) a.X_out[porig[j]] = x[r];
+            }
         }
     };
 
-    persistent_loop<+1>(sc.C, a.nsteps, tcur, pstep, a.err, prepare, ready, solve);
+    chunk_loop<+1>(a.nsteps, a.err, prepare, ready, solve);
 }
 
 // ---------------------------------------------------------------------------
-// adjoint window
+// adjoint window, phase 1: linearisation of every reach-step
+//
+// One step's backward (B28..B1) is linear in the transposed-solve value gb once
+// the forward state is fixed. This kernel evaluates, for every reach and step
+// of the segment at once, the coefficients (S1..S23 from the forward history)
+// and the backward chain for gb = 1; the reverse-time sweep then only solves
+// for gb, and the parameter gradients are gb times these records.
 
 template <int PH>
-__global__ __launch_bounds__(64, 4) void k_adj_window(AdjArgs a) {
-    __shared__ float ldsP[64], ldsG[64];
-    __shared__ float stP[kStageMax], stG[kStageMax], sadj[kStageMax];
-    __shared__ short skind[kStageMax];
-    __shared__ float pgx[kPrepSlots + 1][64], pnc1[kPrepSlots + 1][64], pc2[kPrepSlots + 1][64];
-    __shared__ int pe0[kPrepSlots + 1][64], pe1[kPrepSlots + 1][64], psub[kPrepSlots + 1][64];
-    __shared__ int pstep[kPrepSlots];
-    __shared__ int tcur[kMaxChunksPerWave];
+__global__ __launch_bounds__(256) void k_adj_lin(LinArgs a) {
+    const Sched& sc = a.sc;
+    const int R = sc.R, P = sc.P;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int orig = sc.perm[p];
+    if (orig < 0) return;
+    const int sg = a.s_base + static_cast<int>(blockIdx.y);
+    const int t = a.t0 + sg;
+    const float lb = a.P.q_lb;
+    const int e0 = sc.up_ptr[p], e1 = sc.up_ptr[p + 1];
+    const uint64_t* hx = a.hist + static_cast<size_t>(sg) * P;
+    float q, it;
+    if (sg > 0) {
+        const uint64_t* hq = hx - P;
+        q = fmaxr(word_value(hq[p]), lb);
+        float acc = 0.0f;
+        for (int e = e0; e < e1; ++e)
+            acc = acc + fmaxr(word_value(hq[sc.up_pos[e]]), lb) * sc.up_adj[e];
+        it = acc;
+    } else {
+        q = a.ck_q[p];
+        it = a.ck_i[p];
+    }
+    // B26 for gb = 1: gc1 = -sum_k adj_k * ((-1) * x[col k]) = sum_k x[col k] * adj_k
+    float sx = 0.0f;
+    for (int e = e0; e < e1; ++e) sx = sx + word_value(hx[sc.up_pos[e]]) * sc.up_adj[e];
+    const float qp_raw = a.qprime[static_cast<size_t>(t) * R + orig];
+    const float qp = fmaxr(qp_raw, lb);
+    const Reach rc = load_reach(a.st, p);
+    Coef co;
+    mc_coef<PH>(a.P, rc, q, co);
+    const LocalGrads g = mc_local_bwd<PH>(a.P, co, rc, q, qp, it, 1.0f, sx, 0.0f);
+    const size_t o = static_cast<size_t>(sg) * P + p;
+    const float dq = a.gQ ? a.gQ[static_cast<size_t>(t) * R + orig] : 0.0f;
+    a.rec[o] = make_float4(-co.c1, co.c2, g.gq, dq);
+    a.recm[o] = (!a.mask_lam || q > lb) ? 1 : 0;
+    a.recj[o] = make_float4(g.gn, g.gqsp, g.gp, (!a.mask_qp || qp_raw > lb) ? g.gqp : 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// adjoint window, phase 2: reverse-time transposed solves (B28, B27, B24)
+//
+// Reverse step s of reach p: gx = masked dL/d out[t+1]; gb = gx - sum over
+// downstream d of adj * P_d; P_p = -c1 gb and G_p = c2 gb go to the upstream
+// reaches; dL/d out[t] = direct + (gb * dgq/dgb + sum_d G_d * adj), masked by
+// out[t] > lb when it is passed through the clamp of the previous step.
+
+template <int RPL>
+__global__ __launch_bounds__(64) void k_adj_sweep(SweepArgs a) {
+    using Wd = Width<RPL>;
+    constexpr int CW = Wd::CW;
+    __shared__ float ldsP[CW], ldsG[CW];
+    __shared__ float stP[Wd::kStageDn], stG[Wd::kStageDn], sadj[Wd::kStageDn];
+    __shared__ int spos[Wd::kStageDn];
+    __shared__ short skind[Wd::kStageDn];
+    __shared__ float4 prec[CW];
+    __shared__ float slam[CW];
+    __shared__ unsigned char pm[CW], pvalid[CW];
+    __shared__ short pe0[CW], pe1[CW], psub[CW];
+    __shared__ int claimed;
     const int lane = threadIdx.x;
     const Sched& sc = a.sc;
-    const int R = sc.R;
+    const int P = sc.P;
+    const int c = claim_chunk<-1>(a.claim, a.ch.C, &claimed);
+    const int p0 = c * CW;
+    const int base = sc.dn_ptr[p0];
+    const int nlinks = sc.dn_ptr[p0 + CW] - base;
+    const int nsub = a.ch.nsubr[c];
 
-    // Everything of reverse step s that does not wait for downstream messages:
-    // out[t], N out[t] and q' of the step, the coefficients, and B28.
-    auto prepare = [&](int c, int s, int slot) {
-        const int p = c * 64 + lane;
-        const int base = sc.dn_ptr[c * 64];
-        if (p < R) {
-            const int sg = a.s_base + s;
-            const int t = a.t0 + sg;
-            const int orig = sc.perm[p];
-            const float xs = word_value(a.hist[static_cast<size_t>(sg) * R + p]);
-            float q, it;
-            if (sg > 0) {
-                const uint64_t* hq = a.hist + static_cast<size_t>(sg - 1) * R;
-                q = fmaxr(word_value(hq[p]), a.P.q_lb);
-                float acc = 0.0f;
-                for (int e = sc.up_ptr[p]; e < sc.up_ptr[p + 1]; ++e)
-                    acc = acc + fmaxr(word_value(hq[sc.up_pos[e]]), a.P.q_lb) * sc.up_adj[e];
-                it = acc;
-            } else {
-                q = a.ck_q[p];
-                it = a.ck_i[p];
+    auto prepare = [&](int s, bool first) {
+        if (first) stage_links<RPL>(c, base, base + nlinks, sc.dn_pos, sc.dn_adj, spos, sadj, skind);
+        const size_t row = static_cast<size_t>(a.s_base + s) * P;
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+            const int j = r * 64 + lane;
+            const int p = p0 + j;
+            if (first) pvalid[j] = sc.perm[p] >= 0;
+            if (pvalid[j]) {
+                if (first) {
+                    pe0[j] = static_cast<short>(sc.dn_ptr[p] - base);
+                    pe1[j] = static_cast<short>(sc.dn_ptr[p + 1] - base);
+                    psub[j] = a.ch.subr[p];
+                    slam[j] = a.st_lam[p];
+                }
+                prec[j] = a.rec[row + p];
+                pm[j] = a.recm[row + p];
+            } else if (first) {
+                psub[j] = -1;
             }
-            const float qp_raw = a.qprime[static_cast<size_t>(t) * R + orig];
-            Coef co;
-            mc_coef<PH>(a.P, a.st.n[p], a.st.qsp[p], a.st.p[p], q, a.st.len[p], a.st.slope[p],
-                        a.st.sqrt_s[p], a.st.xst[p], co);
-            pgx[slot][lane] = (xs > a.P.q_lb) ? a.st_lam[p] : 0.0f;            // B28
-            pnc1[slot][lane] = -co.c1;
-            pc2[slot][lane] = co.c2;
-            pe0[slot][lane] = sc.dn_ptr[p] - base;
-            pe1[slot][lane] = sc.dn_ptr[p + 1] - base;
-            psub[slot][lane] = sc.subr[p];
-            a.sc_q[p] = q;
-            a.sc_i[p] = it;
-            a.sc_qp[p] = qp_raw;
-        } else {
-            psub[slot][lane] = -1;
         }
     };
 
-    auto ready = [&](int c, int s) -> bool {
-        const int p0 = c * 64;
-        const int p1 = min(p0 + 64, R);
-        const int base = sc.dn_ptr[p0], end = sc.dn_ptr[p1];
-        const uint64_t* row = a.amsg + static_cast<size_t>(a.s_base + s) * R * 2;
+    auto ready = [&](int s) -> bool {
+        const uint64_t* row = a.amsg + static_cast<size_t>(a.s_base + s) * P * 2;
         bool ok = true;
-        for (int e = base + lane; e < end; e += 64) {
-            const int d = sc.dn_pos[e];
-            sadj[e - base] = sc.dn_adj[e];
-            if ((d >> 6) == c) {
-                skind[e - base] = static_cast<short>(d & 63);
-            } else {
-                skind[e - base] = -1;
+        for (int e = lane; e < nlinks; e += 64) {
+            if (skind[e] < 0) {
+                const int d = spos[e];
                 const uint64_t w0 = poll(row + 2 * d);
                 const uint64_t w1 = poll(row + 2 * d + 1);
                 if (!word_ready(w0, a.epoch) || !word_ready(w1, a.epoch)) {
                     ok = false;
                 } else {
-                    stP[e - base] = word_value(w0);
-                    stG[e - base] = word_value(w1);
+                    stP[e] = word_value(w0);
+                    stG[e] = word_value(w1);
                 }
             }
         }
@@ -324,86 +392,100 @@ __global__ __launch_bounds__(64, 4) void k_adj_window(AdjArgs a) {
         return all;
     };
 
-    auto solve = [&](int c, int s, int slot) {
-        const int p = c * 64 + lane;
+    auto solve = [&](int s, bool last) {
         const int sg = a.s_base + s;
-        const int t = a.t0 + sg;
-        uint64_t* mrow = a.amsg + static_cast<size_t>(sg) * R * 2;
-        const int my = psub[slot][lane];
-        const float gx = pgx[slot][lane];
-        const float nc1 = pnc1[slot][lane];
-        const float c2 = pc2[slot][lane];
-        const int e0 = pe0[slot][lane], e1 = pe1[slot][lane];
-        float gb = 0.f, gq24 = 0.f;
-        const int nsub = uni(sc.nsubr[c]);
+        uint64_t* mrow = a.amsg + static_cast<size_t>(sg) * P * 2;
+        int my[RPL];
+        float gbv[RPL], lamv[RPL];
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+            my[r] = psub[r * 64 + lane];
+            gbv[r] = 0.0f;
+            lamv[r] = 0.0f;
+        }
         for (int l = 0; l < nsub; ++l) {
-            if (my == l) {                                               // B27 transposed solve
-                float sumP = 0.0f, sumG = 0.0f;
-                for (int e = e0; e < e1; ++e) {
-                    const int kd = skind[e];
-                    const float Pd = (kd >= 0) ? ldsP[kd] : stP[e];
-                    const float Gd = (kd >= 0) ? ldsG[kd] : stG[e];
-                    const float adj = sadj[e];
-                    sumP = sumP + adj * Pd;
-                    sumG = sumG + Gd * adj;                              // B24 N^T gi
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                if (my[r] == l) {
+                    const int j = r * 64 + lane;
+                    const float4 rc = prec[j];  // {-c1, c2, dgq/dgb, direct}
+                    const int e1 = pe1[j];
+                    float sumP = 0.0f, sumG = 0.0f;
+                    for (int e = pe0[j]; e < e1; ++e) {
+                        const int kd = skind[e];
+                        const float Pd = (kd >= 0) ? ldsP[kd] : stP[e];
+                        const float Gd = (kd >= 0) ? ldsG[kd] : stG[e];
+                        const float adj = sadj[e];
+                        sumP = sumP + adj * Pd;                          // B27
+                        sumG = sumG + Gd * adj;                          // B24 N^T gi
+                    }
+                    const float ad = 1.0f + rc.x * 0.0f;
+                    const float gb = (slam[j] - sumP) / ad;
+                    const float Pm = rc.x * gb;
+                    const float Gm = rc.y * gb;
+                    ldsP[j] = Pm;
+                    ldsG[j] = Gm;
+                    publish(mrow + 2 * (p0 + j), Pm, a.epoch);
+                    publish(mrow + 2 * (p0 + j) + 1, Gm, a.epoch);
+                    gbv[r] = gb;
+                    lamv[r] = pm[j] ? rc.w + (gb * rc.z + sumG) : 0.0f;
                 }
-                const float ad = 1.0f + nc1 * 0.0f;
-                gb = (gx - sumP) / ad;
-                gq24 = sumG;
-                const float Pm = nc1 * gb;
-                const float Gm = c2 * gb;
-                ldsP[lane] = Pm;
-                ldsG[lane] = Gm;
-                publish(mrow + 2 * p, Pm, a.epoch);
-                publish(mrow + 2 * p + 1, Gm, a.epoch);
             }
             __syncthreads();
         }
-        if (my >= 0) {
-            const uint64_t* hx = a.hist + static_cast<size_t>(sg) * R;
-            // B26: gc1 = -sum_k adj_k * ((-gb) * x[col k]) in CSR order
-            float acc = 0.0f;
-            for (int e = sc.up_ptr[p]; e < sc.up_ptr[p + 1]; ++e) {
-                const float gA = (-gb) * word_value(hx[sc.up_pos[e]]);
-                acc = acc + gA * sc.up_adj[e];
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+            if (my[r] >= 0) {
+                const int j = r * 64 + lane;
+                slam[j] = lamv[r];
+                if (last) a.st_lam[p0 + j] = lamv[r];
+                a.gbuf[static_cast<size_t>(sg) * P + p0 + j] = gbv[r];
             }
-            const float gc1 = -acc;
-            const float q = a.sc_q[p], it = a.sc_i[p], qp_raw = a.sc_qp[p];
-            const float qp = fmaxr(qp_raw, a.P.q_lb);
-            Coef co;
-            mc_coef<PH>(a.P, a.st.n[p], a.st.qsp[p], a.st.p[p], q, a.st.len[p], a.st.slope[p],
-                        a.st.sqrt_s[p], a.st.xst[p], co);
-            const LocalGrads g = mc_local_bwd<PH>(a.P, co, a.st.n[p], a.st.p[p], q, qp,
-                                                  a.st.len[p], a.st.slope[p], a.st.sqrt_s[p],
-                                                  it, gb, gc1, gq24);
-            a.acc_n[p] = a.acc_n[p] + g.gn;
-            a.acc_qsp[p] = a.acc_qsp[p] + g.gqsp;
-            a.acc_p[p] = a.acc_p[p] + g.gp;
-            const int orig = sc.perm[p];
-            const float direct = a.gQ ? a.gQ[static_cast<size_t>(t) * R + orig] : 0.0f;
-            a.st_lam[p] = direct + g.gq;
-            if (a.gqp_out)
-                a.gqp_out[static_cast<size_t>(t) * R + orig] =
-                    (!a.mask_qp || qp_raw > a.P.q_lb) ? g.gqp : 0.0f;
         }
     };
 
-    persistent_loop<-1>(sc.C, a.nsteps, tcur, pstep, a.err, prepare, ready, solve);
+    chunk_loop<-1>(a.nsteps, a.err, prepare, ready, solve);
 }
 
 // ---------------------------------------------------------------------------
-// general triangular solves (one step)
+// adjoint window, phase 3: gradients of n, q_spatial, p_spatial (accumulated
+// over steps in reverse time order) and of q'.
+
+__global__ __launch_bounds__(256) void k_adj_fin(FinArgs a) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.P) return;
+    const int orig = a.perm[p];
+    if (orig < 0) return;
+    float an = a.acc_n[p], aq = a.acc_qsp[p], ap = a.acc_p[p];
+    for (int s = a.nsteps - 1; s >= 0; --s) {
+        const size_t o = static_cast<size_t>(a.s_base + s) * a.P + p;
+        const float gb = a.gbuf[o];
+        const float4 j = a.recj[o];
+        an = an + gb * j.x;
+        aq = aq + gb * j.y;
+        ap = ap + gb * j.z;
+        if (a.gqp_out)
+            a.gqp_out[static_cast<size_t>(a.t0 + a.s_base + s) * a.R + orig] = gb * j.w;
+    }
+    a.acc_n[p] = an;
+    a.acc_qsp[p] = aq;
+    a.acc_p[p] = ap;
+}
+
+// ---------------------------------------------------------------------------
+// general triangular solves (one step, 64-wide chunks)
 
 __global__ __launch_bounds__(64) void k_tri_fwd(TriArgs a) {
     __shared__ float lds_x[64];
-    __shared__ float stage[kStageMax];
-    __shared__ int tcur[kMaxChunksPerWave];
+    __shared__ float stage[kStageUpBase];
+    __shared__ int claimed;
     const int lane = threadIdx.x;
     const Sched& sc = a.sc;
-    const int R = sc.R;
-    auto ready = [&](int c, int) -> bool {
-        const int p0 = c * 64, p1 = min(p0 + 64, R);
-        const int base = sc.up_ptr[p0], end = sc.up_ptr[p1];
+    const int c = claim_chunk<+1>(a.claim, a.ch.C, &claimed);
+    const int base = sc.up_ptr[c * 64];
+    const int end = sc.up_ptr[c * 64 + 64];
+    auto prepare = [&](int, bool) {};
+    auto ready = [&](int) -> bool {
         bool ok = true;
         for (int e = base + lane; e < end; e += 64) {
             const int u = sc.up_pos[e];
@@ -417,13 +499,12 @@ __global__ __launch_bounds__(64) void k_tri_fwd(TriArgs a) {
         __syncthreads();
         return all;
     };
-    auto process = [&](int c, int) {
+    auto solve = [&](int, bool) {
         const int p = c * 64 + lane;
-        const bool valid = p < R;
-        const int base = sc.up_ptr[c * 64];
-        int my = valid ? sc.sub[p] : -1;
+        const bool valid = sc.perm[p] >= 0;
+        int my = valid ? a.ch.sub[p] : -1;
         float x = 0.f;
-        const int nsub = uni(sc.nsub[c]);
+        const int nsub = a.ch.nsub[c];
         for (int l = 0; l < nsub; ++l) {
             if (my == l) {
                 float sum = 0.0f;
@@ -442,22 +523,20 @@ __global__ __launch_bounds__(64) void k_tri_fwd(TriArgs a) {
         }
         if (valid) a.x[sc.perm[p]] = a.clamp ? fmaxr(x, a.lb) : x;
     };
-    __shared__ int pstep[kPrepSlots];
-    auto prepare = [&](int, int, int) {};
-    auto solve = [&](int c, int s, int) { process(c, s); };
-    persistent_loop<+1>(sc.C, 1, tcur, pstep, a.err, prepare, ready, solve);
+    chunk_loop<+1>(1, a.err, prepare, ready, solve);
 }
 
 __global__ __launch_bounds__(64) void k_tri_adj(TriArgs a) {
     __shared__ float lds_g[64];
-    __shared__ float stage[kStageMax];
-    __shared__ int tcur[kMaxChunksPerWave];
+    __shared__ float stage[kStageDnBase];
+    __shared__ int claimed;
     const int lane = threadIdx.x;
     const Sched& sc = a.sc;
-    const int R = sc.R;
-    auto ready = [&](int c, int) -> bool {
-        const int p0 = c * 64, p1 = min(p0 + 64, R);
-        const int base = sc.dn_ptr[p0], end = sc.dn_ptr[p1];
+    const int c = claim_chunk<-1>(a.claim, a.ch.C, &claimed);
+    const int base = sc.dn_ptr[c * 64];
+    const int end = sc.dn_ptr[c * 64 + 64];
+    auto prepare = [&](int, bool) {};
+    auto ready = [&](int) -> bool {
         bool ok = true;
         for (int e = base + lane; e < end; e += 64) {
             const int d = sc.dn_pos[e];
@@ -471,13 +550,12 @@ __global__ __launch_bounds__(64) void k_tri_adj(TriArgs a) {
         __syncthreads();
         return all;
     };
-    auto process = [&](int c, int) {
+    auto solve = [&](int, bool) {
         const int p = c * 64 + lane;
-        const bool valid = p < R;
-        const int base = sc.dn_ptr[c * 64];
-        int my = valid ? sc.subr[p] : -1;
+        const bool valid = sc.perm[p] >= 0;
+        int my = valid ? a.ch.subr[p] : -1;
         float g = 0.f;
-        const int nsub = uni(sc.nsubr[c]);
+        const int nsub = a.ch.nsubr[c];
         for (int l = 0; l < nsub; ++l) {
             if (my == l) {
                 float sum = 0.0f;
@@ -498,10 +576,7 @@ __global__ __launch_bounds__(64) void k_tri_adj(TriArgs a) {
         }
         if (valid) a.x[sc.perm[p]] = g;
     };
-    __shared__ int pstep[kPrepSlots];
-    auto prepare = [&](int, int, int) {};
-    auto solve = [&](int c, int s, int) { process(c, s); };
-    persistent_loop<-1>(sc.C, 1, tcur, pstep, a.err, prepare, ready, solve);
+    chunk_loop<-1>(1, a.err, prepare, ready, solve);
 }
 
 // grad_a[k] = (-gb[row k]) * x[col k] over the CSR entries (dispatch.rs:120-136)
@@ -512,30 +587,30 @@ __global__ void k_grad_a(int nnz, const int* row_for_nnz, const int* col, const 
 }
 
 // ---------------------------------------------------------------------------
-// gathers between reference order and schedule order
+// gathers between reference order and schedule order (over positions p < P;
+// empty positions hold zeros)
 
-__global__ void k_bind_statics(int R, const int* perm, const float* n, const float* qsp,
-                               const float* psp, const float* len, const float* slope,
-                               const float* xst, float slope_lb, Statics st) {
+__global__ void k_bind_statics(Sched sc, const float* n, const float* qsp, const float* psp,
+                               const float* len, const float* slope, const float* xst,
+                               float slope_lb, float4* st) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= R) return;
-    const int o = perm[p];
-    st.n[p] = n[o];
-    st.qsp[p] = qsp[o];
-    st.p[p] = psp[o];
-    st.len[p] = len[o];
-    const float s = fmaxr(slope[o], slope_lb);  // setup_inputs clamp_min (mmc.rs:259-264)
-    st.slope[p] = s;
-    st.sqrt_s[p] = sqrtf(s);
-    st.xst[p] = xst ? xst[o] : 0.3f;
+    if (p >= sc.P) return;
+    const int o = sc.perm[p];
+    if (o < 0) return;
+    const float sl = fmaxr(slope[o], slope_lb);  // setup_inputs clamp_min (mmc.rs:259-264)
+    make_reach(n[o], qsp[o], psp[o], len[o], sl, xst ? xst[o] : 0.3f, st + 3 * static_cast<size_t>(p));
 }
 
 // st_q[p] = (clamp ? max(q[o], lb) : q[o]); optionally Q_out row 0.
-__global__ void k_state_q(int R, const int* perm, const float* q, int clamp, float lb,
-                          float* st_q, float* Q_row0) {
+__global__ void k_state_q(Sched sc, const float* q, int clamp, float lb, float* st_q,
+                          float* Q_row0) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= R) return;
-    const int o = perm[p];
+    if (p >= sc.P) return;
+    const int o = sc.perm[p];
+    if (o < 0) {
+        st_q[p] = 0.0f;
+        return;
+    }
     const float v = clamp ? fmaxr(q[o], lb) : q[o];
     st_q[p] = v;
     if (Q_row0) Q_row0[o] = v;
@@ -544,42 +619,47 @@ __global__ void k_state_q(int R, const int* perm, const float* q, int clamp, flo
 // st_i[p] = sum_k st_q[up k] * adj_k in CSR order (S24 of the first step).
 __global__ void k_state_i(Sched sc, const float* st_q, float* st_i) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= sc.R) return;
+    if (p >= sc.P) return;
     float acc = 0.0f;
     for (int e = sc.up_ptr[p]; e < sc.up_ptr[p + 1]; ++e) acc = acc + st_q[sc.up_pos[e]] * sc.up_adj[e];
     st_i[p] = acc;
 }
 
 // x history row 0 from a caller vector (single-step backward).
-__global__ void k_hist_from_x(int R, const int* perm, const float* x, uint64_t* hist) {
+__global__ void k_hist_from_x(Sched sc, const float* x, uint64_t* hist) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= R) return;
-    hist[p] = tag_word(x[perm[p]], 0u);
+    if (p >= sc.P) return;
+    const int o = sc.perm[p];
+    hist[p] = tag_word(o >= 0 ? x[o] : 0.0f, 0u);
 }
 
-__global__ void k_lam_init(int R, const int* perm, const float* g, float* lam, float* a0,
-                           float* a1, float* a2) {
+// Gradient entering the last step's solve: dL/d out[T-1] through max(x, lb)
+// (B28), with x the last row of the forward history.
+__global__ void k_lam_init(Sched sc, const float* g, const uint64_t* hist_row, float lb,
+                           float* lam, float* a0, float* a1, float* a2) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= R) return;
-    lam[p] = g ? g[perm[p]] : 0.0f;
+    if (p >= sc.P) return;
+    const int o = sc.perm[p];
+    const float gv = (g && o >= 0) ? g[o] : 0.0f;
+    lam[p] = (!hist_row || word_value(hist_row[p]) > lb) ? gv : 0.0f;
     a0[p] = 0.0f;
     a1[p] = 0.0f;
     a2[p] = 0.0f;
 }
 
-__global__ void k_unpermute(int R, const int* perm, const float* src, float* dst) {
+__global__ void k_unpermute(Sched sc, const float* src, float* dst) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= R) return;
-    dst[perm[p]] = src[p];
+    if (p >= sc.P) return;
+    const int o = sc.perm[p];
+    if (o >= 0) dst[o] = src[p];
 }
 
 // dL/dQ0 through out[0] = max(Q0, lb) (mmc.rs:536-541)
-__global__ void k_grad_q0(int R, const int* perm, const float* lam, const float* q0, float lb,
-                          float* g) {
+__global__ void k_grad_q0(Sched sc, const float* lam, const float* q0, float lb, float* g) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= R) return;
-    const int o = perm[p];
-    g[o] = (q0[o] > lb) ? lam[p] : 0.0f;
+    if (p >= sc.P) return;
+    const int o = sc.perm[p];
+    if (o >= 0) g[o] = (q0[o] > lb) ? lam[p] : 0.0f;
 }
 
 __global__ void k_zero_f(size_t n, float* x) {
@@ -590,34 +670,40 @@ __global__ void k_zero_f(size_t n, float* x) {
 // ---------------------------------------------------------------------------
 // launch helpers
 
-template <int PH>
+template <int PH, int RPL>
 static void launch_fwd_t(const FwdArgs& a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_fwd_window<PH>, dim3(grid), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((k_fwd_window<PH, RPL>), dim3(grid), dim3(64), 0, s, a);
 }
-template <int PH>
-static void launch_adj_t(const AdjArgs& a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_window<PH>, dim3(grid), dim3(64), 0, s, a);
-}
-
-static int phys_variant(const ddrs_params* p) {
-    if (p->ddr_match) return PH_DDR;
-    if (p->enforce_positivity) return PH_POS;
-    return PH_DEFAULT;
-}
-
-void launch_fwd(const FwdArgs& a, const ddrs_params* p, int grid, hipStream_t s) {
-    switch (phys_variant(p)) {
-        case PH_DDR: launch_fwd_t<PH_DDR>(a, grid, s); break;
-        case PH_POS: launch_fwd_t<PH_POS>(a, grid, s); break;
-        default: launch_fwd_t<PH_DEFAULT>(a, grid, s); break;
+template <int RPL>
+static void launch_fwd_r(const FwdArgs& a, int variant, int grid, hipStream_t s) {
+    switch (variant) {
+        case PH_DDR: launch_fwd_t<PH_DDR, RPL>(a, grid, s); break;
+        case PH_POS: launch_fwd_t<PH_POS, RPL>(a, grid, s); break;
+        default: launch_fwd_t<PH_DEFAULT, RPL>(a, grid, s); break;
     }
 }
-void launch_adj(const AdjArgs& a, const ddrs_params* p, int grid, hipStream_t s) {
-    switch (phys_variant(p)) {
-        case PH_DDR: launch_adj_t<PH_DDR>(a, grid, s); break;
-        case PH_POS: launch_adj_t<PH_POS>(a, grid, s); break;
-        default: launch_adj_t<PH_DEFAULT>(a, grid, s); break;
+
+void launch_fwd(const FwdArgs& a, int variant, int rpl, int grid, hipStream_t s) {
+    if (rpl == 2) launch_fwd_r<2>(a, variant, grid, s);
+    else launch_fwd_r<1>(a, variant, grid, s);
+}
+
+static inline dim3 g1(size_t n) { return dim3(static_cast<unsigned>((n + 255) / 256)); }
+
+void launch_lin(const LinArgs& a, int variant, hipStream_t s) {
+    const dim3 grid(static_cast<unsigned>((a.sc.P + 255) / 256), static_cast<unsigned>(a.nsteps));
+    switch (variant) {
+        case PH_DDR: hipLaunchKernelGGL(k_adj_lin<PH_DDR>, grid, dim3(256), 0, s, a); break;
+        case PH_POS: hipLaunchKernelGGL(k_adj_lin<PH_POS>, grid, dim3(256), 0, s, a); break;
+        default: hipLaunchKernelGGL(k_adj_lin<PH_DEFAULT>, grid, dim3(256), 0, s, a); break;
     }
+}
+void launch_sweep(const SweepArgs& a, int rpl, int grid, hipStream_t s) {
+    if (rpl == 2) hipLaunchKernelGGL(k_adj_sweep<2>, dim3(grid), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL(k_adj_sweep<1>, dim3(grid), dim3(64), 0, s, a);
+}
+void launch_fin(const FinArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_fin, g1(a.P), dim3(256), 0, s, a);
 }
 void launch_tri_fwd(const TriArgs& a, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_tri_fwd, dim3(grid), dim3(64), 0, s, a);
@@ -626,56 +712,61 @@ void launch_tri_adj(const TriArgs& a, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_tri_adj, dim3(grid), dim3(64), 0, s, a);
 }
 
-static inline dim3 g1(size_t n) { return dim3(static_cast<unsigned>((n + 255) / 256)); }
-
 void launch_grad_a(int nnz, const int* row, const int* col, const float* gb, const float* x,
                    float* ga, hipStream_t s) {
     hipLaunchKernelGGL(k_grad_a, g1(nnz), dim3(256), 0, s, nnz, row, col, gb, x, ga);
 }
-void launch_bind_statics(int R, const int* perm, const float* n, const float* qsp,
-                         const float* psp, const float* len, const float* slope,
-                         const float* xst, float slope_lb, const Statics& st, hipStream_t s) {
-    hipLaunchKernelGGL(k_bind_statics, g1(R), dim3(256), 0, s, R, perm, n, qsp, psp, len, slope,
-                       xst, slope_lb, st);
+void launch_bind_statics(const Sched& sc, const float* n, const float* qsp, const float* psp,
+                         const float* len, const float* slope, const float* xst, float slope_lb,
+                         float4* st, hipStream_t s) {
+    hipLaunchKernelGGL(k_bind_statics, g1(sc.P), dim3(256), 0, s, sc, n, qsp, psp, len, slope, xst,
+                       slope_lb, st);
 }
 void launch_state(const Sched& sc, const float* q, int clamp, float lb, float* st_q,
                   float* st_i, float* Q_row0, hipStream_t s) {
-    hipLaunchKernelGGL(k_state_q, g1(sc.R), dim3(256), 0, s, sc.R, sc.perm, q, clamp, lb, st_q,
-                       Q_row0);
-    hipLaunchKernelGGL(k_state_i, g1(sc.R), dim3(256), 0, s, sc, st_q, st_i);
+    hipLaunchKernelGGL(k_state_q, g1(sc.P), dim3(256), 0, s, sc, q, clamp, lb, st_q, Q_row0);
+    hipLaunchKernelGGL(k_state_i, g1(sc.P), dim3(256), 0, s, sc, st_q, st_i);
 }
-void launch_hist_from_x(int R, const int* perm, const float* x, uint64_t* hist, hipStream_t s) {
-    hipLaunchKernelGGL(k_hist_from_x, g1(R), dim3(256), 0, s, R, perm, x, hist);
+void launch_hist_from_x(const Sched& sc, const float* x, uint64_t* hist, hipStream_t s) {
+    hipLaunchKernelGGL(k_hist_from_x, g1(sc.P), dim3(256), 0, s, sc, x, hist);
 }
-void launch_lam_init(int R, const int* perm, const float* g, float* lam, float* a0, float* a1,
-                     float* a2, hipStream_t s) {
-    hipLaunchKernelGGL(k_lam_init, g1(R), dim3(256), 0, s, R, perm, g, lam, a0, a1, a2);
+void launch_lam_init(const Sched& sc, const float* g, const uint64_t* hist_row, float lb,
+                     float* lam, float* a0, float* a1, float* a2, hipStream_t s) {
+    hipLaunchKernelGGL(k_lam_init, g1(sc.P), dim3(256), 0, s, sc, g, hist_row, lb, lam, a0, a1,
+                       a2);
 }
-void launch_unpermute(int R, const int* perm, const float* src, float* dst, hipStream_t s) {
-    hipLaunchKernelGGL(k_unpermute, g1(R), dim3(256), 0, s, R, perm, src, dst);
+void launch_unpermute(const Sched& sc, const float* src, float* dst, hipStream_t s) {
+    hipLaunchKernelGGL(k_unpermute, g1(sc.P), dim3(256), 0, s, sc, src, dst);
 }
-void launch_grad_q0(int R, const int* perm, const float* lam, const float* q0, float lb,
-                    float* g, hipStream_t s) {
-    hipLaunchKernelGGL(k_grad_q0, g1(R), dim3(256), 0, s, R, perm, lam, q0, lb, g);
+void launch_grad_q0(const Sched& sc, const float* lam, const float* q0, float lb, float* g,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(k_grad_q0, g1(sc.P), dim3(256), 0, s, sc, lam, q0, lb, g);
 }
 void launch_zero(size_t n, float* x, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_zero_f, g1(n), dim3(256), 0, s, n, x);
 }
 
-int occupancy_blocks(int which) {
+template <class K>
+static int occ(K kernel) {
     int nb = 0;
-    hipError_t e = hipSuccess;
-    switch (which) {
-        case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fwd_window<PH_DEFAULT>, 64, 0); break;
-        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fwd_window<PH_DDR>, 64, 0); break;
-        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fwd_window<PH_POS>, 64, 0); break;
-        case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_adj_window<PH_DEFAULT>, 64, 0); break;
-        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_adj_window<PH_DDR>, 64, 0); break;
-        case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_adj_window<PH_POS>, 64, 0); break;
-        case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tri_fwd, 64, 0); break;
-        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tri_adj, 64, 0); break;
+    return (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 64, 0) == hipSuccess) ? nb : 0;
+}
+
+int occupancy_blocks(int kernel, int variant, int rpl) {
+    switch (kernel) {
+        case OCC_FWD:
+            if (rpl == 2) {
+                if (variant == PH_DDR) return occ(k_fwd_window<PH_DDR, 2>);
+                if (variant == PH_POS) return occ(k_fwd_window<PH_POS, 2>);
+                return occ(k_fwd_window<PH_DEFAULT, 2>);
+            }
+            if (variant == PH_DDR) return occ(k_fwd_window<PH_DDR, 1>);
+            if (variant == PH_POS) return occ(k_fwd_window<PH_POS, 1>);
+            return occ(k_fwd_window<PH_DEFAULT, 1>);
+        case OCC_SWEEP: return (rpl == 2) ? occ(k_adj_sweep<2>) : occ(k_adj_sweep<1>);
+        case OCC_TRI_FWD: return occ(k_tri_fwd);
+        default: return occ(k_tri_adj);
     }
-    return (e == hipSuccess) ? nb : 0;
 }
 
 }  // namespace ddrs

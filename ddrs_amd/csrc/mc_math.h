@@ -76,16 +76,10 @@ DDRS_HD inline double exp_reduced(double t) {
     return ldexp(p, static_cast<int>(k));
 }
 
-// x^y for fp32 x, y, rounded once to fp32. Special operands follow C99 pow for
-// the cases the physics can produce: x = 0, x = +inf, NaN, negative x with
-// y = 2 (exact square) and negative x otherwise (NaN).
-DDRS_HD inline __attribute__((noinline)) float pow_f32(float x, float y) {
-    const double xd = static_cast<double>(x), yd = static_cast<double>(y);
-    if (xd > 0.0 && xd < INFINITY && yd == yd && fabs(yd) < INFINITY) {
-        double t = yd * ln_reduced(xd);
-        t = fmin(fmax(t, -800.0), 800.0);  // beyond this the fp32 result is 0 or inf
-        return static_cast<float>(exp_reduced(t));
-    }
+// C99 pow results for the operands the fast path leaves out (x <= 0, x = +inf,
+// NaN or infinite y), for the cases the physics can produce: x = 0, x = +inf,
+// NaN, negative x with y = 2 (exact square) and negative x otherwise (NaN).
+DDRS_HD inline float pow_special(double xd, double yd) {
     if (xd != xd || yd != yd) return (yd == 0.0) ? 1.0f : static_cast<float>(xd + yd);
     if (yd == 0.0) return 1.0f;
     if (xd == 0.0) return (yd > 0.0) ? 0.0f : static_cast<float>(INFINITY);
@@ -94,13 +88,55 @@ DDRS_HD inline __attribute__((noinline)) float pow_f32(float x, float y) {
     return static_cast<float>(NAN);
 }
 
-// ln x for fp32 x, rounded once to fp32 (C99 log for 0, negative, inf, NaN)
-DDRS_HD inline __attribute__((noinline)) float log_f32(float x) {
-    const double xd = static_cast<double>(x);
-    if (xd > 0.0 && xd < INFINITY) return static_cast<float>(ln_reduced(xd));
+// C99 log for x <= 0, x = +inf and NaN
+DDRS_HD inline float log_special(double xd) {
     if (xd == 0.0) return -static_cast<float>(INFINITY);
     if (xd == INFINITY) return static_cast<float>(INFINITY);
     return static_cast<float>(NAN);
+}
+
+DDRS_HD inline bool pow_fast_path(double xd, double yd) {
+    return xd > 0.0 && xd < INFINITY && yd == yd && fabs(yd) < INFINITY;
+}
+
+// e^(y ln x) rounded once to fp32
+DDRS_HD inline float pow_from_ln(double l, double yd) {
+    double t = yd * l;
+    t = fmin(fmax(t, -800.0), 800.0);  // beyond this the fp32 result is 0 or inf
+    return static_cast<float>(exp_reduced(t));
+}
+
+// x^y for fp32 x, y, rounded once to fp32.
+DDRS_HD inline __attribute__((noinline)) float pow_f32(float x, float y) {
+    const double xd = static_cast<double>(x), yd = static_cast<double>(y);
+    if (pow_fast_path(xd, yd)) return pow_from_ln(ln_reduced(xd), yd);
+    return pow_special(xd, yd);
+}
+
+// ln x for fp32 x, rounded once to fp32
+DDRS_HD inline __attribute__((noinline)) float log_f32(float x) {
+    const double xd = static_cast<double>(x);
+    if (xd > 0.0 && xd < INFINITY) return static_cast<float>(ln_reduced(xd));
+    return log_special(xd);
+}
+
+// pow_f32(x, y) and log_f32(x) from one evaluation of ln x; both results are
+// bit-identical to the separate calls.
+struct PowLn {
+    float pw, ln;
+};
+DDRS_HD inline __attribute__((noinline)) PowLn pow_ln_f32(float x, float y) {
+    const double xd = static_cast<double>(x), yd = static_cast<double>(y);
+    PowLn r;
+    if (xd > 0.0 && xd < INFINITY) {
+        const double l = ln_reduced(xd);
+        r.ln = static_cast<float>(l);
+        r.pw = pow_fast_path(xd, yd) ? pow_from_ln(l, yd) : pow_special(xd, yd);
+    } else {
+        r.ln = log_special(xd);
+        r.pw = pow_special(xd, yd);
+    }
+    return r;
 }
 
 }  // namespace ddrs

@@ -1,6 +1,7 @@
 // Per-reach Muskingum-Cunge physics for gfx950, held in registers.
 //
-// mc_coef:  S1..S23 of forward_chain_inner (src/routing/mmc_op.rs:1113-1236).
+// mc_coef:  S1..S23 of forward_chain_inner (src/routing/mmc_op.rs:1113-1236),
+//           with the time-invariant S1/S3/S5/S15 terms precomputed per reach.
 // mc_local_bwd: the per-reach part of timestep_backward_core B25..B1
 //           (src/routing/mmc_op.rs:340-835); the two network terms (the
 //           transposed-solve result gb and the N^T gi sum) come in as arguments.
@@ -33,7 +34,6 @@ constexpr float kDelta = 1e-2f;  // POSITIVITY_DELTA, mmc_op.rs:57
 // on gfx950); every disagreement seeds a rounding difference that the time
 // loop amplifies.
 __device__ __forceinline__ float rpow(float x, float y) { return pow_f32(x, y); }
-__device__ __forceinline__ float rlog(float x) { return log_f32(x); }
 
 __device__ __forceinline__ float fmaxr(float x, float lo) { return (x < lo) ? lo : x; }
 __device__ __forceinline__ float fclamp(float x, float lo, float hi) {
@@ -41,46 +41,76 @@ __device__ __forceinline__ float fclamp(float x, float lo, float hi) {
 }
 __device__ __forceinline__ float fminr(float a, float b) { return (b < a) ? b : a; }
 
+// Per-reach inputs that do not change over a window: the parameters and the
+// reference's time-invariant subexpressions, evaluated once per window with
+// the reference's own ops (so every step sees the bits it would compute).
+struct Reach {
+    float n, p, len, slope;         // slope after the slope_lb clamp
+    float sqrt_s, xst, q_eps, qe1;  // sqrt(slope); x_storage; S1 q_spatial + 1e-6; S2 q_eps + 1
+    float denominator, exponent;    // S3 p sqrt(S) + 1e-8; S5 (1 / (3 q_eps + 5)) * 3
+    float inv_n, f53sq;             // S15 1 / n; B5 (3 q_eps + 5)^2
+};
+
+// Reach statics in HBM: three float4 per reach, schedule order.
+__device__ __forceinline__ Reach load_reach(const float4* st, int p) {
+    const float4 a = st[3 * p], b = st[3 * p + 1], c = st[3 * p + 2];
+    Reach r;
+    r.n = a.x; r.p = a.y; r.len = a.z; r.slope = a.w;
+    r.sqrt_s = b.x; r.xst = b.y; r.q_eps = b.z; r.qe1 = b.w;
+    r.denominator = c.x; r.exponent = c.y; r.inv_n = c.z; r.f53sq = c.w;
+    return r;
+}
+
+__device__ __forceinline__ void make_reach(float n, float qsp, float psp, float len, float slope,
+                                           float xst, float4* out) {
+    const float sqrt_s = sqrtf(slope);
+    const float q_eps = qsp + 1e-6f;
+    const float f53 = q_eps * 3.0f + 5.0f;
+    out[0] = make_float4(n, psp, len, slope);
+    out[1] = make_float4(sqrt_s, xst, q_eps, q_eps + 1.0f);
+    out[2] = make_float4(psp * sqrt_s + 1e-8f, (1.0f / f53) * 3.0f, 1.0f / n, f53 * f53);
+}
+
 struct Coef {
-    float q_eps, ratio, denominator, depth, tw, ss_raw, ss, bw_raw, bw, area, wp, hyd_r;
+    float ratio, ln_ratio, depth, ln_depth, tw, ss_raw, ss, bw_raw, bw, hyd_r;
     float v_un, v_cl, cel, k_musk, x_eff, denom, c1, c2, c3, c4;
 };
 
-// S1..S23. sqrt_s = sqrtf(slope) (S3 and S15 both take sqrt(slope)).
+// S1..S23 for out[t] = q. ln_ratio and ln_depth are log_f32 of ratio and depth
+// (needed by B6 and B7), taken from the same ln evaluation as the powers.
 template <int PH>
-__device__ __forceinline__ void mc_coef(const Phys& P, float nman, float qsp, float psp, float q,
-                                        float length, float slope, float sqrt_s, float xst,
-                                        Coef& o) {
-    o.q_eps = qsp + 1e-6f;                                              // S1
-    float numerator = (q * nman) * (o.q_eps + 1.0f);                    // S2
-    o.denominator = psp * sqrt_s + 1e-8f;                               // S3
-    o.ratio = numerator / o.denominator;                                // S4
-    float exponent = (1.0f / (o.q_eps * 3.0f + 5.0f)) * 3.0f;          // S5
-    o.depth = fmaxr(rpow(o.ratio, exponent), P.depth_lb);               // S6
-    o.tw = psp * rpow(o.depth, o.q_eps);                                // S7
-    o.ss_raw = (o.tw * o.q_eps) / (o.depth * 2.0f);                     // S8
+__device__ __forceinline__ void mc_coef(const Phys& P, const Reach& r, float q, Coef& o) {
+    float numerator = (q * r.n) * r.qe1;                                // S2
+    o.ratio = numerator / r.denominator;                                // S3, S4
+    const PowLn d = pow_ln_f32(o.ratio, r.exponent);                    // S5
+    o.ln_ratio = d.ln;
+    o.depth = fmaxr(d.pw, P.depth_lb);                                  // S6
+    const PowLn w = pow_ln_f32(o.depth, r.q_eps);
+    o.ln_depth = w.ln;
+    o.tw = r.p * w.pw;                                                  // S7
+    o.ss_raw = (o.tw * r.q_eps) / (o.depth * 2.0f);                     // S8
     o.ss = fclamp(o.ss_raw, 0.5f, 50.0f);                               // S9
     o.bw_raw = o.tw - (o.ss * o.depth) * 2.0f;                          // S10
     o.bw = fmaxr(o.bw_raw, P.bw_lb);                                    // S11
-    o.area = ((o.tw + o.bw) * o.depth) / 2.0f;                          // S12
-    float root = sqrtf(rpow(o.ss, 2.0f) + 1.0f);
-    o.wp = o.bw + (o.depth * root) * 2.0f;                              // S13
-    o.hyd_r = o.area / o.wp;                                            // S14
-    o.v_un = ((1.0f / nman) * rpow(o.hyd_r, 2.0f / 3.0f)) * sqrt_s;     // S15
+    const float area = ((o.tw + o.bw) * o.depth) / 2.0f;                // S12
+    const float root = sqrtf(o.ss * o.ss + 1.0f);
+    const float wp = o.bw + (o.depth * root) * 2.0f;                    // S13
+    o.hyd_r = area / wp;                                                // S14
+    o.v_un = (r.inv_n * rpow(o.hyd_r, 2.0f / 3.0f)) * r.sqrt_s;         // S15
     o.v_cl = fclamp(o.v_un, P.v_lb, 15.0f);                             // S16
     if (PH == PH_DDR) {                                                 // S17
         o.cel = o.v_cl * (5.0f / 3.0f);
     } else {
-        float beta = ((-(o.area * root)) / (o.tw * o.wp)) * (4.0f / 3.0f) + (5.0f / 3.0f);
+        float beta = ((-(area * root)) / (o.tw * wp)) * (4.0f / 3.0f) + (5.0f / 3.0f);
         o.cel = o.v_cl * beta;
     }
-    float k_raw = length / o.cel;                                       // S18
+    float k_raw = r.len / o.cel;                                        // S18
     o.k_musk = (PH == PH_POS) ? fmaxr(k_raw, P.dt * (1.0f + kDelta) / 2.0f) : k_raw;
     if (PH == PH_DDR) {                                                 // S19
-        o.x_eff = xst;
+        o.x_eff = r.xst;
     } else {
-        float w = q / (((o.tw * slope) * o.cel) * length + 1e-12f);
-        float x_cunge = fclamp(((-w) + 1.0f) * 0.5f, 0.0f, 0.5f);
+        float wq = q / (((o.tw * r.slope) * o.cel) * r.len + 1e-12f);
+        float x_cunge = fclamp(((-wq) + 1.0f) * 0.5f, 0.0f, 0.5f);
         if (PH == PH_POS) {
             float cr = (1.0f / o.k_musk) * P.dt;
             float hi_a = cr * (0.5f * (1.0f - kDelta));
@@ -109,13 +139,18 @@ struct LocalGrads {
 // assemble-backward value; i_t: N q_t at this reach; gq24: (N^T gi) at this
 // reach. Geometry is recomputed from the saved primitives exactly as the
 // reference backward does (wp from S13's operands, area = R wp).
+//
+// Given the forward state, every output is linear in (gb, gc1, gq24): the
+// clamp and branch masks depend on the forward state only, and gq24 enters gq
+// alone. Since gc1 = gb * sum_k adj_k x[col k], the step's whole backward is
+// gb times the call with gb = 1, gc1 = that sum, gq24 = 0, plus gq24 in gq.
 template <int PH>
-__device__ __forceinline__ LocalGrads mc_local_bwd(const Phys& P, const Coef& c, float nman,
-                                                   float psp, float q, float qp, float length,
-                                                   float slope, float sqrt_s, float i_t,
-                                                   float gb, float gc1, float gq24) {
+__device__ __forceinline__ LocalGrads mc_local_bwd(const Phys& P, const Coef& c, const Reach& r,
+                                                   float q, float qp, float i_t, float gb,
+                                                   float gc1, float gq24) {
     LocalGrads g;
     const float dt = P.dt;
+    const float nman = r.n, psp = r.p, length = r.len, slope = r.slope, sqrt_s = r.sqrt_s;
     // B25
     float gc2 = gb * i_t;
     float gc3 = gb * q;
@@ -231,32 +266,30 @@ __device__ __forceinline__ LocalGrads mc_local_bwd(const Phys& P, const Coef& c,
     float gss_clamp = ((c.ss_raw > 0.5f) && (c.ss_raw < 50.0f)) ? gss_comb : 0.0f;
     // B8
     float two_d = c.depth * 2.0f;
-    float gtw8 = (gss_clamp * c.q_eps) / two_d;
+    float gtw8 = (gss_clamp * r.q_eps) / two_d;
     float gqeps8 = (gss_clamp * c.tw) / two_d;
-    float gd8 = (((-gss_clamp) * c.tw) * c.q_eps) / (two_d * c.depth);
+    float gd8 = (((-gss_clamp) * c.tw) * r.q_eps) / (two_d * c.depth);
     float gtw_total = (gtw12 + gtw10) + gtw8;
     if (PH != PH_DDR) gtw_total = gtw_total + bg_tw;
     if (PH != PH_DDR) gtw_total = gtw_total + xg_tw;
     // B7
     float gp7 = (gtw_total * c.tw) / psp;
-    float gdepth7 = ((gtw_total * c.tw) * c.q_eps) / c.depth;
-    float gqeps7 = (gtw_total * c.tw) * rlog(c.depth);
+    float gdepth7 = ((gtw_total * c.tw) * r.q_eps) / c.depth;
+    float gqeps7 = (gtw_total * c.tw) * c.ln_depth;
     // B6
     float gd_total = (((gd13 + gd12) + gd10) + gd8) + gdepth7;
     float gd_pre = (c.depth > P.depth_lb) ? gd_total : 0.0f;
-    float exponent = (1.0f / (c.q_eps * 3.0f + 5.0f)) * 3.0f;
-    float gratio6 = ((gd_pre * exponent) * c.depth) / c.ratio;
-    float gexp6 = (gd_pre * c.depth) * rlog(c.ratio);
+    float gratio6 = ((gd_pre * r.exponent) * c.depth) / c.ratio;
+    float gexp6 = (gd_pre * c.depth) * c.ln_ratio;
     // B5
-    float f53 = c.q_eps * 3.0f + 5.0f;
-    float gqeps5 = ((-gexp6) * 9.0f) / (f53 * f53);
+    float gqeps5 = ((-gexp6) * 9.0f) / r.f53sq;
     // B4
-    float gnum = gratio6 / c.denominator;
-    float gden = ((-gratio6) * c.ratio) / c.denominator;
+    float gnum = gratio6 / r.denominator;
+    float gden = ((-gratio6) * c.ratio) / r.denominator;
     // B3
     float gp3 = gden * sqrt_s;
     // B2
-    float qe1 = c.q_eps + 1.0f;
+    const float qe1 = r.qe1;
     float gq2 = (gnum * nman) * qe1;
     float gn2 = (gnum * q) * qe1;
     float gqeps2 = (gnum * q) * nman;

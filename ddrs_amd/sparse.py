@@ -136,6 +136,20 @@ class Workspace:
     def check(self):
         N.check(N.lib.ddrs_workspace_check(self.handle, _stream()), "ddrs_workspace_check")
 
+    def layout(self) -> dict:
+        """Chunk width (reaches per lane) and workgroup count of the persistent
+        forward and adjoint-sweep launches."""
+        v = [ctypes.c_int32() for _ in range(4)]
+        N.check(N.lib.ddrs_workspace_layout(self.handle, *[ctypes.byref(x) for x in v]))
+        return dict(fwd_reaches_per_lane=v[0].value, fwd_workgroups=v[1].value,
+                    sweep_reaches_per_lane=v[2].value, sweep_workgroups=v[3].value)
+
+    def set_chunk_width(self, fwd: int = 0, sweep: int = 0):
+        """Override the chunk width (1 or 2 reaches per lane; 0 keeps the
+        automatic choice). Results do not depend on it."""
+        N.check(N.lib.ddrs_workspace_set_chunk_width(self.handle, int(fwd), int(sweep)),
+                "ddrs_workspace_set_chunk_width")
+
 
 class _TriSolve(torch.autograd.Function):
     @staticmethod

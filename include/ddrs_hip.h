@@ -185,14 +185,27 @@ ddrs_status ddrs_hotstart_adj(ddrs_workspace_t ws, const ddrs_params* params,
                               float* d_grad_q_prime0, void* stream);
 
 /* ---------------------------------------------------------------------------
- * Timing of the dominant kernels of the last window call on its own stream
+ * Timing of the window kernels of the last window calls on their own stream
  * (HIP events around each launch): number of launches and summed milliseconds
- * of the forward-window and adjoint-window kernels. Enabled by
- * ddrs_set_timing(ws, 1).
+ * of the forward-window kernel and of the adjoint's three kernels
+ * (linearisation, reverse-time sweep, gradient finish). Enabled by
+ * ddrs_set_timing(ws, 1). No reference counterpart (the reference times with
+ * host clocks in benches/).
  * ------------------------------------------------------------------------- */
 ddrs_status ddrs_set_timing(ddrs_workspace_t ws, int32_t enable);
 ddrs_status ddrs_get_timing(ddrs_workspace_t ws, int32_t* fwd_launches, float* fwd_ms,
                             int32_t* adj_launches, float* adj_ms);
+ddrs_status ddrs_get_adj_phase_timing(ddrs_workspace_t ws, float* lin_ms, float* sweep_ms,
+                                      float* fin_ms);
+/* Launch shape of the persistent kernels: reaches per lane (chunk width / 64)
+ * and workgroups (= chunks), forward (default physics) and adjoint sweep. The
+ * workspace picks the narrowest width whose chunks fit on the GPU at once;
+ * set_chunk_width overrides it (0 keeps the current choice; results do not
+ * depend on the width). No reference counterpart. */
+ddrs_status ddrs_workspace_layout(ddrs_workspace_t ws, int32_t* rpl_fwd, int32_t* grid_fwd,
+                                  int32_t* rpl_sweep, int32_t* grid_sweep);
+ddrs_status ddrs_workspace_set_chunk_width(ddrs_workspace_t ws, int32_t rpl_fwd,
+                                           int32_t rpl_sweep);
 
 /* ---------------------------------------------------------------------------
  * Index construction (host, bit-exact): the reference's topological order

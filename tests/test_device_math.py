@@ -18,7 +18,7 @@ SRC = r'''
 static uint64_t st = 88172645463325252ull;
 static double u() { st ^= st << 13; st ^= st >> 7; st ^= st << 17; return (st >> 11) * (1.0 / 9007199254740992.0); }
 int main() {
-  long n = 0, d_libm = 0, d_cr = 0, dl_libm = 0, dl_cr = 0;
+  long n = 0, d_libm = 0, d_cr = 0, dl_libm = 0, dl_cr = 0, fused_bad = 0;
   double worst = 0.0;
   for (int k = 0; k < 3000000; ++k) {
     float x, y;
@@ -33,6 +33,8 @@ int main() {
     if (rel > worst) worst = rel;
     n++; d_libm += (a != powf(x, y)); d_cr += (a != cr);
     float lx = ddrs::log_f32(x);
+    ddrs::PowLn pl = ddrs::pow_ln_f32(x, y);
+    fused_bad += (pl.pw != a) || (pl.ln != lx);
     dl_libm += (lx != logf(x));
     dl_cr += (lx != (float)logl((long double)x));
   }
@@ -45,12 +47,27 @@ int main() {
     if (!((a == b) || (std::isnan(a) && std::isnan(b)) || fabs((double)a - b) <= 1e-7 * fabs((double)b))) {
       printf("# pow special %g %g: %g vs %g\n", xv, yv, a, b); special_ok = 0; }
   }
+  for (float xv : xs) for (float yv : ys) {
+    ddrs::PowLn pl = ddrs::pow_ln_f32(xv, yv);
+    float a = ddrs::pow_f32(xv, yv), l = ddrs::log_f32(xv);
+    if (!((pl.pw == a || (std::isnan(a) && std::isnan(pl.pw))) && (pl.ln == l || (std::isnan(l) && std::isnan(pl.ln))))) {
+      printf("# fused special %g %g\n", xv, yv); special_ok = 0; }
+  }
+  /* side slope squared: x*x against powf(x, 2) over [0.5, 50] */
+  long sq_libm = 0, sq_pow = 0, nsq = 0;
+  for (float xv = 0.5f; xv <= 50.0f; xv = nextafterf(xv, 100.0f)) {
+    if ((nsq++ & 15) != 0) continue;
+    float b = powf(xv, 2.0f);
+    sq_libm += (xv * xv != b);
+    sq_pow += (ddrs::pow_f32(xv, 2.0f) != b);
+  }
   for (float xv : xs) {
     float a = ddrs::log_f32(xv), b = logf(xv);
     if (!((a == b) || (std::isnan(a) && std::isnan(b)) || fabs((double)a - b) <= 1e-7 * fabs((double)b))) {
       printf("# log special %g: %g vs %g\n", xv, a, b); special_ok = 0; }
   }
-  printf("%ld %ld %ld %ld %.3e %d %ld\n", n, d_libm, d_cr, dl_libm, worst, special_ok, dl_cr);
+  printf("%ld %ld %ld %ld %.3e %d %ld %ld %ld %ld\n", n, d_libm, d_cr, dl_libm, worst, special_ok, dl_cr,
+         fused_bad, sq_libm, sq_pow);
   return 0;
 }
 '''
@@ -68,6 +85,7 @@ def test_device_pow_log_match_libm():
     out = [ln for ln in raw.splitlines() if not ln.startswith('#')][0].split()
     n, d_libm, d_cr, dl_libm = (int(v) for v in out[:4])
     worst, special_ok, dl_cr = float(out[4]), int(out[5]), int(out[6])
+    fused_bad, sq_libm, sq_pow = int(out[7]), int(out[8]), int(out[9])
     print(f"pow_f32 vs libm powf: {d_libm}/{n} differ; vs correctly rounded: {d_cr}/{n}; "
           f"log_f32 vs logf: {dl_libm}/{n}, vs correctly rounded {dl_cr}/{n}; "
           f"worst rel err of the fp32 result {worst:.3e}")
@@ -77,3 +95,9 @@ def test_device_pow_log_match_libm():
     assert dl_libm <= n * 5e-3         # libm logf misrounds on ~0.2% of these
     assert worst <= 2.0 ** -24 * 1.0001  # within half an ulp (+ ties)
     assert special_ok == 1
+    # pow_ln_f32 (one ln evaluation for both results) is bit-identical to the pair
+    assert fused_bad == 0
+    # S13 squares the side slope with x*x: libm powf(x, 2) is not correctly rounded
+    # either, and x*x disagrees with it no more often than pow_f32 does
+    print(f"side slope square vs powf(x,2): x*x {sq_libm}, pow_f32 {sq_pow} differ")
+    assert sq_libm <= 1.1 * sq_pow + 50

@@ -341,6 +341,42 @@ def test_structures(kind):
         grad_close(name, got[name], radj[name], radj64[name])
 
 
+@pytest.mark.parametrize("kind", ["tree", "long_chain", "forest"])
+def test_chunk_width_invariance(kind):
+    """64-reach and 128-reach chunks (one and two reaches per lane) give
+    bit-identical discharge and gradients: the width changes which workgroup
+    solves a reach, not the arithmetic or the CSR summation order. The
+    128-reach result is also checked against the oracle."""
+    say("chunk width", kind)
+    rng = np.random.default_rng(21)
+    if kind == "tree":
+        c = Case(5000, 30, seed=22, storms=True)
+    elif kind == "long_chain":
+        rows, cols = chain(700)
+        c = Case(700, 12, seed=23, rows=rows, cols=cols)
+    else:
+        rows, cols, off = [], [], 0
+        for size in [3, 130, 127, 1, 400, 257]:
+            r, cl = random_tree(size, 8, rng) if size > 1 else (np.zeros(0, np.int32),) * 2
+            rows.append(r + off)
+            cols.append(cl + off)
+            off += size
+        rows, cols = np.concatenate(rows).astype(np.int32), np.concatenate(cols).astype(np.int32)
+        c = Case(off, 16, seed=24, rows=rows, cols=cols)
+    gQ = rng.uniform(0, 1, (c.T, c.R)).astype(np.float32)
+    ws = c.net.workspace(c.T - 1, c.T - 1)
+    out = {}
+    for rpl in (1, 2):
+        ws.set_chunk_width(rpl, rpl)
+        lay = ws.layout()
+        assert lay["fwd_reaches_per_lane"] == rpl and lay["sweep_reaches_per_lane"] == rpl
+        assert lay["fwd_workgroups"] == -(-c.net.n_chunks // rpl)
+        out[rpl] = c.gpu_window(D.FORM_PERSISTENT, gQ=gQ)
+    for k in out[1]:
+        np.testing.assert_array_equal(out[1][k], out[2][k], err_msg=k)
+    check_discharge(kind, out[2]["Q"], c.oracle_Q(), c.oracle_Q(np.float64))
+
+
 @pytest.mark.parametrize("variant", ["ddr_match", "positivity"])
 def test_window_physics_variants(variant):
     say("window variant", variant)
