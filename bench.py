@@ -28,8 +28,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-FWD_BYTES = 40    # algorithmic bytes per reach-timestep, forward
-ADJ_BYTES = 72    # adjoint
+# Algorithmic HBM bytes per reach-timestep of each window kernel:
+#   forward window: q' read 4, Q written 4, x history written 8 and read back
+#     8 by the downstream chunks (one link per reach in a tree);
+#   linearise: x history of the step and the previous step with their
+#     upstream links 4 x 8, q' 4, dL/dQ 4; records written 16 + 16 + 1;
+#   sweep: records read 16 + 1, (P, G) published 16 and read 16, gb written 4;
+#   finish: records read 16, gb read 4, dL/dq' written 4.
+KERNEL_BYTES = {"fwd": 24, "lin": 73, "sweep": 53, "fin": 24}
+KERNEL_NAMES = {"fwd": "k_fwd_window (persistent forward)", "lin": "k_adj_lin (linearise)",
+                "sweep": "k_adj_sweep (persistent reverse sweep)", "fin": "k_adj_fin"}
 CONFIGS = {
     "c1": dict(R=5000, T=240, window=256, storms=False),
     "c2": dict(R=350000, T=720, window=256, storms=True),
@@ -49,6 +57,12 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=48)
+    ap.add_argument("--sublevels", type=int, default=0,
+                    help="largest chain of links in one schedule block (0: library default)")
+    ap.add_argument("--block", type=int, default=0, help="schedule block, 64 or 128 (0: default)")
+    ap.add_argument("--pmc-calibration", action="store_true",
+                    help="first copy 1 GiB device to device once, as the known-size reference "
+                         "for FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -61,6 +75,12 @@ def main():
     from ddrs_amd import network as NW
     from ddrs_amd.sparse import Network, SparseAdjacency
 
+    if args.pmc_calibration:
+        a = torch.empty(2**28, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     cfg = CONFIGS[args.config]
     R, T = cfg["R"], cfg["T"]
     ci = 1 if args.config == "c1" else 2
@@ -71,7 +91,7 @@ def main():
     at = NW.gen_attributes(R, 2000 + ci)
     adj = SparseAdjacency.from_coo(R, rows, cols, None, at["length"], at["slope"])
     t0 = time.time()
-    net = Network(adj)
+    net = Network(adj, args.sublevels, args.block)
     log(f"[bench] plan {time.time() - t0:.2f}s: {net.n_chunks} chunks, {net.n_levels} levels")
     f = lambda x: torch.as_tensor(np.ascontiguousarray(x, np.float32), device=dev)
     n, qsp = f(at["n"]), f(at["q_spatial"])
@@ -166,13 +186,17 @@ def main():
                      adj_kernel_ms=float(a[3][0]), bit_identical_to_persistent=same)
         log(f"[bench] plain: {1000 * tp:.1f} ms/step ({value * tp / rts:.1f}x slower), identical={same}")
 
-    adj_launch_ms = adj_ms / max(na, 1)
-    fwd_launch_ms = fwd_ms / max(nf, 1)
-    steps_per_launch = T - 1
-    roof_adj = ADJ_BYTES * R * steps_per_launch / (adj_launch_ms * 1e-3) / 1e9
-    roof_fwd = FWD_BYTES * R * steps_per_launch / (fwd_launch_ms * 1e-3) / 1e9
+    # per-launch times of the persistent form (one launch of each kernel per step)
+    launch_ms = {"fwd": fwd_ms / args.steps, "lin": phase_ms[0], "sweep": phase_ms[1],
+                 "fin": phase_ms[2]}
+    rs = R * (T - 1)  # reach-steps per launch
+    kernels = {k: {"launch_ms": v, "bytes_per_reach_step": KERNEL_BYTES[k],
+                   "achieved_GBs": KERNEL_BYTES[k] * rs / (v * 1e-3) / 1e9}
+               for k, v in launch_ms.items()}
+    dom = max(launch_ms, key=launch_ms.get)
+    roof = kernels[dom]["achieved_GBs"]
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_adj_window_{args.config}.json")
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{dom}_window_{args.config}.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
@@ -198,12 +222,13 @@ def main():
                                f"{T} hourly steps, forward + adjoint, MSE at 64 gauges",
                    "reaches": R, "timesteps": T, "levels": net.n_levels, "form": "persistent",
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": roof_adj, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": roof_adj / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_adj_window (persistent adjoint)",
-                     "bytes_per_reach_step": ADJ_BYTES, "launch_ms": adj_launch_ms,
-                     "fwd_kernel": {"achieved": roof_fwd, "bytes_per_reach_step": FWD_BYTES,
-                                    "launch_ms": fwd_launch_ms}},
+        "roofline": {"bound": "hbm", "achieved": roof, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": roof / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": KERNEL_NAMES[dom], "launch_ms": launch_ms[dom],
+                     "bytes_per_reach_step": KERNEL_BYTES[dom], "reach_steps_per_launch": rs,
+                     "note": "latency- and VALU-bound persistent pipeline; see kernels for "
+                             "every window kernel",
+                     "kernels": kernels, "layout": layout},
         "plain": plain,
         "batched_over_plain": (value / plain["value"]) if plain else None,
         "grad_check": grad_check(torch, D, NW, Network, SparseAdjacency),

@@ -41,6 +41,7 @@ _SIGS = {
     "ddrs_last_error": (ctypes.c_char_p, []),
     "ddrs_version": (ctypes.c_char_p, []),
     "ddrs_plan_create": (c_int, [c_int, c_int, P, P, P, ctypes.POINTER(P)]),
+    "ddrs_plan_create_ex": (c_int, [c_int, c_int, P, P, P, c_int, c_int, ctypes.POINTER(P)]),
     "ddrs_plan_destroy": (None, [P]),
     "ddrs_plan_info": (c_int, [P, P, P, P, P]),
     "ddrs_plan_get_csr": (c_int, [P, P, P, P, P, P, P, P, P]),
@@ -64,6 +65,8 @@ _SIGS = {
     "ddrs_get_adj_phase_timing": (c_int, [P, P, P, P]),
     "ddrs_workspace_layout": (c_int, [P, P, P, P, P]),
     "ddrs_workspace_set_chunk_width": (c_int, [P, c_int, c_int]),
+    "ddrs_set_trace": (c_int, [P, c_int]),
+    "ddrs_get_trace": (c_int, [P, c_int, P, c_int, P]),
     "ddrs_build_topology": (c_int, [c_int, P, P, c_int, P, P, P, c_int, P, P, c_int, P]),
     "ddrs_compress": (c_int, [c_int, P, P, c_int, P, P, c_int, P, P, P, P, P]),
     "ddrs_gen_tree": (c_int, [c_int, c_int, c_u64, P, P]),

@@ -49,6 +49,11 @@ struct ChunkHost {
     bool ok = false;  // every chunk's links fit the LDS staging capacity
     std::vector<unsigned char> sub, subr;
     std::vector<int> nsub, nsubr;
+    // representative links per chunk (index relative to the chunk's first
+    // link), one per distinct producing chunk: upstream (forward), downstream
+    // (adjoint)
+    std::vector<int> rep_up_ptr, rep_dn_ptr;
+    std::vector<short> rep_up, rep_dn;
     Chunks dev{};
 };
 
@@ -98,6 +103,9 @@ struct ddrs_workspace {
     bool timing = false;
     std::vector<hipEvent_t> ev[4];  // forward window, lin, sweep, fin
     int n_ev[4] = {0, 0, 0, 0};
+    // per-chunk traces of the last forward and sweep launches (null: off)
+    unsigned long long* trace[2] = {nullptr, nullptr};
+    int trace_n[2] = {0, 0};
 };
 
 namespace {
@@ -158,8 +166,21 @@ const char* ddrs_version(void) { return "ddrs_hip 0.1 (gfx950)"; }
 
 ddrs_status ddrs_plan_create(int32_t n, int32_t nnz_off, const int32_t* h_rows,
                              const int32_t* h_cols, const float* h_vals, ddrs_plan_t* out) {
+    return ddrs_plan_create_ex(n, nnz_off, h_rows, h_cols, h_vals, 0, 0, out);
+}
+
+ddrs_status ddrs_plan_create_ex(int32_t n, int32_t nnz_off, const int32_t* h_rows,
+                                const int32_t* h_cols, const float* h_vals,
+                                int32_t max_sub_levels, int32_t block, ddrs_plan_t* out) {
     if (!out) return fail(DDRS_ERR_INVALID, "out is NULL");
     *out = nullptr;
+    if (max_sub_levels < 0 || max_sub_levels > 64)
+        return fail(DDRS_ERR_INVALID, "max_sub_levels must be in [0, 64] (0: default)");
+    if (block != 0 && block != 64 && block != 128)
+        return fail(DDRS_ERR_INVALID, "block must be 0 (default), 64 or 128");
+    const int sub_cap = max_sub_levels ? max_sub_levels : kMaxSubLevels;
+    const int blk = block ? block : kLayoutBlock;
+    const int blk_shift = (blk == 128) ? 7 : 6;
     if (n <= 0) return fail(DDRS_ERR_INVALID, "network must have at least one reach");
     if (nnz_off < 0 || (nnz_off > 0 && (!h_rows || !h_cols)))
         return fail(DDRS_ERR_INVALID, "bad COO arrays");
@@ -242,12 +263,13 @@ ddrs_status ddrs_plan_create(int32_t n, int32_t nnz_off, const int32_t* h_rows,
     for (int l = 0; l <= maxlev; ++l) lc[l + 1] += lc[l];
     pl->order.resize(R);
     for (int i = 0; i < R; ++i) pl->order[lc[pl->level[i]]++] = i;
-    // Positions: the level-major order cut into 64-position chunks of at most
-    // 64 reaches and at most kMaxSubLevels sub-levels (the longest chain of
-    // links inside the chunk). A chunk is closed early when the next reach
+    // Positions: the level-major order cut into blocks of blk positions holding
+    // at most blk reaches and at most sub_cap sub-levels (the longest chain of
+    // links inside the block). A block is closed early when the next reach
     // would deepen it past the cap, and its remaining positions stay empty, so
     // a thin chain is spread over several chunks that pipeline over time
-    // steps instead of one chunk walking the whole chain every step.
+    // steps instead of one chunk walking the whole chain every step. Kernel
+    // chunks are 64 or 128 consecutive positions.
     std::vector<int> inv(R, -1), subl(R, 0);
     {
         int cur = 0, fill = 0;
@@ -256,19 +278,21 @@ ddrs_status ddrs_plan_create(int32_t n, int32_t nnz_off, const int32_t* h_rows,
             int sl = 0;
             for (int e = pl->crow[i]; e < pl->crow[i + 1]; ++e) {
                 const int u = pl->col[e];
-                if (u != i && (inv[u] >> 6) == cur) sl = std::max(sl, subl[u] + 1);
+                if (u != i && (inv[u] >> blk_shift) == cur) sl = std::max(sl, subl[u] + 1);
             }
-            if (fill == 64 || sl >= kMaxSubLevels) {
+            if (fill == blk || sl >= sub_cap) {
                 ++cur;
                 fill = 0;
                 sl = 0;
             }
-            inv[i] = cur * 64 + fill;
+            inv[i] = cur * blk + fill;
             subl[i] = sl;
             ++fill;
         }
-        pl->C = cur + 1;
-        pl->P = 128 * ((pl->C + 1) / 2);  // whole chunks at both widths
+        pl->P = 128 * ((cur * blk / 64 + blk / 64 + 1) / 2);  // whole chunks at both widths
+        // 64-position chunks up to the last occupied one
+        pl->C = 0;
+        for (int i = 0; i < R; ++i) pl->C = std::max(pl->C, (inv[i] >> 6) + 1);
     }
     const int P = pl->P;
     pl->perm.assign(P, -1);
@@ -350,6 +374,44 @@ ddrs_status ddrs_plan_create(int32_t n, int32_t nnz_off, const int32_t* h_rows,
             }
             ch.nsubr[c] = mx + 1;
         }
+        // Representative links: for every other chunk that feeds chunk c, the
+        // link to the value that chunk publishes last in a step (highest
+        // sub-level of its solve, then highest position). Kept only where they
+        // are much fewer than the links themselves (at least 32 cross-chunk
+        // links and 4 per representative): a waiting chunk then polls the
+        // representatives and reads its links again once they arrive, which
+        // costs one more round trip than polling the links directly.
+        auto reps = [&](const std::vector<int>& ptr, const std::vector<int>& pos,
+                        const std::vector<unsigned char>& lvl, std::vector<int>& rptr,
+                        std::vector<short>& rl) {
+            rptr.assign(C + 1, 0);
+            rl.clear();
+            std::vector<std::pair<int, int>> best;  // (producer chunk, link)
+            for (int c = 0; c < C; ++c) {
+                const int p0 = c * cw, base = ptr[p0], end = ptr[p0 + cw];
+                best.clear();
+                for (int e = base; e < end; ++e) {
+                    const int q = pos[e], pc = q / cw;
+                    if (pc == c) continue;
+                    auto it = std::find_if(best.begin(), best.end(),
+                                           [&](const std::pair<int, int>& b) { return b.first == pc; });
+                    if (it == best.end()) {
+                        best.emplace_back(pc, e);
+                    } else {
+                        const int o = pos[it->second];
+                        if (lvl[q] > lvl[o] || (lvl[q] == lvl[o] && q > o)) it->second = e;
+                    }
+                }
+                int cross = 0;
+                for (int e = base; e < end; ++e) cross += (pos[e] / cw != c);
+                const int nb = static_cast<int>(best.size());
+                if (cross >= 32 && cross >= 4 * nb)
+                    for (const auto& b : best) rl.push_back(static_cast<short>(b.second - base));
+                rptr[c + 1] = static_cast<int>(rl.size());
+            }
+        };
+        reps(up_ptr, up_pos, ch.sub, ch.rep_up_ptr, ch.rep_up);
+        reps(dn_ptr, dn_pos, ch.subr, ch.rep_dn_ptr, ch.rep_dn);
     }
 
     pl->h_up_ptr.swap(up_ptr);
@@ -446,6 +508,16 @@ ddrs_status plan_to_device(ddrs_plan* pl) {
         UP(d_subr, ch.subr);
         UP(d_nsub, ch.nsub);
         UP(d_nsubr, ch.nsubr);
+        int *d_rup_ptr = nullptr, *d_rdn_ptr = nullptr;
+        short *d_rup = nullptr, *d_rdn = nullptr;
+        UP(d_rup_ptr, ch.rep_up_ptr);
+        UP(d_rdn_ptr, ch.rep_dn_ptr);
+        UP(d_rup, ch.rep_up);
+        UP(d_rdn, ch.rep_dn);
+        ch.dev.rep_up_ptr = d_rup_ptr;
+        ch.dev.rep_dn_ptr = d_rdn_ptr;
+        ch.dev.rep_up = d_rup;
+        ch.dev.rep_dn = d_rdn;
         ch.dev.C = ch.C;
         ch.dev.shift = 6 + wi;
         ch.dev.sub = d_sub;
@@ -629,6 +701,8 @@ static Claim next_claim(ddrs_workspace* ws, int grid) {
 static void launch_fwd_ws(ddrs_workspace* ws, FwdArgs a, const ddrs_params* p, hipStream_t s) {
     const int v = variant(p);
     a.claim = next_claim(ws, ws->grid_fwd[v]);
+    a.trace.rec = ws->trace[0];
+    ws->trace_n[0] = ws->grid_fwd[v];
     launch_fwd(a, v, ws->rpl_fwd[v], ws->grid_fwd[v], s);
 }
 
@@ -688,6 +762,8 @@ static void run_adj_segment(ddrs_workspace* ws, const ddrs_params* p, const AdjS
         L.nsteps = W.nsteps = F.nsteps = n;
         W.epoch = ++ws->epoch;
         W.claim = next_claim(ws, ws->grid_sweep);
+        W.trace.rec = ws->trace[1];
+        ws->trace_n[1] = ws->grid_sweep;
         record(ws, TK_LIN, s);
         launch_lin(L, v, s);
         record(ws, TK_LIN, s);
@@ -1057,6 +1133,37 @@ ddrs_status ddrs_get_adj_phase_timing(ddrs_workspace_t ws, float* lin_ms, float*
     if (st == DDRS_OK) st = sum_pairs(ws, TK_SWEEP, nullptr, sweep_ms);
     if (st == DDRS_OK) st = sum_pairs(ws, TK_FIN, nullptr, fin_ms);
     return st;
+}
+
+ddrs_status ddrs_set_trace(ddrs_workspace_t ws, int32_t enable) {
+    if (!ws) return fail(DDRS_ERR_INVALID, "workspace is NULL");
+    if (enable) {
+        for (int k = 0; k < 2; ++k) {
+            if (ws->trace[k]) continue;
+            // room for the most chunks any launch of this workspace can have
+            ddrs_status st = dalloc(ws, reinterpret_cast<void**>(&ws->trace[k]),
+                                    static_cast<size_t>(ws->plan->ch[0].C) * kTraceWords * 8);
+            if (st != DDRS_OK) return st;
+            HIPCHK(hipMemset(ws->trace[k], 0, static_cast<size_t>(ws->plan->ch[0].C) * kTraceWords * 8));
+        }
+    } else {
+        ws->trace[0] = ws->trace[1] = nullptr;  // buffers stay with the workspace
+    }
+    return DDRS_OK;
+}
+
+ddrs_status ddrs_get_trace(ddrs_workspace_t ws, int32_t which, uint64_t* h_rec, int32_t cap,
+                           int32_t* n_chunks) {
+    if (!ws || (which != 0 && which != 1)) return fail(DDRS_ERR_INVALID, "bad argument");
+    const int n = ws->trace[which] ? ws->trace_n[which] : 0;
+    if (n_chunks) *n_chunks = n;
+    if (h_rec && n > 0) {
+        if (cap < n) return fail(DDRS_ERR_CAPACITY, "trace buffer too small");
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipMemcpy(h_rec, ws->trace[which], static_cast<size_t>(n) * kTraceWords * 8,
+                         hipMemcpyDeviceToHost));
+    }
+    return DDRS_OK;
 }
 
 ddrs_status ddrs_workspace_set_chunk_width(ddrs_workspace_t ws, int32_t rpl_fwd,

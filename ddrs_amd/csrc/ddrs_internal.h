@@ -21,6 +21,9 @@ constexpr int kNumWidths = 2;
 // Longest chain of links inside a 64-position chunk (its sub-levels): every
 // time step walks it serially, so it bounds the step rate of the chunk.
 constexpr int kMaxSubLevels = 8;
+// Positions per block of the schedule layout (64 or 128): the sub-level cap
+// holds per block, and so for chunks of the block's width or narrower.
+constexpr int kLayoutBlock = 64;
 
 // Level-major schedule, device side. Position p in [0, P); reach-indexed
 // (reference order) arrays have R entries.
@@ -46,7 +49,17 @@ struct Chunks {
     const unsigned char* subr;  // [P] reverse sub-level inside the chunk
     const int* nsub;            // [C]
     const int* nsubr;           // [C]
+    // One representative link per producing chunk (link index relative to the
+    // chunk's first link): polled while a chunk waits, instead of every link.
+    const int* rep_up_ptr;      // [C+1]
+    const short* rep_up;
+    const int* rep_dn_ptr;      // [C+1]
+    const short* rep_dn;
 };
+
+// Representatives a chunk keeps in LDS; a chunk fed by more chunks polls all
+// of its links while it waits.
+constexpr int kRepMax = 64;
 
 // Chunk claims of a persistent launch: the workgroup that increments *ctr to
 // base + k takes the k-th chunk in dependency order.
@@ -55,10 +68,21 @@ struct Claim {
     unsigned long long base;
 };
 
+// Optional per-chunk trace of a persistent launch (null: off): for chunk c,
+// rec[8c..8c+7] = wall clock (100 MHz ticks) when the workgroup claimed c,
+// when its first step was solved, when its last step was solved, the number
+// of polls that found an input missing, and the ticks spent in prepare, in
+// the first poll of each step, waiting for late inputs, and in solve.
+constexpr int kTraceWords = 8;
+struct Trace {
+    unsigned long long* rec;
+};
+
 struct FwdArgs {
     Sched sc;
     Chunks ch;
     Claim claim;
+    Trace trace;
     const float4* st;      // [P][3] reach statics, schedule order
     Phys P;
     int t0;          // out[t0] is the window's initial state
@@ -98,6 +122,7 @@ struct SweepArgs {
     Sched sc;
     Chunks ch;
     Claim claim;
+    Trace trace;
     int s_base, nsteps;
     uint32_t epoch;
     const float4* rec;

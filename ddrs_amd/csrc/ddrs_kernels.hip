@@ -82,26 +82,48 @@ __device__ __forceinline__ int claim_chunk(const Claim& cl, int C, int* lds) {
 }
 
 // For the launch's steps in time order (Dir = +1) or reverse (Dir = -1):
+//   issue(s)           starts the loads of the cross-chunk inputs of step s
+//                      (device-coherent, so each costs a full round trip);
 //   prepare(s, first)  everything of step s that does not depend on other
 //                      chunks (coefficients, right-hand side, into LDS);
 //                      `first` asks for the chunk's step-invariant data too;
-//   ready(s)           polls the cross-chunk inputs of step s into LDS;
+//   check(s)           waits for the issued loads and stages the inputs in
+//                      LDS; true when all of them belong to step s;
+//   probe(s)           while waiting: polls one representative input per
+//                      producing chunk (the one it publishes last); true when
+//                      all of them belong to step s, so the inputs are read again;
 //   solve(s, last)     the level-by-level solve inside the chunk, publishing,
 //                      and the step's epilogue; `last` is the launch's final step.
-// All three are called by the whole wave. Returns early (error flag set) when
+// The inputs are requested before prepare so that their round trip can
+// overlap the coefficient arithmetic. A waiting chunk polls only the
+// representatives, after a growing pause (0.2 to 1.7 us), so that waiting
+// chunks do not flood the memory system. All five are called by the whole wave. Returns early (error flag set) when
 // the inputs of a step do not arrive within the watchdog time or another
 // workgroup has raised the flag.
-template <int Dir, class Prepare, class Ready, class Solve>
-__device__ __forceinline__ void chunk_loop(int nsteps, int* err, Prepare prepare, Ready ready,
-                                           Solve solve) {
+template <int Dir, class Issue, class Prepare, class Check, class Probe, class Solve>
+__device__ __forceinline__ void chunk_loop(int nsteps, int* err, unsigned long long* trace,
+                                           Issue issue, Prepare prepare, Check check,
+                                           Probe probe, Solve solve) {
+    const uint64_t t_claim = wall_ticks();
+    uint64_t t_first = 0, t_prep = 0, t_poll = 0, t_wait = 0, t_solve = 0;
+    unsigned long long missed = 0;
     for (int done = 0; done < nsteps; ++done) {
         const int s = (Dir > 0) ? done : (nsteps - 1 - done);
+        const uint64_t ta = trace ? wall_ticks() : 0;
+        if (done > 0) issue(s);  // the first step stages its links in prepare
         prepare(s, done == 0);
         __syncthreads();
-        if (!ready(s)) {
+        if (done == 0) issue(s);
+        const uint64_t tb = trace ? wall_ticks() : 0;
+        const bool ok = check(s);
+        const uint64_t tc = trace ? wall_ticks() : 0;
+        if (!ok) {
             const uint64_t t0 = wall_ticks();
-            do {
-                __builtin_amdgcn_s_sleep(2);
+            int nap = 1;
+            for (;;) {
+                ++missed;
+                for (int k = 0; k < nap; ++k) __builtin_amdgcn_s_sleep(8);
+                nap = (nap < 8) ? 2 * nap : 8;
                 const bool give_up =
                     (wall_ticks() - t0 > kWatchdogTicks) ||
                     __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
@@ -109,11 +131,42 @@ __device__ __forceinline__ void chunk_loop(int nsteps, int* err, Prepare prepare
                     if (threadIdx.x == 0) atomicExch(err, 1);
                     return;
                 }
-            } while (!ready(s));
+                if (!probe(s)) continue;
+                issue(s);
+                if (check(s)) break;
+            }
         }
+        const uint64_t td = trace ? wall_ticks() : 0;
         solve(s, done == nsteps - 1);
+        if (trace) {
+            const uint64_t te = wall_ticks();
+            if (done == 0) t_first = te;
+            t_prep += tb - ta;
+            t_poll += tc - tb;
+            t_wait += td - tc;
+            t_solve += te - td;
+        }
+    }
+    if (trace && threadIdx.x == 0) {
+        trace[0] = t_claim;
+        trace[1] = t_first;
+        trace[2] = wall_ticks();
+        trace[3] = missed;
+        trace[4] = t_prep;
+        trace[5] = t_poll;
+        trace[6] = t_wait;
+        trace[7] = t_solve;
     }
 }
+
+// Link slots per lane whose polls are issued ahead of prepare; links beyond
+// 64 * kIssueSlots in a chunk are polled when checked.
+constexpr int kIssueSlots = 1;
+
+// (v / ad) for the diagonal ad = 1 + (-c1) * 0 of I - c1 N: exactly v when
+// ad == 1, the division otherwise (c1 infinite or NaN).
+__device__ __forceinline__ float div_diag(float v, float ad) { return (ad == 1.0f) ? v : v / ad; }
+
 
 template <int RPL>
 struct Width {
@@ -122,6 +175,16 @@ struct Width {
     static constexpr int kStageUp = kStageUpBase * RPL;  // staged upstream links
     static constexpr int kStageDn = kStageDnBase * RPL;  // staged downstream links
 };
+
+// A reach's first links, kept per lane so that a sub-level of the solve reads
+// only the linked values: code >= 0 is the chunk-local index of a link inside
+// the chunk, code < 0 is -1 - (its slot among the chunk's staged links).
+constexpr int kFastLinks = 2;    // forward: upstream links
+constexpr int kFastLinksDn = 1;  // adjoint: downstream links (one per reach in a tree)
+template <int RPL>
+__device__ __forceinline__ int link_code(int c, int base, int e, int q) {
+    return ((q >> Width<RPL>::SHIFT) == c) ? (q & (Width<RPL>::CW - 1)) : (-1 - (e - base));
+}
 
 // Stage the links [base, end) of chunk c into LDS: link position, weight, and
 // kind (>= 0: chunk-local index of a link inside the chunk, -1: polled from
@@ -144,7 +207,7 @@ __device__ __forceinline__ void stage_links(int c, int base, int end, const int*
 // forward window
 
 template <int PH, int RPL>
-__global__ __launch_bounds__(64) void k_fwd_window(FwdArgs a) {
+__global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
     using Wd = Width<RPL>;
     constexpr int CW = Wd::CW;
     __shared__ float lds_x[CW];
@@ -154,51 +217,98 @@ __global__ __launch_bounds__(64) void k_fwd_window(FwdArgs a) {
     __shared__ float pb[CW], pnc1[CW], sq[CW], si[CW];
     __shared__ short pe0[CW], pe1[CW], psub[CW];
     __shared__ int porig[CW];
+    __shared__ short srep[kRepMax];
+    __shared__ int lkc[kFastLinks][CW];
+    __shared__ float lka[kFastLinks][CW];
     __shared__ int claimed;
     const int lane = threadIdx.x;
     const Sched& sc = a.sc;
     const int R = sc.R, P = sc.P;
     const int c = claim_chunk<+1>(a.claim, a.ch.C, &claimed);
+    bool unit = true;  // every diagonal of this step's I - c1 N is exactly 1
+    const int rep0 = a.ch.rep_up_ptr[c];
+    const int nrep_all = a.ch.rep_up_ptr[c + 1] - rep0;
+    const int nrep = (nrep_all <= kRepMax) ? nrep_all : -1;
     const int p0 = c * CW;
     const int base = sc.up_ptr[p0];
     const int nlinks = sc.up_ptr[p0 + CW] - base;
     const int nsub = a.ch.nsub[c];
 
+    // Reach statics stay in registers for the whole launch; q' of the next
+    // step is loaded one step ahead.
+    Reach rs[RPL];
+    float qp_next[RPL];
+
     // S1..S25 for window step s: coefficients and right-hand side.
     auto prepare = [&](int s, bool first) {
-        if (first) stage_links<RPL>(c, base, base + nlinks, sc.up_pos, sc.up_adj, spos, sadj, skind);
+        bool ok_diag = true;
+        if (first) {
+            stage_links<RPL>(c, base, base + nlinks, sc.up_pos, sc.up_adj, spos, sadj, skind);
+            if (lane < nrep) srep[lane] = a.ch.rep_up[rep0 + lane];
+        }
         const int t = a.t0 + a.s_base + s;
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
             const int j = r * 64 + lane;
             const int p = p0 + j;
             if (first) porig[j] = sc.perm[p];
-            if (porig[j] >= 0) {
+            const int orig = porig[j];
+            if (orig >= 0) {
                 if (first) {
-                    pe0[j] = static_cast<short>(sc.up_ptr[p] - base);
-                    pe1[j] = static_cast<short>(sc.up_ptr[p + 1] - base);
+                    const int ea = sc.up_ptr[p], eb = sc.up_ptr[p + 1];
+                    pe0[j] = static_cast<short>(ea - base);
+                    pe1[j] = static_cast<short>(eb - base);
                     psub[j] = a.ch.sub[p];
                     sq[j] = a.st_q[p];
                     si[j] = a.st_i[p];
+                    rs[r] = load_reach(a.st, p);
+                    qp_next[r] = a.qprime[static_cast<size_t>(t) * R + orig];
+#pragma unroll
+                    for (int k = 0; k < kFastLinks; ++k) {
+                        if (ea + k < eb) {
+                            lkc[k][j] = link_code<RPL>(c, base, ea + k, sc.up_pos[ea + k]);
+                            lka[k][j] = sc.up_adj[ea + k];
+                        }
+                    }
                 }
                 const float q = sq[j];
                 const float it = si[j];
-                const float qp = fmaxr(a.qprime[static_cast<size_t>(t) * R + porig[j]], a.P.q_lb);
-                const Reach rc = load_reach(a.st, p);
+                const float qp = fmaxr(qp_next[r], a.P.q_lb);
+                if (s + 1 < a.nsteps)
+                    qp_next[r] = a.qprime[static_cast<size_t>(t + 1) * R + orig];
                 Coef co;
-                mc_coef<PH>(a.P, rc, q, co);
+                mc_coef<PH>(a.P, rs[r], q, co);
                 pb[j] = ((co.c2 * it) + (co.c3 * q)) + (co.c4 * qp);             // S25
                 pnc1[j] = -co.c1;
+                ok_diag = ok_diag && (1.0f + (-co.c1) * 0.0f == 1.0f);
             } else if (first) {
                 psub[j] = -1;
             }
         }
+        unit = __all(ok_diag);
     };
 
-    auto ready = [&](int s) -> bool {
+    uint64_t pw[kIssueSlots];
+    auto issue = [&](int s) {
+        const uint64_t* row = a.hist + static_cast<size_t>(a.s_base + s) * P;
+#pragma unroll
+        for (int k = 0; k < kIssueSlots; ++k) {
+            const int e = lane + 64 * k;
+            pw[k] = (e < nlinks && skind[e] < 0) ? poll(row + spos[e]) : 0;
+        }
+    };
+    auto check = [&](int s) -> bool {
         const uint64_t* row = a.hist + static_cast<size_t>(a.s_base + s) * P;
         bool ok = true;
-        for (int e = lane; e < nlinks; e += 64) {
+#pragma unroll
+        for (int k = 0; k < kIssueSlots; ++k) {
+            const int e = lane + 64 * k;
+            if (e < nlinks && skind[e] < 0) {
+                if (!word_ready(pw[k], a.epoch)) ok = false;
+                else sval[e] = word_value(pw[k]);
+            }
+        }
+        for (int e = lane + 64 * kIssueSlots; e < nlinks; e += 64) {
             if (skind[e] < 0) {
                 const uint64_t wv = poll(row + spos[e]);
                 if (!word_ready(wv, a.epoch)) ok = false;
@@ -210,37 +320,64 @@ __global__ __launch_bounds__(64) void k_fwd_window(FwdArgs a) {
         return all;
     };
 
+    auto probe = [&](int s) -> bool {
+        if (nrep <= 0) return true;  // no representatives: read the links again
+        const uint64_t* row = a.hist + static_cast<size_t>(a.s_base + s) * P;
+        bool ok = true;
+        if (lane < nrep) ok = word_ready(poll(row + spos[srep[lane]]), a.epoch);
+        return __all(ok);
+    };
+
     // S26..S28 and the next step's S24 (N Q_{t+1}) from LDS only.
     auto solve = [&](int s, bool last) {
         const int sg = a.s_base + s;
         const int t = a.t0 + sg;
         uint64_t* row = a.hist + static_cast<size_t>(sg) * P;
-        int my[RPL];
-        float x[RPL], inext[RPL];
+        // per-lane operands of the solve, read once per step
+        int my[RPL], e0[RPL], nl[RPL], code[RPL][kFastLinks];
+        float x[RPL], inext[RPL], b[RPL], nc1[RPL], w[RPL][kFastLinks];
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
-            my[r] = psub[r * 64 + lane];
+            const int j = r * 64 + lane;
+            my[r] = psub[j];
             x[r] = 0.0f;
             inext[r] = 0.0f;
+            b[r] = pb[j];
+            nc1[r] = pnc1[j];
+            e0[r] = pe0[j];
+            nl[r] = pe1[j] - e0[r];
+#pragma unroll
+            for (int k = 0; k < kFastLinks; ++k) {
+                code[r][k] = lkc[k][j];
+                w[r][k] = lka[k][j];
+            }
         }
         for (int l = 0; l < nsub; ++l) {
 #pragma unroll
             for (int r = 0; r < RPL; ++r) {
                 if (my[r] == l) {
                     const int j = r * 64 + lane;
-                    const float b = pb[j];
-                    const float nc1 = pnc1[j];
-                    const int e1 = pe1[j];
                     float sum = 0.0f, in = 0.0f;
-                    for (int e = pe0[j]; e < e1; ++e) {
-                        const int kd = skind[e];
-                        const float v = (kd >= 0) ? lds_x[kd] : sval[e];
-                        const float adj = sadj[e];
-                        sum = sum + (nc1 * adj) * v;
-                        in = in + fmaxr(v, a.P.q_lb) * adj;
+                    if (nl[r] <= kFastLinks) {
+#pragma unroll
+                        for (int k = 0; k < kFastLinks; ++k) {
+                            if (k < nl[r]) {
+                                const int cd = code[r][k];
+                                const float v = (cd >= 0) ? lds_x[cd] : sval[-1 - cd];
+                                sum = sum + (nc1[r] * w[r][k]) * v;
+                                in = in + fmaxr(v, a.P.q_lb) * w[r][k];
+                            }
+                        }
+                    } else {
+                        for (int e = e0[r]; e < e0[r] + nl[r]; ++e) {
+                            const int kd = skind[e];
+                            const float v = (kd >= 0) ? lds_x[kd] : sval[e];
+                            const float adj = sadj[e];
+                            sum = sum + (nc1[r] * adj) * v;
+                            in = in + fmaxr(v, a.P.q_lb) * adj;
+                        }
                     }
-                    const float ad = 1.0f + nc1 * 0.0f;
-                    x[r] = (b - sum) / ad;
+                    x[r] = unit ? (b[r] - sum) : div_diag(b[r] - sum, 1.0f + nc1[r] * 0.0f);
                     inext[r] = in;
                     lds_x[j] = x[r];
                     publish(row + p0 + j, x[r], a.epoch);
@@ -265,7 +402,8 @@ __global__ __launch_bounds__(64) void k_fwd_window(FwdArgs a) {
         }
     };
 
-    chunk_loop<+1>(a.nsteps, a.err, prepare, ready, solve);
+    chunk_loop<+1>(a.nsteps, a.err, a.trace.rec ? a.trace.rec + kTraceWords * c : nullptr, issue,
+                   prepare, check, probe, solve);
 }
 
 // ---------------------------------------------------------------------------
@@ -273,9 +411,16 @@ __global__ __launch_bounds__(64) void k_fwd_window(FwdArgs a) {
 //
 // One step's backward (B28..B1) is linear in the transposed-solve value gb once
 // the forward state is fixed. This kernel evaluates, for every reach and step
-// of the segment at once, the coefficients (S1..S23 from the forward history)
-// and the backward chain for gb = 1; the reverse-time sweep then only solves
-// for gb, and the parameter gradients are gb times these records.
+// of the segment, the coefficients (S1..S23 from the forward history) and the
+// backward chain for gb = 1; the reverse-time sweep then only solves for gb,
+// and the parameter gradients are gb times these records. A thread walks one
+// reach through kLinSteps consecutive steps: its statics and links are loaded
+// once, and each history word (own and upstream x of a step) is read once and
+// carried into the next step as out[t] and N out[t]. The backward chain scales
+// gradients with the hardware reciprocal (about 1 ulp); the coefficients and
+// every clamp decision are computed exactly as in the forward.
+
+constexpr int kLinSteps = 32;
 
 template <int PH>
 __global__ __launch_bounds__(256) void k_adj_lin(LinArgs a) {
@@ -285,14 +430,15 @@ __global__ __launch_bounds__(256) void k_adj_lin(LinArgs a) {
     if (p >= P) return;
     const int orig = sc.perm[p];
     if (orig < 0) return;
-    const int sg = a.s_base + static_cast<int>(blockIdx.y);
-    const int t = a.t0 + sg;
+    const int s_first = a.s_base + static_cast<int>(blockIdx.y) * kLinSteps;
+    const int s_end = min(s_first + kLinSteps, a.s_base + a.nsteps);
     const float lb = a.P.q_lb;
     const int e0 = sc.up_ptr[p], e1 = sc.up_ptr[p + 1];
-    const uint64_t* hx = a.hist + static_cast<size_t>(sg) * P;
+    const Reach rc = load_reach(a.st, p);
+    // state entering the first step: out[t] and N out[t]
     float q, it;
-    if (sg > 0) {
-        const uint64_t* hq = hx - P;
+    if (s_first > 0) {
+        const uint64_t* hq = a.hist + static_cast<size_t>(s_first - 1) * P;
         q = fmaxr(word_value(hq[p]), lb);
         float acc = 0.0f;
         for (int e = e0; e < e1; ++e)
@@ -302,20 +448,31 @@ __global__ __launch_bounds__(256) void k_adj_lin(LinArgs a) {
         q = a.ck_q[p];
         it = a.ck_i[p];
     }
-    // B26 for gb = 1: gc1 = -sum_k adj_k * ((-1) * x[col k]) = sum_k x[col k] * adj_k
-    float sx = 0.0f;
-    for (int e = e0; e < e1; ++e) sx = sx + word_value(hx[sc.up_pos[e]]) * sc.up_adj[e];
-    const float qp_raw = a.qprime[static_cast<size_t>(t) * R + orig];
-    const float qp = fmaxr(qp_raw, lb);
-    const Reach rc = load_reach(a.st, p);
-    Coef co;
-    mc_coef<PH>(a.P, rc, q, co);
-    const LocalGrads g = mc_local_bwd<PH>(a.P, co, rc, q, qp, it, 1.0f, sx, 0.0f);
-    const size_t o = static_cast<size_t>(sg) * P + p;
-    const float dq = a.gQ ? a.gQ[static_cast<size_t>(t) * R + orig] : 0.0f;
-    a.rec[o] = make_float4(-co.c1, co.c2, g.gq, dq);
-    a.recm[o] = (!a.mask_lam || q > lb) ? 1 : 0;
-    a.recj[o] = make_float4(g.gn, g.gqsp, g.gp, (!a.mask_qp || qp_raw > lb) ? g.gqp : 0.0f);
+    for (int sg = s_first; sg < s_end; ++sg) {
+        const int t = a.t0 + sg;
+        const uint64_t* hx = a.hist + static_cast<size_t>(sg) * P;
+        // B26 for gb = 1: gc1 = -sum_k adj_k ((-1) x[col k]) = sum_k x[col k] adj_k,
+        // and N out[t+1] for the next step
+        float sx = 0.0f, inext = 0.0f;
+        for (int e = e0; e < e1; ++e) {
+            const float xu = word_value(hx[sc.up_pos[e]]);
+            const float w = sc.up_adj[e];
+            sx = sx + xu * w;
+            inext = inext + fmaxr(xu, lb) * w;
+        }
+        const float qp_raw = a.qprime[static_cast<size_t>(t) * R + orig];
+        const float qp = fmaxr(qp_raw, lb);
+        Coef co;
+        mc_coef<PH>(a.P, rc, q, co);
+        const LocalGrads g = mc_local_bwd<PH, true>(a.P, co, rc, q, qp, it, 1.0f, sx, 0.0f);
+        const size_t o = static_cast<size_t>(sg) * P + p;
+        const float dq = a.gQ ? a.gQ[static_cast<size_t>(t) * R + orig] : 0.0f;
+        a.rec[o] = make_float4(-co.c1, co.c2, g.gq, dq);
+        a.recm[o] = (!a.mask_lam || q > lb) ? 1 : 0;
+        a.recj[o] = make_float4(g.gn, g.gqsp, g.gp, (!a.mask_qp || qp_raw > lb) ? g.gqp : 0.0f);
+        q = fmaxr(word_value(hx[p]), lb);
+        it = inext;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -327,7 +484,7 @@ __global__ __launch_bounds__(256) void k_adj_lin(LinArgs a) {
 // out[t] > lb when it is passed through the clamp of the previous step.
 
 template <int RPL>
-__global__ __launch_bounds__(64) void k_adj_sweep(SweepArgs a) {
+__global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
     using Wd = Width<RPL>;
     constexpr int CW = Wd::CW;
     __shared__ float ldsP[CW], ldsG[CW];
@@ -338,18 +495,34 @@ __global__ __launch_bounds__(64) void k_adj_sweep(SweepArgs a) {
     __shared__ float slam[CW];
     __shared__ unsigned char pm[CW], pvalid[CW];
     __shared__ short pe0[CW], pe1[CW], psub[CW];
+    __shared__ short srep[kRepMax];
+    __shared__ int lkc[kFastLinksDn][CW];
+    __shared__ float lka[kFastLinksDn][CW];
     __shared__ int claimed;
     const int lane = threadIdx.x;
     const Sched& sc = a.sc;
     const int P = sc.P;
     const int c = claim_chunk<-1>(a.claim, a.ch.C, &claimed);
+    bool unit = true;  // every diagonal of this step's I - c1 N is exactly 1
+    const int rep0 = a.ch.rep_dn_ptr[c];
+    const int nrep_all = a.ch.rep_dn_ptr[c + 1] - rep0;
+    const int nrep = (nrep_all <= kRepMax) ? nrep_all : -1;
     const int p0 = c * CW;
     const int base = sc.dn_ptr[p0];
     const int nlinks = sc.dn_ptr[p0 + CW] - base;
     const int nsub = a.ch.nsubr[c];
 
+    // The records of the next (earlier) step are loaded one step ahead, into
+    // named registers (an indexed array here would live in scratch memory).
+    float4 rn0 = make_float4(0.f, 0.f, 0.f, 0.f), rn1 = rn0;
+    int mn0 = 0, mn1 = 0;
+
     auto prepare = [&](int s, bool first) {
-        if (first) stage_links<RPL>(c, base, base + nlinks, sc.dn_pos, sc.dn_adj, spos, sadj, skind);
+        bool ok_diag = true;
+        if (first) {
+            stage_links<RPL>(c, base, base + nlinks, sc.dn_pos, sc.dn_adj, spos, sadj, skind);
+            if (lane < nrep) srep[lane] = a.ch.rep_dn[rep0 + lane];
+        }
         const size_t row = static_cast<size_t>(a.s_base + s) * P;
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
@@ -358,20 +531,40 @@ __global__ __launch_bounds__(64) void k_adj_sweep(SweepArgs a) {
             if (first) pvalid[j] = sc.perm[p] >= 0;
             if (pvalid[j]) {
                 if (first) {
-                    pe0[j] = static_cast<short>(sc.dn_ptr[p] - base);
-                    pe1[j] = static_cast<short>(sc.dn_ptr[p + 1] - base);
+                    const int ea = sc.dn_ptr[p], eb = sc.dn_ptr[p + 1];
+                    pe0[j] = static_cast<short>(ea - base);
+                    pe1[j] = static_cast<short>(eb - base);
                     psub[j] = a.ch.subr[p];
                     slam[j] = a.st_lam[p];
+                    (r == 0 ? rn0 : rn1) = a.rec[row + p];
+                    (r == 0 ? mn0 : mn1) = a.recm[row + p];
+#pragma unroll
+                    for (int k = 0; k < kFastLinksDn; ++k) {
+                        if (ea + k < eb) {
+                            lkc[k][j] = link_code<RPL>(c, base, ea + k, sc.dn_pos[ea + k]);
+                            lka[k][j] = sc.dn_adj[ea + k];
+                        }
+                    }
                 }
-                prec[j] = a.rec[row + p];
-                pm[j] = a.recm[row + p];
+                const float4 rcur = (r == 0) ? rn0 : rn1;
+                prec[j] = rcur;
+                pm[j] = static_cast<unsigned char>((r == 0) ? mn0 : mn1);
+                ok_diag = ok_diag && (1.0f + rcur.x * 0.0f == 1.0f);
+                if (s > 0) {
+                    (r == 0 ? rn0 : rn1) = a.rec[row - P + p];
+                    (r == 0 ? mn0 : mn1) = a.recm[row - P + p];
+                }
             } else if (first) {
                 psub[j] = -1;
             }
         }
+        unit = __all(ok_diag);
     };
 
-    auto ready = [&](int s) -> bool {
+    // The sweep's prepare is short, so its polls are issued when checked
+    // (registers held across prepare would be spilled).
+    auto issue = [&](int) {};
+    auto check = [&](int s) -> bool {
         const uint64_t* row = a.amsg + static_cast<size_t>(a.s_base + s) * P * 2;
         bool ok = true;
         for (int e = lane; e < nlinks; e += 64) {
@@ -392,16 +585,35 @@ __global__ __launch_bounds__(64) void k_adj_sweep(SweepArgs a) {
         return all;
     };
 
+    // the G word of a link is published right after its P word
+    auto probe = [&](int s) -> bool {
+        if (nrep <= 0) return true;  // no representatives: read the links again
+        const uint64_t* row = a.amsg + static_cast<size_t>(a.s_base + s) * P * 2;
+        bool ok = true;
+        if (lane < nrep) ok = word_ready(poll(row + 2 * spos[srep[lane]] + 1), a.epoch);
+        return __all(ok);
+    };
+
     auto solve = [&](int s, bool last) {
         const int sg = a.s_base + s;
         uint64_t* mrow = a.amsg + static_cast<size_t>(sg) * P * 2;
-        int my[RPL];
-        float gbv[RPL], lamv[RPL];
+        // per-lane operands of the solve, read once per step
+        int my[RPL], e0[RPL], nl[RPL], code[RPL][kFastLinksDn];
+        float gbv[RPL], lamv[RPL], gx[RPL], w[RPL][kFastLinksDn];
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
-            my[r] = psub[r * 64 + lane];
+            const int j = r * 64 + lane;
+            my[r] = psub[j];
             gbv[r] = 0.0f;
             lamv[r] = 0.0f;
+            gx[r] = slam[j];
+            e0[r] = pe0[j];
+            nl[r] = pe1[j] - e0[r];
+#pragma unroll
+            for (int k = 0; k < kFastLinksDn; ++k) {
+                code[r][k] = lkc[k][j];
+                w[r][k] = lka[k][j];
+            }
         }
         for (int l = 0; l < nsub; ++l) {
 #pragma unroll
@@ -409,18 +621,30 @@ __global__ __launch_bounds__(64) void k_adj_sweep(SweepArgs a) {
                 if (my[r] == l) {
                     const int j = r * 64 + lane;
                     const float4 rc = prec[j];  // {-c1, c2, dgq/dgb, direct}
-                    const int e1 = pe1[j];
                     float sumP = 0.0f, sumG = 0.0f;
-                    for (int e = pe0[j]; e < e1; ++e) {
-                        const int kd = skind[e];
-                        const float Pd = (kd >= 0) ? ldsP[kd] : stP[e];
-                        const float Gd = (kd >= 0) ? ldsG[kd] : stG[e];
-                        const float adj = sadj[e];
-                        sumP = sumP + adj * Pd;                          // B27
-                        sumG = sumG + Gd * adj;                          // B24 N^T gi
+                    if (nl[r] <= kFastLinksDn) {
+#pragma unroll
+                        for (int k = 0; k < kFastLinksDn; ++k) {
+                            if (k < nl[r]) {
+                                const int cd = code[r][k];
+                                const float Pd = (cd >= 0) ? ldsP[cd] : stP[-1 - cd];
+                                const float Gd = (cd >= 0) ? ldsG[cd] : stG[-1 - cd];
+                                sumP = sumP + w[r][k] * Pd;              // B27
+                                sumG = sumG + Gd * w[r][k];              // B24 N^T gi
+                            }
+                        }
+                    } else {
+                        for (int e = e0[r]; e < e0[r] + nl[r]; ++e) {
+                            const int kd = skind[e];
+                            const float Pd = (kd >= 0) ? ldsP[kd] : stP[e];
+                            const float Gd = (kd >= 0) ? ldsG[kd] : stG[e];
+                            const float adj = sadj[e];
+                            sumP = sumP + adj * Pd;
+                            sumG = sumG + Gd * adj;
+                        }
                     }
-                    const float ad = 1.0f + rc.x * 0.0f;
-                    const float gb = (slam[j] - sumP) / ad;
+                    const float gb = unit ? (gx[r] - sumP)
+                                          : div_diag(gx[r] - sumP, 1.0f + rc.x * 0.0f);
                     const float Pm = rc.x * gb;
                     const float Gm = rc.y * gb;
                     ldsP[j] = Pm;
@@ -444,7 +668,8 @@ __global__ __launch_bounds__(64) void k_adj_sweep(SweepArgs a) {
         }
     };
 
-    chunk_loop<-1>(a.nsteps, a.err, prepare, ready, solve);
+    chunk_loop<-1>(a.nsteps, a.err, a.trace.rec ? a.trace.rec + kTraceWords * c : nullptr, issue,
+                   prepare, check, probe, solve);
 }
 
 // ---------------------------------------------------------------------------
@@ -523,7 +748,9 @@ __global__ __launch_bounds__(64) void k_tri_fwd(TriArgs a) {
         }
         if (valid) a.x[sc.perm[p]] = a.clamp ? fmaxr(x, a.lb) : x;
     };
-    chunk_loop<+1>(1, a.err, prepare, ready, solve);
+    auto issue = [&](int) {};
+    auto probe = [&](int) { return true; };
+    chunk_loop<+1>(1, a.err, nullptr, issue, prepare, ready, probe, solve);
 }
 
 __global__ __launch_bounds__(64) void k_tri_adj(TriArgs a) {
@@ -576,7 +803,9 @@ __global__ __launch_bounds__(64) void k_tri_adj(TriArgs a) {
         }
         if (valid) a.x[sc.perm[p]] = g;
     };
-    chunk_loop<-1>(1, a.err, prepare, ready, solve);
+    auto issue = [&](int) {};
+    auto probe = [&](int) { return true; };
+    chunk_loop<-1>(1, a.err, nullptr, issue, prepare, ready, probe, solve);
 }
 
 // grad_a[k] = (-gb[row k]) * x[col k] over the CSR entries (dispatch.rs:120-136)
@@ -691,7 +920,8 @@ void launch_fwd(const FwdArgs& a, int variant, int rpl, int grid, hipStream_t s)
 static inline dim3 g1(size_t n) { return dim3(static_cast<unsigned>((n + 255) / 256)); }
 
 void launch_lin(const LinArgs& a, int variant, hipStream_t s) {
-    const dim3 grid(static_cast<unsigned>((a.sc.P + 255) / 256), static_cast<unsigned>(a.nsteps));
+    const dim3 grid(static_cast<unsigned>((a.sc.P + 255) / 256),
+                    static_cast<unsigned>((a.nsteps + kLinSteps - 1) / kLinSteps));
     switch (variant) {
         case PH_DDR: hipLaunchKernelGGL(k_adj_lin<PH_DDR>, grid, dim3(256), 0, s, a); break;
         case PH_POS: hipLaunchKernelGGL(k_adj_lin<PH_POS>, grid, dim3(256), 0, s, a); break;

@@ -25,6 +25,13 @@
 #define DDRS_HD
 #endif
 
+// pow_f32 / log_f32 / pow_ln_f32 are out of line (a call makes the caller
+// wait for its outstanding memory loads, but inlined at every call site the
+// powers raise register use); pow_f32_inl / pow_ln_f32_inl are the same
+// evaluations for callers with registers to spare, where the polynomial
+// constants stay in registers across calls.
+#define DDRS_MATH_LINKAGE __attribute__((noinline))
+
 namespace ddrs {
 
 // ln(x) for finite x > 0 (double), ~2^-50 relative (absolute near x = 1)
@@ -107,14 +114,15 @@ DDRS_HD inline float pow_from_ln(double l, double yd) {
 }
 
 // x^y for fp32 x, y, rounded once to fp32.
-DDRS_HD inline __attribute__((noinline)) float pow_f32(float x, float y) {
+DDRS_HD inline __attribute__((always_inline)) float pow_f32_inl(float x, float y) {
     const double xd = static_cast<double>(x), yd = static_cast<double>(y);
     if (pow_fast_path(xd, yd)) return pow_from_ln(ln_reduced(xd), yd);
     return pow_special(xd, yd);
 }
+DDRS_HD inline DDRS_MATH_LINKAGE float pow_f32(float x, float y) { return pow_f32_inl(x, y); }
 
 // ln x for fp32 x, rounded once to fp32
-DDRS_HD inline __attribute__((noinline)) float log_f32(float x) {
+DDRS_HD inline DDRS_MATH_LINKAGE float log_f32(float x) {
     const double xd = static_cast<double>(x);
     if (xd > 0.0 && xd < INFINITY) return static_cast<float>(ln_reduced(xd));
     return log_special(xd);
@@ -125,7 +133,7 @@ DDRS_HD inline __attribute__((noinline)) float log_f32(float x) {
 struct PowLn {
     float pw, ln;
 };
-DDRS_HD inline __attribute__((noinline)) PowLn pow_ln_f32(float x, float y) {
+DDRS_HD inline __attribute__((always_inline)) PowLn pow_ln_f32_inl(float x, float y) {
     const double xd = static_cast<double>(x), yd = static_cast<double>(y);
     PowLn r;
     if (xd > 0.0 && xd < INFINITY) {
@@ -138,5 +146,6 @@ DDRS_HD inline __attribute__((noinline)) PowLn pow_ln_f32(float x, float y) {
     }
     return r;
 }
+DDRS_HD inline DDRS_MATH_LINKAGE PowLn pow_ln_f32(float x, float y) { return pow_ln_f32_inl(x, y); }
 
 }  // namespace ddrs

@@ -78,14 +78,15 @@ struct Coef {
 
 // S1..S23 for out[t] = q. ln_ratio and ln_depth are log_f32 of ratio and depth
 // (needed by B6 and B7), taken from the same ln evaluation as the powers.
-template <int PH>
+// INL evaluates the powers inline (same bits).
+template <int PH, bool INL = false>
 __device__ __forceinline__ void mc_coef(const Phys& P, const Reach& r, float q, Coef& o) {
     float numerator = (q * r.n) * r.qe1;                                // S2
     o.ratio = numerator / r.denominator;                                // S3, S4
-    const PowLn d = pow_ln_f32(o.ratio, r.exponent);                    // S5
+    const PowLn d = INL ? pow_ln_f32_inl(o.ratio, r.exponent) : pow_ln_f32(o.ratio, r.exponent);  // S5
     o.ln_ratio = d.ln;
     o.depth = fmaxr(d.pw, P.depth_lb);                                  // S6
-    const PowLn w = pow_ln_f32(o.depth, r.q_eps);
+    const PowLn w = INL ? pow_ln_f32_inl(o.depth, r.q_eps) : pow_ln_f32(o.depth, r.q_eps);
     o.ln_depth = w.ln;
     o.tw = r.p * w.pw;                                                  // S7
     o.ss_raw = (o.tw * r.q_eps) / (o.depth * 2.0f);                     // S8
@@ -96,7 +97,8 @@ __device__ __forceinline__ void mc_coef(const Phys& P, const Reach& r, float q, 
     const float root = sqrtf(o.ss * o.ss + 1.0f);
     const float wp = o.bw + (o.depth * root) * 2.0f;                    // S13
     o.hyd_r = area / wp;                                                // S14
-    o.v_un = (r.inv_n * rpow(o.hyd_r, 2.0f / 3.0f)) * r.sqrt_s;         // S15
+    const float hr_pow = INL ? pow_f32_inl(o.hyd_r, 2.0f / 3.0f) : rpow(o.hyd_r, 2.0f / 3.0f);
+    o.v_un = (r.inv_n * hr_pow) * r.sqrt_s;                             // S15
     o.v_cl = fclamp(o.v_un, P.v_lb, 15.0f);                             // S16
     if (PH == PH_DDR) {                                                 // S17
         o.cel = o.v_cl * (5.0f / 3.0f);
@@ -144,7 +146,18 @@ struct LocalGrads {
 // clamp and branch masks depend on the forward state only, and gq24 enters gq
 // alone. Since gc1 = gb * sum_k adj_k x[col k], the step's whole backward is
 // gb times the call with gb = 1, gc1 = that sum, gq24 = 0, plus gq24 in gq.
-template <int PH>
+// a / b: correctly rounded, or (F) as a times the hardware reciprocal (about
+// 1 ulp), for values that only scale gradients.
+template <bool F>
+__device__ __forceinline__ float dv(float a, float b) {
+    if (F) return a * __builtin_amdgcn_rcpf(b);
+    return a / b;
+}
+
+// F = true evaluates the divisions that scale gradient values with the
+// hardware reciprocal; the ones that decide clamp and branch masks stay
+// correctly rounded, so the masks are those of the forward.
+template <int PH, bool F = false>
 __device__ __forceinline__ LocalGrads mc_local_bwd(const Phys& P, const Coef& c, const Reach& r,
                                                    float q, float qp, float i_t, float gb,
                                                    float gc1, float gq24) {
@@ -166,13 +179,13 @@ __device__ __forceinline__ LocalGrads mc_local_bwd(const Phys& P, const Coef& c,
     float num_c1 = (-two_kx) + dt;
     float num_c2 = two_kx + dt;
     float num_c3 = two_k_1mx - dt;
-    float gd1 = ((-gc1) * num_c1) / denom_sq;
-    float gd2 = ((-gc2) * num_c2) / denom_sq;
-    float gd3 = ((-gc3) * num_c3) / denom_sq;
-    float gd4 = ((-gc4) * (2.0f * dt)) / denom_sq;
+    float gd1 = dv<F>((-gc1) * num_c1, denom_sq);
+    float gd2 = dv<F>((-gc2) * num_c2, denom_sq);
+    float gd3 = dv<F>((-gc3) * num_c3, denom_sq);
+    float gd4 = dv<F>((-gc4) * (2.0f * dt), denom_sq);
     float gdenom_total = ((gd1 + gd2) + gd3) + gd4;
-    float g_2kx_total = ((-gc1) / c.denom) + (gc2 / c.denom);
-    float g_2k1mx_total = (gc3 / c.denom) + gdenom_total;
+    float g_2kx_total = dv<F>(-gc1, c.denom) + dv<F>(gc2, c.denom);
+    float g_2k1mx_total = dv<F>(gc3, c.denom) + gdenom_total;
     // B19 / B19'
     float xg_q = 0.0f, xg_tw = 0.0f, xg_c = 0.0f, gk_cap = 0.0f;
     if (PH != PH_DDR) {
@@ -193,15 +206,15 @@ __device__ __forceinline__ LocalGrads mc_local_bwd(const Phys& P, const Coef& c,
             float g_hi_b = m_b ? gxx : 0.0f;
             float gx_cunge = m_cunge ? gxx : 0.0f;
             float gcr = (g_hi_a - g_hi_b) * (0.5f * (1.0f - kDelta));
-            gk_cap = ((-gcr) * dt) / (c.k_musk * c.k_musk);
+            gk_cap = dv<F>((-gcr) * dt, c.k_musk * c.k_musk);
             gxx = gx_cunge;
             x_for_clamp = x_cunge;
         }
         bool unsat = (x_for_clamp > 0.0f) && (x_for_clamp < 0.5f);
         gxx = unsat ? gxx : 0.0f;
-        xg_q = ((-gxx) * 0.5f) / bscl;
-        xg_tw = (gxx * half_w) / c.tw;
-        xg_c = (gxx * half_w) / c.cel;
+        xg_q = dv<F>((-gxx) * 0.5f, bscl);
+        xg_tw = dv<F>(gxx * half_w, c.tw);
+        xg_c = dv<F>(gxx * half_w, c.cel);
     }
     float g_2k_total = (g_2kx_total * c.x_eff) + (g_2k1mx_total * one_minus_x);
     float gk = g_2k_total * 2.0f;
@@ -219,35 +232,35 @@ __device__ __forceinline__ LocalGrads mc_local_bwd(const Phys& P, const Coef& c,
         gk_raw = (k_raw > k_floor) ? gk : 0.0f;
     }
     // B18
-    float gcel = ((-gk_raw) * length) / (c.cel * c.cel);
+    float gcel = dv<F>((-gk_raw) * length, c.cel * c.cel);
     if (PH != PH_DDR) gcel = gcel + xg_c;
     // B17
     float gvcl, bg_area = 0.0f, bg_tw = 0.0f, bg_wp = 0.0f, bg_ss = 0.0f;
     if (PH == PH_DDR) {
         gvcl = gcel * (5.0f / 3.0f);
     } else {
-        float g_term = ((area * sq1) / (c.tw * wp)) * (4.0f / 3.0f);
+        float g_term = dv<F>(area * sq1, c.tw * wp) * (4.0f / 3.0f);
         float beta = (-g_term) + (5.0f / 3.0f);
         float gbeta = gcel * c.v_cl;
-        bg_area = ((-gbeta) * g_term) / area;
-        bg_tw = (gbeta * g_term) / c.tw;
-        bg_wp = (gbeta * g_term) / wp;
-        bg_ss = (((-gbeta) * g_term) * c.ss) / one_plus_ss_sq;
+        bg_area = dv<F>((-gbeta) * g_term, area);
+        bg_tw = dv<F>(gbeta * g_term, c.tw);
+        bg_wp = dv<F>(gbeta * g_term, wp);
+        bg_ss = dv<F>(((-gbeta) * g_term) * c.ss, one_plus_ss_sq);
         gvcl = gcel * beta;
     }
     // B16
     float gvun = ((c.v_un > P.v_lb) && (c.v_un < 15.0f)) ? gvcl : 0.0f;
     // B15
-    float gn15 = gvun * ((-c.v_un) / nman);
-    float gr = (gvun * (c.v_un * (2.0f / 3.0f))) / c.hyd_r;
+    float gn15 = gvun * dv<F>(-c.v_un, nman);
+    float gr = dv<F>(gvun * (c.v_un * (2.0f / 3.0f)), c.hyd_r);
     // B14
-    float garea_r = gr / wp;
-    float gwp_r = ((-gr) * area) / (wp * wp);
+    float garea_r = dv<F>(gr, wp);
+    float gwp_r = dv<F>((-gr) * area, wp * wp);
     // B13
     float gwp = (PH == PH_DDR) ? gwp_r : (gwp_r + bg_wp);
     float gbw13 = gwp;
     float gd13 = (gwp * sq1) * 2.0f;
-    float gss13 = (((gwp * c.depth) * 2.0f) * c.ss) / sq1;
+    float gss13 = dv<F>(((gwp * c.depth) * 2.0f) * c.ss, sq1);
     // B12
     float garea = (PH == PH_DDR) ? garea_r : (garea_r + bg_area);
     float half_d = c.depth * 0.5f;
@@ -266,26 +279,26 @@ __device__ __forceinline__ LocalGrads mc_local_bwd(const Phys& P, const Coef& c,
     float gss_clamp = ((c.ss_raw > 0.5f) && (c.ss_raw < 50.0f)) ? gss_comb : 0.0f;
     // B8
     float two_d = c.depth * 2.0f;
-    float gtw8 = (gss_clamp * r.q_eps) / two_d;
-    float gqeps8 = (gss_clamp * c.tw) / two_d;
-    float gd8 = (((-gss_clamp) * c.tw) * r.q_eps) / (two_d * c.depth);
+    float gtw8 = dv<F>(gss_clamp * r.q_eps, two_d);
+    float gqeps8 = dv<F>(gss_clamp * c.tw, two_d);
+    float gd8 = dv<F>(((-gss_clamp) * c.tw) * r.q_eps, two_d * c.depth);
     float gtw_total = (gtw12 + gtw10) + gtw8;
     if (PH != PH_DDR) gtw_total = gtw_total + bg_tw;
     if (PH != PH_DDR) gtw_total = gtw_total + xg_tw;
     // B7
-    float gp7 = (gtw_total * c.tw) / psp;
-    float gdepth7 = ((gtw_total * c.tw) * r.q_eps) / c.depth;
+    float gp7 = dv<F>(gtw_total * c.tw, psp);
+    float gdepth7 = dv<F>((gtw_total * c.tw) * r.q_eps, c.depth);
     float gqeps7 = (gtw_total * c.tw) * c.ln_depth;
     // B6
     float gd_total = (((gd13 + gd12) + gd10) + gd8) + gdepth7;
     float gd_pre = (c.depth > P.depth_lb) ? gd_total : 0.0f;
-    float gratio6 = ((gd_pre * r.exponent) * c.depth) / c.ratio;
+    float gratio6 = dv<F>((gd_pre * r.exponent) * c.depth, c.ratio);
     float gexp6 = (gd_pre * c.depth) * c.ln_ratio;
     // B5
-    float gqeps5 = ((-gexp6) * 9.0f) / r.f53sq;
+    float gqeps5 = dv<F>((-gexp6) * 9.0f, r.f53sq);
     // B4
-    float gnum = gratio6 / r.denominator;
-    float gden = ((-gratio6) * c.ratio) / r.denominator;
+    float gnum = dv<F>(gratio6, r.denominator);
+    float gden = dv<F>((-gratio6) * c.ratio, r.denominator);
     // B3
     float gp3 = gden * sqrt_s;
     // B2

@@ -69,11 +69,15 @@ class Network:
     """A routing network on the GPU: the bit-exact CsrPattern on the host plus
     the level-major device schedule (ddrs_plan_create)."""
 
-    def __init__(self, adj: SparseAdjacency):
+    def __init__(self, adj: SparseAdjacency, max_sub_levels: int = 0, block: int = 0):
+        """max_sub_levels: longest chain of links one block of the device
+        schedule may hold; block: positions per block, 64 or 128 (0: library
+        defaults). Results do not depend on either."""
         self.adj = adj
         h = ctypes.c_void_p()
-        N.check(N.lib.ddrs_plan_create(adj.n, adj.nnz, _ptr(adj.rows), _ptr(adj.cols),
-                                       _ptr(adj.values), ctypes.byref(h)), "ddrs_plan_create")
+        N.check(N.lib.ddrs_plan_create_ex(adj.n, adj.nnz, _ptr(adj.rows), _ptr(adj.cols),
+                                          _ptr(adj.values), int(max_sub_levels), int(block),
+                                          ctypes.byref(h)), "ddrs_plan_create_ex")
         self.handle = h
         info = [ctypes.c_int32() for _ in range(4)]
         N.check(N.lib.ddrs_plan_info(h, *[ctypes.byref(x) for x in info]))

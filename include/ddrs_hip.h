@@ -74,6 +74,14 @@ const char* ddrs_version(void);
 ddrs_status ddrs_plan_create(int32_t n, int32_t nnz_off, const int32_t* h_rows,
                              const int32_t* h_cols, const float* h_vals,
                              ddrs_plan_t* out);
+/* As ddrs_plan_create, with the layout of the device schedule: blocks of
+ * `block` positions (64 or 128; 0: 64) holding at most `max_sub_levels`
+ * sub-levels (longest chain of links inside the block; 0: 8). Smaller caps
+ * spread thin chains over more chunks, which pipeline over time steps.
+ * Results do not depend on either. No reference counterpart. */
+ddrs_status ddrs_plan_create_ex(int32_t n, int32_t nnz_off, const int32_t* h_rows,
+                                const int32_t* h_cols, const float* h_vals,
+                                int32_t max_sub_levels, int32_t block, ddrs_plan_t* out);
 void ddrs_plan_destroy(ddrs_plan_t plan);
 /* n, nnz (= nnz_off + n), number of 64-reach chunks, number of levels. */
 ddrs_status ddrs_plan_info(ddrs_plan_t plan, int32_t* n, int32_t* nnz, int32_t* n_chunks,
@@ -206,6 +214,16 @@ ddrs_status ddrs_workspace_layout(ddrs_workspace_t ws, int32_t* rpl_fwd, int32_t
                                   int32_t* rpl_sweep, int32_t* grid_sweep);
 ddrs_status ddrs_workspace_set_chunk_width(ddrs_workspace_t ws, int32_t rpl_fwd,
                                            int32_t rpl_sweep);
+/* Per-chunk trace of the persistent forward (which = 0) and adjoint-sweep
+ * (which = 1) launches: for each chunk, 8 values: GPU wall clock (100 MHz
+ * ticks) when it was claimed, when its first and last steps were solved; the
+ * number of polls that found an input missing; and ticks spent in prepare, in
+ * the first poll of each step, waiting for late inputs, and in solve. h_rec
+ * holds cap chunk records (8 each); n_chunks gets the chunk count of the last
+ * launch. No reference counterpart. */
+ddrs_status ddrs_set_trace(ddrs_workspace_t ws, int32_t enable);
+ddrs_status ddrs_get_trace(ddrs_workspace_t ws, int32_t which, uint64_t* h_rec, int32_t cap,
+                           int32_t* n_chunks);
 
 /* ---------------------------------------------------------------------------
  * Index construction (host, bit-exact): the reference's topological order
