@@ -137,7 +137,11 @@ __device__ __forceinline__ void chunk_loop(int nsteps, int* err, unsigned long l
             }
         }
         const uint64_t td = trace ? wall_ticks() : 0;
+        // The solve is a chain of dependent LDS round trips; it issues ahead of
+        // the other waves on the SIMD, whose arithmetic fills the gaps.
+        __builtin_amdgcn_s_setprio(3);
         solve(s, done == nsteps - 1);
+        __builtin_amdgcn_s_setprio(0);
         if (trace) {
             const uint64_t te = wall_ticks();
             if (done == 0) t_first = te;
@@ -277,7 +281,8 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
                 if (s + 1 < a.nsteps)
                     qp_next[r] = a.qprime[static_cast<size_t>(t + 1) * R + orig];
                 Coef co;
-                mc_coef<PH>(a.P, rs[r], q, co);
+                PowEval pe;
+                mc_coef<PH>(a.P, rs[r], q, co, pe);
                 pb[j] = ((co.c2 * it) + (co.c3 * q)) + (co.c4 * qp);             // S25
                 pnc1[j] = -co.c1;
                 ok_diag = ok_diag && (1.0f + (-co.c1) * 0.0f == 1.0f);
@@ -380,11 +385,16 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
                     x[r] = unit ? (b[r] - sum) : div_diag(b[r] - sum, 1.0f + nc1[r] * 0.0f);
                     inext[r] = in;
                     lds_x[j] = x[r];
-                    publish(row + p0 + j, x[r], a.epoch);
                 }
             }
             __syncthreads();
         }
+        // Publish after the solve: other chunks read a step's values only once
+        // its solve is done, and a fixed number of stores per step lets the
+        // next step's loads wait for their own data alone.
+#pragma unroll
+        for (int r = 0; r < RPL; ++r)
+            if (my[r] >= 0) publish(row + p0 + r * 64 + lane, x[r], a.epoch);
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
             if (my[r] >= 0) {
@@ -463,7 +473,8 @@ __global__ __launch_bounds__(256) void k_adj_lin(LinArgs a) {
         const float qp_raw = a.qprime[static_cast<size_t>(t) * R + orig];
         const float qp = fmaxr(qp_raw, lb);
         Coef co;
-        mc_coef<PH>(a.P, rc, q, co);
+        PowEval pe;
+        mc_coef<PH>(a.P, rc, q, co, pe);
         const LocalGrads g = mc_local_bwd<PH, true>(a.P, co, rc, q, qp, it, 1.0f, sx, 0.0f);
         const size_t o = static_cast<size_t>(sg) * P + p;
         const float dq = a.gQ ? a.gQ[static_cast<size_t>(t) * R + orig] : 0.0f;
@@ -649,13 +660,20 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
                     const float Gm = rc.y * gb;
                     ldsP[j] = Pm;
                     ldsG[j] = Gm;
-                    publish(mrow + 2 * (p0 + j), Pm, a.epoch);
-                    publish(mrow + 2 * (p0 + j) + 1, Gm, a.epoch);
                     gbv[r] = gb;
                     lamv[r] = pm[j] ? rc.w + (gb * rc.z + sumG) : 0.0f;
                 }
             }
             __syncthreads();
+        }
+        // published after the solve, as in the forward
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+            if (my[r] >= 0) {
+                const int j = r * 64 + lane;
+                publish(mrow + 2 * (p0 + j), ldsP[j], a.epoch);
+                publish(mrow + 2 * (p0 + j) + 1, ldsG[j], a.epoch);
+            }
         }
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {

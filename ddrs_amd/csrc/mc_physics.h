@@ -76,17 +76,45 @@ struct Coef {
     float v_un, v_cl, cel, k_musk, x_eff, denom, c1, c2, c3, c4;
 };
 
+// The powers of one reach-step: S5 ratio^exponent and S7 depth^q_eps with the
+// logarithms of their bases (B6, B7 use them), and S15 hyd_r^(2/3).
+struct PowVals {
+    float pw5, ln5, pw7, ln7, pw15;
+};
+
+// The power source of mc_coef: evaluates the three powers (out of line) and
+// keeps their values.
+struct PowEval {
+    PowVals v;
+    __device__ __forceinline__ PowLn p5(float x, float y) {
+        const PowLn r = pow_ln_f32(x, y);
+        v.pw5 = r.pw;
+        v.ln5 = r.ln;
+        return r;
+    }
+    __device__ __forceinline__ PowLn p7(float x, float y) {
+        const PowLn r = pow_ln_f32(x, y);
+        v.pw7 = r.pw;
+        v.ln7 = r.ln;
+        return r;
+    }
+    __device__ __forceinline__ float p15(float x) {
+        v.pw15 = pow_f32(x, 2.0f / 3.0f);
+        return v.pw15;
+    }
+};
+
 // S1..S23 for out[t] = q. ln_ratio and ln_depth are log_f32 of ratio and depth
 // (needed by B6 and B7), taken from the same ln evaluation as the powers.
-// INL evaluates the powers inline (same bits).
-template <int PH, bool INL = false>
-__device__ __forceinline__ void mc_coef(const Phys& P, const Reach& r, float q, Coef& o) {
+template <int PH, class Pow>
+__device__ __forceinline__ void mc_coef(const Phys& P, const Reach& r, float q, Coef& o,
+                                        Pow& pw) {
     float numerator = (q * r.n) * r.qe1;                                // S2
     o.ratio = numerator / r.denominator;                                // S3, S4
-    const PowLn d = INL ? pow_ln_f32_inl(o.ratio, r.exponent) : pow_ln_f32(o.ratio, r.exponent);  // S5
+    const PowLn d = pw.p5(o.ratio, r.exponent);                         // S5
     o.ln_ratio = d.ln;
     o.depth = fmaxr(d.pw, P.depth_lb);                                  // S6
-    const PowLn w = INL ? pow_ln_f32_inl(o.depth, r.q_eps) : pow_ln_f32(o.depth, r.q_eps);
+    const PowLn w = pw.p7(o.depth, r.q_eps);
     o.ln_depth = w.ln;
     o.tw = r.p * w.pw;                                                  // S7
     o.ss_raw = (o.tw * r.q_eps) / (o.depth * 2.0f);                     // S8
@@ -97,8 +125,7 @@ __device__ __forceinline__ void mc_coef(const Phys& P, const Reach& r, float q, 
     const float root = sqrtf(o.ss * o.ss + 1.0f);
     const float wp = o.bw + (o.depth * root) * 2.0f;                    // S13
     o.hyd_r = area / wp;                                                // S14
-    const float hr_pow = INL ? pow_f32_inl(o.hyd_r, 2.0f / 3.0f) : rpow(o.hyd_r, 2.0f / 3.0f);
-    o.v_un = (r.inv_n * hr_pow) * r.sqrt_s;                             // S15
+    o.v_un = (r.inv_n * pw.p15(o.hyd_r)) * r.sqrt_s;                    // S15
     o.v_cl = fclamp(o.v_un, P.v_lb, 15.0f);                             // S16
     if (PH == PH_DDR) {                                                 // S17
         o.cel = o.v_cl * (5.0f / 3.0f);

@@ -2,15 +2,13 @@
 fixtures and FP64 finite differences. Needs an MI355X.
 
 Tolerances (stated per test):
-* one timestep: discharge within 1e-4 max relative error of the f32 oracle;
-* multi-step windows: at least 99.9% of reach-steps within 1e-4 relative of the
-  f32 oracle, and everywhere an error against the FP64 oracle no larger than
-  3x the f32 oracle's own error against FP64. A literal 1e-4 bound on every
-  reach-step is met by no f32 implementation that is not bit-identical: with
-  Cunge X most reaches have c1 < 0, the recurrence amplifies rounding, and the
-  reference's own f32 path is 3.7e-4 from FP64 on the C1 case. gfx950's powf
-  differs from glibc's in the last bit on a small fraction of inputs, so
-  bit-identity is not available;
+* discharge, one timestep and every reach-step of multi-step windows: within
+  1e-4 max relative error of the f32 oracle, and an error against the FP64
+  oracle no larger than 3x the f32 oracle's own error against FP64. The
+  recurrence amplifies rounding (with Cunge X most reaches have c1 < 0; the
+  reference's own f32 path is 3.7e-4 from FP64 on the C1 case), so the bar
+  holds only because the device powers and logarithms are correctly rounded
+  (as glibc's are on all but ~0.06% of inputs) and S13 squares exactly;
 * gradients: per gradient vector, max|g - g_ref| <= 1e-3 max|g_ref| against the
   f32 oracle, and an error against the FP64 oracle no larger than 3x the f32
   oracle's own (+1e-6 max|g|). Element-wise relative error is not a usable bar:
@@ -95,7 +93,7 @@ def check_discharge(label, got, ref32, ref64):
     frac = float((r32 <= 1e-4).mean())
     print(f"{label}: max rel err vs f32 oracle {r32.max():.3e} (fraction within 1e-4: "
           f"{frac:.6f}); vs f64 {e_gpu64:.3e}; f32 oracle vs f64 {e_ref64:.3e}")
-    assert frac >= 0.999, label
+    assert r32.max() <= 1e-4, label
     assert e_gpu64 <= 3.0 * e_ref64 + 1e-6, label
 
 
