@@ -88,6 +88,10 @@ struct ddrs_workspace {
     uint64_t *hist = nullptr, *amsg = nullptr, *slot = nullptr;
     unsigned long long* claim_ctr = nullptr;  // chunk claims of persistent launches
     unsigned long long claims = 0;            // claims made so far
+    float* qps = nullptr;                    // [max_steps][P] q' in schedule order
+    float* gqs = nullptr;                    // [max_steps][P] dL/dQ in schedule order
+    const float* qps_src = nullptr;          // q' it was permuted from (rows qps_rows)
+    int qps_rows = 0;
     float4 *rec = nullptr, *recj = nullptr;  // [window][P]
     unsigned char* recm = nullptr;           // [window][P]
     float* gbuf = nullptr;                   // [window][P]
@@ -581,6 +585,8 @@ ddrs_status ddrs_workspace_create(ddrs_plan_t plan, int32_t window, int32_t max_
     A(&ws->rec, WR * sizeof(float4));
     A(&ws->recj, WR * sizeof(float4));
     A(&ws->recm, WR);
+    A(&ws->qps, static_cast<size_t>(std::max(max_steps, 1)) * R * 4);
+    A(&ws->gqs, static_cast<size_t>(std::max(max_steps, 1)) * R * 4);
     A(&ws->gbuf, WR * 4);
     A(&ws->slot, R * 8);
     A(&ws->claim_ctr, 8);
@@ -685,10 +691,19 @@ static FwdArgs fwd_args(ddrs_workspace* ws, const ddrs_params* p) {
     a.st = ws->st;
     a.P = to_phys(p);
     a.hist = ws->hist;
+    a.qps = ws->qps;
     a.st_q = ws->st_q;
     a.st_i = ws->st_i;
     a.err = ws->err;
     return a;
+}
+
+// q' rows [0, rows) of `q` in schedule order, unless already there.
+static void permute_qprime(ddrs_workspace* ws, const float* q, int rows, hipStream_t s) {
+    if (q == ws->qps_src && rows <= ws->qps_rows) return;
+    launch_permute_rows(ws->plan->sc, q, 0, rows, ws->qps, s);
+    ws->qps_src = q;
+    ws->qps_rows = rows;
 }
 
 // Claims of the next persistent launch of `grid` workgroups.
@@ -710,7 +725,6 @@ static void launch_fwd_ws(ddrs_workspace* ws, FwdArgs a, const ddrs_params* p, h
 // segment starting at t0, in one launch each (plain form: per step).
 struct AdjSegment {
     int t0 = 0, len = 0;
-    const float* qprime = nullptr;
     const float* gQ = nullptr;
     float* gqp_out = nullptr;
     int mask_qp = 1, mask_lam = 1;
@@ -726,8 +740,7 @@ static void run_adj_segment(ddrs_workspace* ws, const ddrs_params* p, const AdjS
     L.st = ws->st;
     L.P = to_phys(p);
     L.t0 = g.t0;
-    L.qprime = g.qprime;
-    L.gQ = g.gQ;
+    L.qps = ws->qps;
     L.mask_qp = g.mask_qp;
     L.mask_lam = g.mask_lam;
     L.hist = ws->hist;
@@ -739,6 +752,8 @@ static void run_adj_segment(ddrs_workspace* ws, const ddrs_params* p, const AdjS
     SweepArgs W{};
     W.sc = sc;
     W.ch = ws->plan->ch[ws->rpl_sweep == 2 ? 1 : 0].dev;
+    W.t0 = g.t0;
+    W.gqs = g.gQ ? ws->gqs : nullptr;
     W.rec = ws->rec;
     W.recm = ws->recm;
     W.amsg = ws->amsg;
@@ -791,12 +806,13 @@ ddrs_status ddrs_mc_step_fwd(ddrs_workspace_t ws, const ddrs_params* params, con
     if (!d_q_t || !d_q_prime_t || !d_q_next) return fail(DDRS_ERR_INVALID, "NULL input/output");
     hipStream_t s = S(stream);
     launch_state(ws->plan->sc, d_q_t, 0, params->discharge_lb, ws->st_q, ws->st_i, nullptr, s);
+    launch_permute_rows(ws->plan->sc, d_q_prime_t, 0, 1, ws->qps, s);
+    ws->qps_src = nullptr;
     FwdArgs a = fwd_args(ws, params);
     a.t0 = 0;
     a.s_base = 0;
     a.nsteps = 1;
     a.epoch = ++ws->epoch;
-    a.qprime = d_q_prime_t;
     a.X_out = d_x_sol;
     a.Q_out = nullptr;
     launch_fwd_ws(ws, a, params, s);
@@ -822,12 +838,13 @@ ddrs_status ddrs_mc_step_bwd(ddrs_workspace_t ws, const ddrs_params* params, con
     const Sched& sc = ws->plan->sc;
     launch_state(sc, d_q_t, 0, params->discharge_lb, ws->st_q, ws->st_i, nullptr, s);
     launch_hist_from_x(sc, d_x_sol, ws->hist, s);
+    launch_permute_rows(sc, d_q_prime_t, 0, 1, ws->qps, s);
+    ws->qps_src = nullptr;
     launch_lam_init(sc, d_grad_q_next, ws->hist, params->discharge_lb, ws->st_lam, ws->acc_n,
                     ws->acc_qsp, ws->acc_p, s);
     AdjSegment g;
     g.t0 = 0;
     g.len = 1;
-    g.qprime = d_q_prime_t;
     g.gqp_out = d_grad_q_prime_t;  // row 0: c4 * gb without the q' clamp mask
     g.mask_qp = 0;
     g.mask_lam = 0;                // grad_q_t is the gradient of the raw q_t input
@@ -864,7 +881,6 @@ ddrs_status run_fwd_segment(ddrs_workspace* ws, const ddrs_params* params, int T
     const int len = seg_len(ws, T, seg);
     FwdArgs a = fwd_args(ws, params);
     a.t0 = t0;
-    a.qprime = qprime;
     a.Q_out = Q_out;
     if (form == DDRS_FORM_PERSISTENT) {
         a.s_base = 0;
@@ -914,6 +930,8 @@ ddrs_status ddrs_mc_window_fwd(ddrs_workspace_t ws, const ddrs_params* params, i
     launch_state(ws->plan->sc, d_q0, 1, params->discharge_lb, ws->st_q, ws->st_i, d_Q_out, s);
     const int steps = T - 1;
     const int nseg = (steps + ws->window - 1) / ws->window;
+    ws->qps_src = nullptr;  // the caller may have rewritten q' in place
+    permute_qprime(ws, d_q_prime, steps, s);
     for (int j = 0; j < nseg; ++j) {
         HIPCHK(hipMemcpyAsync(ck_q(ws, j), ws->st_q, P * sizeof(float), hipMemcpyDeviceToDevice, s));
         HIPCHK(hipMemcpyAsync(ck_i(ws, j), ws->st_i, P * sizeof(float), hipMemcpyDeviceToDevice, s));
@@ -944,6 +962,8 @@ ddrs_status ddrs_mc_window_adj(ddrs_workspace_t ws, const ddrs_params* params, i
     ws->n_ev[TK_LIN] = ws->n_ev[TK_SWEEP] = ws->n_ev[TK_FIN] = 0;
     const int steps = T - 1;
     const int nseg = (steps + ws->window - 1) / ws->window;
+    permute_qprime(ws, d_q_prime, steps, s);
+    if (d_grad_Q) launch_permute_rows(sc, d_grad_Q, 0, steps, ws->gqs, s);
     const float* g_last = d_grad_Q ? d_grad_Q + static_cast<size_t>(T - 1) * R : nullptr;
     if (d_grad_q_prime) launch_zero(static_cast<size_t>(R), d_grad_q_prime + static_cast<size_t>(T - 1) * R, s);
     if (nseg == 0) {
@@ -967,7 +987,6 @@ ddrs_status ddrs_mc_window_adj(ddrs_workspace_t ws, const ddrs_params* params, i
         AdjSegment g;
         g.t0 = j * ws->window;
         g.len = len;
-        g.qprime = d_q_prime;
         g.gQ = d_grad_Q;
         g.gqp_out = d_grad_q_prime;
         g.mask_qp = 1;

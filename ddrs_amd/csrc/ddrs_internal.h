@@ -89,7 +89,7 @@ struct FwdArgs {
     int s_base;      // window-local index of this launch's first step
     int nsteps;      // steps in this launch
     uint32_t epoch;
-    const float* qprime;   // [T][R] reference order
+    const float* qps;      // [T-1][P] q' in schedule order
     float* Q_out;          // [T][R] reference order or null
     float* X_out;          // [R] pre-clamp solve (single step) or null
     uint64_t* hist;        // [window][P] tagged x, schedule order
@@ -105,14 +105,13 @@ struct LinArgs {
     const float4* st;
     Phys P;
     int t0, s_base, nsteps;
-    const float* qprime;   // [T][R]
-    const float* gQ;       // [T][R] or null
+    const float* qps;      // [T-1][P] q' in schedule order
     int mask_qp;           // 1: gradient w.r.t. raw q' through max(q', lb)
     int mask_lam;          // 1: dL/d out[t] is passed on through out[t] = max(x, lb)
     const uint64_t* hist;  // forward history of the segment
     const float* ck_q;     // [P] out[t0] (schedule order)
     const float* ck_i;     // [P] N out[t0]
-    float4* rec;           // [window][P] {-c1, c2, d gq / d gb, dL/d out[t] direct}
+    float4* rec;           // [window][P] {-c1, c2, d gq / d gb, 0}
     unsigned char* recm;   // [window][P] mask of dL/d out[t]
     float4* recj;          // [window][P] d(gn, gqsp, gp, gq') / d gb
 };
@@ -123,7 +122,8 @@ struct SweepArgs {
     Chunks ch;
     Claim claim;
     Trace trace;
-    int s_base, nsteps;
+    int t0, s_base, nsteps;
+    const float* gqs;      // [T-1][P] dL/dQ (direct) in schedule order, or null
     uint32_t epoch;
     const float4* rec;
     const unsigned char* recm;
@@ -182,6 +182,9 @@ void launch_unpermute(const Sched& sc, const float* src, float* dst, hipStream_t
 void launch_grad_q0(const Sched& sc, const float* lam, const float* q0, float lb, float* g,
                     hipStream_t s);
 void launch_zero(size_t n, float* x, hipStream_t s);
+// out[k][p] = q[(t_first + k) * R + perm[p]] for k < rows (0 at empty positions)
+void launch_permute_rows(const Sched& sc, const float* q, int t_first, int rows, float* out,
+                         hipStream_t s);
 
 // Resident workgroups per CU of a persistent kernel.
 enum OccKernel : int { OCC_FWD = 0, OCC_SWEEP = 1, OCC_TRI_FWD = 2, OCC_TRI_ADJ = 3 };

@@ -266,7 +266,7 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
                     sq[j] = a.st_q[p];
                     si[j] = a.st_i[p];
                     rs[r] = load_reach(a.st, p);
-                    qp_next[r] = a.qprime[static_cast<size_t>(t) * R + orig];
+                    qp_next[r] = a.qps[static_cast<size_t>(t) * P + p];
 #pragma unroll
                     for (int k = 0; k < kFastLinks; ++k) {
                         if (ea + k < eb) {
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
                 const float it = si[j];
                 const float qp = fmaxr(qp_next[r], a.P.q_lb);
                 if (s + 1 < a.nsteps)
-                    qp_next[r] = a.qprime[static_cast<size_t>(t + 1) * R + orig];
+                    qp_next[r] = a.qps[static_cast<size_t>(t + 1) * P + p];
                 Coef co;
                 PowEval pe;
                 mc_coef<PH>(a.P, rs[r], q, co, pe);
@@ -435,7 +435,7 @@ constexpr int kLinSteps = 32;
 template <int PH>
 __global__ __launch_bounds__(256) void k_adj_lin(LinArgs a) {
     const Sched& sc = a.sc;
-    const int R = sc.R, P = sc.P;
+    const int P = sc.P;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
     const int orig = sc.perm[p];
@@ -470,15 +470,14 @@ __global__ __launch_bounds__(256) void k_adj_lin(LinArgs a) {
             sx = sx + xu * w;
             inext = inext + fmaxr(xu, lb) * w;
         }
-        const float qp_raw = a.qprime[static_cast<size_t>(t) * R + orig];
+        const float qp_raw = a.qps[static_cast<size_t>(t) * P + p];
         const float qp = fmaxr(qp_raw, lb);
         Coef co;
         PowEval pe;
         mc_coef<PH>(a.P, rc, q, co, pe);
         const LocalGrads g = mc_local_bwd<PH, true>(a.P, co, rc, q, qp, it, 1.0f, sx, 0.0f);
         const size_t o = static_cast<size_t>(sg) * P + p;
-        const float dq = a.gQ ? a.gQ[static_cast<size_t>(t) * R + orig] : 0.0f;
-        a.rec[o] = make_float4(-co.c1, co.c2, g.gq, dq);
+        a.rec[o] = make_float4(-co.c1, co.c2, g.gq, 0.0f);
         a.recm[o] = (!a.mask_lam || q > lb) ? 1 : 0;
         a.recj[o] = make_float4(g.gn, g.gqsp, g.gp, (!a.mask_qp || qp_raw > lb) ? g.gqp : 0.0f);
         q = fmaxr(word_value(hx[p]), lb);
@@ -503,7 +502,7 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
     __shared__ int spos[Wd::kStageDn];
     __shared__ short skind[Wd::kStageDn];
     __shared__ float4 prec[CW];
-    __shared__ float slam[CW];
+    __shared__ float slam[CW], pdq[CW];
     __shared__ unsigned char pm[CW], pvalid[CW];
     __shared__ short pe0[CW], pe1[CW], psub[CW];
     __shared__ short srep[kRepMax];
@@ -527,6 +526,11 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
     // named registers (an indexed array here would live in scratch memory).
     float4 rn0 = make_float4(0.f, 0.f, 0.f, 0.f), rn1 = rn0;
     int mn0 = 0, mn1 = 0;
+    // dL/d out[t] taken directly by the loss, also a step ahead
+    float dq0 = 0.f, dq1 = 0.f;
+    auto direct = [&](int sg, int p) -> float {
+        return a.gqs ? a.gqs[static_cast<size_t>(a.t0 + sg) * P + p] : 0.0f;
+    };
 
     auto prepare = [&](int s, bool first) {
         bool ok_diag = true;
@@ -549,6 +553,7 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
                     slam[j] = a.st_lam[p];
                     (r == 0 ? rn0 : rn1) = a.rec[row + p];
                     (r == 0 ? mn0 : mn1) = a.recm[row + p];
+                    (r == 0 ? dq0 : dq1) = direct(a.s_base + s, p);
 #pragma unroll
                     for (int k = 0; k < kFastLinksDn; ++k) {
                         if (ea + k < eb) {
@@ -560,10 +565,12 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
                 const float4 rcur = (r == 0) ? rn0 : rn1;
                 prec[j] = rcur;
                 pm[j] = static_cast<unsigned char>((r == 0) ? mn0 : mn1);
+                pdq[j] = (r == 0) ? dq0 : dq1;
                 ok_diag = ok_diag && (1.0f + rcur.x * 0.0f == 1.0f);
                 if (s > 0) {
                     (r == 0 ? rn0 : rn1) = a.rec[row - P + p];
                     (r == 0 ? mn0 : mn1) = a.recm[row - P + p];
+                    (r == 0 ? dq0 : dq1) = direct(a.s_base + s - 1, p);
                 }
             } else if (first) {
                 psub[j] = -1;
@@ -572,8 +579,7 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
         unit = __all(ok_diag);
     };
 
-    // The sweep's prepare is short, so its polls are issued when checked
-    // (registers held across prepare would be spilled).
+    // The sweep's prepare is short, so its polls are issued when checked.
     auto issue = [&](int) {};
     auto check = [&](int s) -> bool {
         const uint64_t* row = a.amsg + static_cast<size_t>(a.s_base + s) * P * 2;
@@ -631,7 +637,7 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
             for (int r = 0; r < RPL; ++r) {
                 if (my[r] == l) {
                     const int j = r * 64 + lane;
-                    const float4 rc = prec[j];  // {-c1, c2, dgq/dgb, direct}
+                    const float4 rc = prec[j];  // {-c1, c2, dgq/dgb, -}
                     float sumP = 0.0f, sumG = 0.0f;
                     if (nl[r] <= kFastLinksDn) {
 #pragma unroll
@@ -661,7 +667,7 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
                     ldsP[j] = Pm;
                     ldsG[j] = Gm;
                     gbv[r] = gb;
-                    lamv[r] = pm[j] ? rc.w + (gb * rc.z + sumG) : 0.0f;
+                    lamv[r] = pm[j] ? pdq[j] + (gb * rc.z + sumG) : 0.0f;
                 }
             }
             __syncthreads();
@@ -909,6 +915,16 @@ __global__ void k_grad_q0(Sched sc, const float* lam, const float* q0, float lb,
     if (o >= 0) g[o] = (q0[o] > lb) ? lam[p] : 0.0f;
 }
 
+__global__ void k_permute_rows(Sched sc, const float* q, int t_first, int rows, float* out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= sc.P) return;
+    const int o = sc.perm[p];
+    for (int k = blockIdx.y; k < rows; k += gridDim.y) {
+        out[static_cast<size_t>(k) * sc.P + p] =
+            (o >= 0) ? q[static_cast<size_t>(t_first + k) * sc.R + o] : 0.0f;
+    }
+}
+
 __global__ void k_zero_f(size_t n, float* x) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) x[i] = 0.0f;
@@ -989,6 +1005,13 @@ void launch_unpermute(const Sched& sc, const float* src, float* dst, hipStream_t
 void launch_grad_q0(const Sched& sc, const float* lam, const float* q0, float lb, float* g,
                     hipStream_t s) {
     hipLaunchKernelGGL(k_grad_q0, g1(sc.P), dim3(256), 0, s, sc, lam, q0, lb, g);
+}
+void launch_permute_rows(const Sched& sc, const float* q, int t_first, int rows, float* out,
+                         hipStream_t s) {
+    if (rows <= 0) return;
+    const dim3 grid(static_cast<unsigned>((sc.P + 255) / 256),
+                    static_cast<unsigned>(rows < 1024 ? rows : 1024));
+    hipLaunchKernelGGL(k_permute_rows, grid, dim3(256), 0, s, sc, q, t_first, rows, out);
 }
 void launch_zero(size_t n, float* x, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_zero_f, g1(n), dim3(256), 0, s, n, x);

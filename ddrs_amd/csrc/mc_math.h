@@ -34,49 +34,59 @@
 
 namespace ddrs {
 
+// A wave-uniform double constant placed in a scalar register pair: the vector
+// multiply-adds then take it as an operand, instead of each constant being
+// rebuilt in vector registers with two vector moves (same values, same bits).
+DDRS_HD inline double kc(double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm("" : "+s"(c));
+#endif
+    return c;
+}
+
 // ln(x) for finite x > 0 (double), ~2^-50 relative (absolute near x = 1)
 DDRS_HD inline double ln_reduced(double x) {
     int e;
     double m = frexp(x, &e);  // m in [0.5, 1)
-    if (m < 0.70710678118654752440) {
+    if (m < kc(0.70710678118654752440)) {
         m = m + m;
         e -= 1;
     }
     const double s = (m - 1.0) / (m + 1.0);
     const double z = s * s;
-    double p = 1.0 / 19.0;
-    p = fma(p, z, 1.0 / 17.0);
-    p = fma(p, z, 1.0 / 15.0);
-    p = fma(p, z, 1.0 / 13.0);
-    p = fma(p, z, 1.0 / 11.0);
-    p = fma(p, z, 1.0 / 9.0);
-    p = fma(p, z, 1.0 / 7.0);
-    p = fma(p, z, 1.0 / 5.0);
-    p = fma(p, z, 1.0 / 3.0);
+    double p = kc(1.0 / 19.0);
+    p = fma(p, z, kc(1.0 / 17.0));
+    p = fma(p, z, kc(1.0 / 15.0));
+    p = fma(p, z, kc(1.0 / 13.0));
+    p = fma(p, z, kc(1.0 / 11.0));
+    p = fma(p, z, kc(1.0 / 9.0));
+    p = fma(p, z, kc(1.0 / 7.0));
+    p = fma(p, z, kc(1.0 / 5.0));
+    p = fma(p, z, kc(1.0 / 3.0));
     const double lnm = fma(2.0 * s * z, p, 2.0 * s);  // 2 s (1 + z p)
-    const double ln2_hi = 6.93147180369123816490e-01;
-    const double ln2_lo = 1.90821492927058770002e-10;
+    const double ln2_hi = kc(6.93147180369123816490e-01);
+    const double ln2_lo = kc(1.90821492927058770002e-10);
     const double de = static_cast<double>(e);
     return fma(de, ln2_hi, fma(de, ln2_lo, lnm));
 }
 
 // e^t for |t| <= 800 (overflow to inf and underflow to 0 through ldexp), ~2^-50 relative
 DDRS_HD inline double exp_reduced(double t) {
-    const double inv_ln2 = 1.44269504088896338700e+00;
-    const double ln2_hi = 6.93147180369123816490e-01;
-    const double ln2_lo = 1.90821492927058770002e-10;
+    const double inv_ln2 = kc(1.44269504088896338700e+00);
+    const double ln2_hi = kc(6.93147180369123816490e-01);
+    const double ln2_lo = kc(1.90821492927058770002e-10);
     const double k = rint(t * inv_ln2);
     const double r = fma(-k, ln2_lo, fma(-k, ln2_hi, t));
-    double p = 1.0 / 479001600.0;        // 1/12!
-    p = fma(p, r, 1.0 / 39916800.0);     // 1/11!
-    p = fma(p, r, 1.0 / 3628800.0);
-    p = fma(p, r, 1.0 / 362880.0);
-    p = fma(p, r, 1.0 / 40320.0);
-    p = fma(p, r, 1.0 / 5040.0);
-    p = fma(p, r, 1.0 / 720.0);
-    p = fma(p, r, 1.0 / 120.0);
-    p = fma(p, r, 1.0 / 24.0);
-    p = fma(p, r, 1.0 / 6.0);
+    double p = kc(1.0 / 479001600.0);    // 1/12!
+    p = fma(p, r, kc(1.0 / 39916800.0)); // 1/11!
+    p = fma(p, r, kc(1.0 / 3628800.0));
+    p = fma(p, r, kc(1.0 / 362880.0));
+    p = fma(p, r, kc(1.0 / 40320.0));
+    p = fma(p, r, kc(1.0 / 5040.0));
+    p = fma(p, r, kc(1.0 / 720.0));
+    p = fma(p, r, kc(1.0 / 120.0));
+    p = fma(p, r, kc(1.0 / 24.0));
+    p = fma(p, r, kc(1.0 / 6.0));
     p = fma(p, r, 0.5);
     p = fma(p, r, 1.0);
     p = fma(p, r, 1.0);
