@@ -65,6 +65,7 @@ _SIGS = {
     "ddrs_get_adj_phase_timing": (c_int, [P, P, P, P]),
     "ddrs_workspace_layout": (c_int, [P, P, P, P, P]),
     "ddrs_workspace_set_chunk_width": (c_int, [P, c_int, c_int]),
+    "ddrs_workspace_set_linearize_in_forward": (c_int, [P, c_int, c_int]),
     "ddrs_set_trace": (c_int, [P, c_int]),
     "ddrs_get_trace": (c_int, [P, c_int, P, c_int, P]),
     "ddrs_build_topology": (c_int, [c_int, P, P, c_int, P, P, P, c_int, P, P, c_int, P]),

@@ -114,6 +114,27 @@ struct LinArgs {
     float4* rec;           // [window][P] {-c1, c2, d gq / d gb, 0}
     unsigned char* recm;   // [window][P] mask of dL/d out[t]
     float4* recj;          // [window][P] d(gn, gqsp, gp, gq') / d gb
+    // With items == null the grid holds one workgroup per 256 positions and 32
+    // steps. Otherwise one-wave workgroups take items (64 positions, 32 steps)
+    // in step-block-major order from the counter *items (zero before the first
+    // launch that uses it) until n_items are taken.
+    unsigned long long* items;
+    int n_items;
+    // Linearising while the forward window still runs (fwd_epoch != 0): the
+    // history is read with device-coherent loads and every word checked for
+    // the forward's epoch (awaited when missing). A workgroup starts taking
+    // items only once every forward workgroup has claimed its chunk
+    // (*fwd_claim_ctr >= fwd_claims), and leaves its share to a later launch
+    // on the same counter if the claims stop advancing for a while.
+    uint32_t fwd_epoch;
+    const unsigned long long* fwd_claim_ctr;
+    unsigned long long fwd_claims;
+    int* err;
+    // Optional trace of a launch on the counter (null: off): for item k,
+    // trace[8k..8k+2] = wall clock when taken and when done, and `tag`; the
+    // record after the last item counts workgroups that left without an item.
+    unsigned long long* trace;
+    int tag;
 };
 
 // Adjoint, phase 2 (persistent, reverse time): the transposed solves.
@@ -163,7 +184,11 @@ struct TriArgs {
 
 // Persistent launches run one workgroup per chunk (grid = Chunks::C).
 void launch_fwd(const FwdArgs& a, int variant, int rpl, int grid, hipStream_t s);
-void launch_lin(const LinArgs& a, int variant, hipStream_t s);
+// grid: one workgroup per item (a.items null), else `grid` workgroups
+void launch_lin(const LinArgs& a, int variant, int grid, hipStream_t s);
+// linearisation work items of a launch of nsteps steps over P positions
+int lin_items(int nsteps, int P);
+void launch_zero_u64(size_t n, unsigned long long* x, hipStream_t s);
 void launch_sweep(const SweepArgs& a, int rpl, int grid, hipStream_t s);
 void launch_fin(const FinArgs& a, hipStream_t s);
 void launch_tri_fwd(const TriArgs& a, int grid, hipStream_t s);

@@ -431,29 +431,97 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
 // every clamp decision are computed exactly as in the forward.
 
 constexpr int kLinSteps = 32;
+constexpr int kLinBlock = 256;   // positions per workgroup of the one-launch grid
+constexpr int kLinItem = 64;     // positions per item of a launch on the item counter
+// A concurrent launch waits for the forward's claims while they advance; it
+// leaves when they stall this long (its own waves may be what keeps the
+// forward's last workgroups from becoming resident).
+constexpr uint64_t kClaimStallTicks = 5000;  // 50 us at 100 MHz
 
-template <int PH>
-__global__ __launch_bounds__(256) void k_adj_lin(LinArgs a) {
+// Steps [s_first, s_first + kLinSteps) of position p. Coh: the forward that
+// writes the history may still run; history words are read with coherent
+// loads and checked for its epoch (awaited when missing).
+template <int PH, bool Coh>
+__device__ __forceinline__ void lin_item(const LinArgs& a, int sb, int p) {
     const Sched& sc = a.sc;
     const int P = sc.P;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
     const int orig = sc.perm[p];
     if (orig < 0) return;
-    const int s_first = a.s_base + static_cast<int>(blockIdx.y) * kLinSteps;
+    const int s_first = a.s_base + sb * kLinSteps;
     const int s_end = min(s_first + kLinSteps, a.s_base + a.nsteps);
     const float lb = a.P.q_lb;
     const int e0 = sc.up_ptr[p], e1 = sc.up_ptr[p + 1];
     const Reach rc = load_reach(a.st, p);
+    const uint32_t ep = a.fwd_epoch;
+    // history word, its epoch folded into `ok`
+    auto rd = [&](const uint64_t* w, bool& ok) -> float {
+        if constexpr (!Coh) {
+            return word_value(*w);
+        } else {
+            const uint64_t v = poll(w);
+            ok = ok && word_ready(v, ep);
+            return word_value(v);
+        }
+    };
+    // the same word, awaited until the forward has written it; gives up (error
+    // flag set) after the watchdog time or once any workgroup has raised the
+    // flag, as the forward's workgroups do when they give up
+    auto wt = [&](const uint64_t* w) -> float {
+        uint64_t v = poll(w);
+        const uint64_t t0 = wall_ticks();
+        int nap = 1;
+        while (!word_ready(v, ep)) {
+            for (int k = 0; k < nap; ++k) __builtin_amdgcn_s_sleep(8);
+            nap = (nap < 8) ? 2 * nap : 8;
+            if (wall_ticks() - t0 > kWatchdogTicks ||
+                __hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                atomicExch(a.err, 1);
+                return 0.0f;
+            }
+            v = poll(w);
+        }
+        return word_value(v);
+    };
+    // sx = sum_k adj_k x[up k] (B26 at gb = 1: gc1 = -sum_k adj_k ((-1) x[col k]))
+    // and in = sum_k adj_k max(x[up k], lb) = N out over the upstream of p in `row`
+    auto inflow = [&](const uint64_t* row, float& sx, float& in, bool& ok) {
+        sx = 0.0f;
+        in = 0.0f;
+        for (int e = e0; e < e1; ++e) {
+            const float xu = rd(row + sc.up_pos[e], ok);
+            const float w = sc.up_adj[e];
+            sx = sx + xu * w;
+            in = in + fmaxr(xu, lb) * w;
+        }
+    };
+    auto inflow_wait = [&](const uint64_t* row, float& sx, float& in) {
+        sx = 0.0f;
+        in = 0.0f;
+        for (int e = e0; e < e1; ++e) {
+            const float xu = wt(row + sc.up_pos[e]);
+            const float w = sc.up_adj[e];
+            sx = sx + xu * w;
+            in = in + fmaxr(xu, lb) * w;
+        }
+    };
+    if constexpr (Coh) {
+        // start once this position's last step of the item is out; its
+        // upstream words were published before it
+        (void)wt(a.hist + static_cast<size_t>(s_end - 1) * P + p);
+    }
     // state entering the first step: out[t] and N out[t]
     float q, it;
     if (s_first > 0) {
         const uint64_t* hq = a.hist + static_cast<size_t>(s_first - 1) * P;
-        q = fmaxr(word_value(hq[p]), lb);
-        float acc = 0.0f;
-        for (int e = e0; e < e1; ++e)
-            acc = acc + fmaxr(word_value(hq[sc.up_pos[e]]), lb) * sc.up_adj[e];
-        it = acc;
+        bool ok = true;
+        float xq = rd(hq + p, ok), sx_unused;
+        inflow(hq, sx_unused, it, ok);
+        if (Coh && !ok) {
+            xq = wt(hq + p);
+            inflow_wait(hq, sx_unused, it);
+        }
+        q = fmaxr(xq, lb);
     } else {
         q = a.ck_q[p];
         it = a.ck_i[p];
@@ -461,14 +529,13 @@ __global__ __launch_bounds__(256) void k_adj_lin(LinArgs a) {
     for (int sg = s_first; sg < s_end; ++sg) {
         const int t = a.t0 + sg;
         const uint64_t* hx = a.hist + static_cast<size_t>(sg) * P;
-        // B26 for gb = 1: gc1 = -sum_k adj_k ((-1) x[col k]) = sum_k x[col k] adj_k,
-        // and N out[t+1] for the next step
-        float sx = 0.0f, inext = 0.0f;
-        for (int e = e0; e < e1; ++e) {
-            const float xu = word_value(hx[sc.up_pos[e]]);
-            const float w = sc.up_adj[e];
-            sx = sx + xu * w;
-            inext = inext + fmaxr(xu, lb) * w;
+        bool ok = true;
+        float sx, inext;
+        inflow(hx, sx, inext, ok);
+        float xs = rd(hx + p, ok);
+        if (Coh && !ok) {
+            inflow_wait(hx, sx, inext);
+            xs = wt(hx + p);
         }
         const float qp_raw = a.qps[static_cast<size_t>(t) * P + p];
         const float qp = fmaxr(qp_raw, lb);
@@ -480,8 +547,61 @@ __global__ __launch_bounds__(256) void k_adj_lin(LinArgs a) {
         a.rec[o] = make_float4(-co.c1, co.c2, g.gq, 0.0f);
         a.recm[o] = (!a.mask_lam || q > lb) ? 1 : 0;
         a.recj[o] = make_float4(g.gn, g.gqsp, g.gp, (!a.mask_qp || qp_raw > lb) ? g.gqp : 0.0f);
-        q = fmaxr(word_value(hx[p]), lb);
+        q = fmaxr(xs, lb);
         it = inext;
+    }
+}
+
+// One workgroup per item: grid x = block of kLinBlock positions, y = step block.
+template <int PH>
+__global__ __launch_bounds__(kLinBlock) void k_adj_lin(LinArgs a) {
+    lin_item<PH, false>(a, static_cast<int>(blockIdx.y),
+                        static_cast<int>(blockIdx.x) * kLinBlock + static_cast<int>(threadIdx.x));
+}
+
+// One-wave workgroups taking items of kLinItem positions from the counter, in
+// step-block-major order.
+template <int PH, bool Coh>
+__global__ __launch_bounds__(kLinItem) void k_adj_lin_items(LinArgs a) {
+    const int npb = (a.sc.P + kLinItem - 1) / kLinItem;
+    if constexpr (Coh) {
+        // wait for the forward's claims while they advance
+        const auto* ctr = const_cast<unsigned long long*>(a.fwd_claim_ctr);
+        unsigned long long seen = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint64_t t_seen = wall_ticks();
+        while (seen < a.fwd_claims) {
+            __builtin_amdgcn_s_sleep(8);
+            const unsigned long long v =
+                __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint64_t now = wall_ticks();
+            if (v != seen) {
+                seen = v;
+                t_seen = now;
+            } else if (now - t_seen > kClaimStallTicks) {
+                if (a.trace && threadIdx.x == 0) atomicAdd(a.trace + kTraceWords * a.n_items, 1ull);
+                return;
+            }
+        }
+    }
+    for (;;) {
+        int k = 0;
+        if (threadIdx.x == 0) {
+            const unsigned long long c = atomicAdd(a.items, 1ull);
+            k = (c < static_cast<unsigned long long>(a.n_items)) ? static_cast<int>(c) : -1;
+        }
+        k = __shfl(k, 0);
+        if (k < 0) break;
+        const uint64_t t_take = a.trace ? wall_ticks() : 0;
+        lin_item<PH, Coh>(a, k / npb, (k % npb) * kLinItem + static_cast<int>(threadIdx.x));
+        if (a.trace) {
+            __builtin_amdgcn_wave_barrier();
+            if (threadIdx.x == 0) {
+                unsigned long long* r = a.trace + static_cast<size_t>(kTraceWords) * k;
+                r[0] = t_take;
+                r[1] = wall_ticks();
+                r[2] = static_cast<unsigned long long>(a.tag);
+            }
+        }
     }
 }
 
@@ -929,6 +1049,10 @@ __global__ void k_zero_f(size_t n, float* x) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) x[i] = 0.0f;
 }
+__global__ void k_zero_u64(size_t n, unsigned long long* x) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0ull;
+}
 
 // ---------------------------------------------------------------------------
 // launch helpers
@@ -953,13 +1077,37 @@ void launch_fwd(const FwdArgs& a, int variant, int rpl, int grid, hipStream_t s)
 
 static inline dim3 g1(size_t n) { return dim3(static_cast<unsigned>((n + 255) / 256)); }
 
-void launch_lin(const LinArgs& a, int variant, hipStream_t s) {
-    const dim3 grid(static_cast<unsigned>((a.sc.P + 255) / 256),
-                    static_cast<unsigned>((a.nsteps + kLinSteps - 1) / kLinSteps));
+int lin_items(int nsteps, int P) {
+    return ((nsteps + kLinSteps - 1) / kLinSteps) * ((P + kLinItem - 1) / kLinItem);
+}
+
+void launch_lin(const LinArgs& a, int variant, int grid, hipStream_t s) {
+    if (!a.items) {
+        const dim3 g(static_cast<unsigned>((a.sc.P + kLinBlock - 1) / kLinBlock),
+                     static_cast<unsigned>((a.nsteps + kLinSteps - 1) / kLinSteps));
+        const dim3 b(kLinBlock);
+        switch (variant) {
+            case PH_DDR: hipLaunchKernelGGL(k_adj_lin<PH_DDR>, g, b, 0, s, a); break;
+            case PH_POS: hipLaunchKernelGGL(k_adj_lin<PH_POS>, g, b, 0, s, a); break;
+            default: hipLaunchKernelGGL(k_adj_lin<PH_DEFAULT>, g, b, 0, s, a); break;
+        }
+        return;
+    }
+    const dim3 g(static_cast<unsigned>(grid)), b(kLinItem);
+    const bool coh = a.fwd_epoch != 0;
     switch (variant) {
-        case PH_DDR: hipLaunchKernelGGL(k_adj_lin<PH_DDR>, grid, dim3(256), 0, s, a); break;
-        case PH_POS: hipLaunchKernelGGL(k_adj_lin<PH_POS>, grid, dim3(256), 0, s, a); break;
-        default: hipLaunchKernelGGL(k_adj_lin<PH_DEFAULT>, grid, dim3(256), 0, s, a); break;
+        case PH_DDR:
+            if (coh) hipLaunchKernelGGL((k_adj_lin_items<PH_DDR, true>), g, b, 0, s, a);
+            else hipLaunchKernelGGL((k_adj_lin_items<PH_DDR, false>), g, b, 0, s, a);
+            break;
+        case PH_POS:
+            if (coh) hipLaunchKernelGGL((k_adj_lin_items<PH_POS, true>), g, b, 0, s, a);
+            else hipLaunchKernelGGL((k_adj_lin_items<PH_POS, false>), g, b, 0, s, a);
+            break;
+        default:
+            if (coh) hipLaunchKernelGGL((k_adj_lin_items<PH_DEFAULT, true>), g, b, 0, s, a);
+            else hipLaunchKernelGGL((k_adj_lin_items<PH_DEFAULT, false>), g, b, 0, s, a);
+            break;
     }
 }
 void launch_sweep(const SweepArgs& a, int rpl, int grid, hipStream_t s) {
@@ -1015,6 +1163,9 @@ void launch_permute_rows(const Sched& sc, const float* q, int t_first, int rows,
 }
 void launch_zero(size_t n, float* x, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_zero_f, g1(n), dim3(256), 0, s, n, x);
+}
+void launch_zero_u64(size_t n, unsigned long long* x, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_zero_u64, g1(n), dim3(256), 0, s, n, x);
 }
 
 template <class K>

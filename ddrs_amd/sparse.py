@@ -154,6 +154,16 @@ class Workspace:
         N.check(N.lib.ddrs_workspace_set_chunk_width(self.handle, int(fwd), int(sweep)),
                 "ddrs_workspace_set_chunk_width")
 
+    def set_linearize_in_forward(self, enable: bool, grid: int = 0):
+        """Linearise the adjoint while a persistent forward window runs, so
+        that the adjoint that follows skips that phase (bit-identical results;
+        worth it only when an adjoint follows). grid: workgroups of the
+        concurrent launch, 0 for the default. Currently disabled in the
+        library: enable=True raises (a 350000-reach window did not finish)."""
+        N.check(N.lib.ddrs_workspace_set_linearize_in_forward(self.handle, int(bool(enable)),
+                                                              int(grid)),
+                "ddrs_workspace_set_linearize_in_forward")
+
 
 class _TriSolve(torch.autograd.Function):
     @staticmethod

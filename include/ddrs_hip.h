@@ -214,13 +214,32 @@ ddrs_status ddrs_workspace_layout(ddrs_workspace_t ws, int32_t* rpl_fwd, int32_t
                                   int32_t* rpl_sweep, int32_t* grid_sweep);
 ddrs_status ddrs_workspace_set_chunk_width(ddrs_workspace_t ws, int32_t rpl_fwd,
                                            int32_t rpl_sweep);
+/* Linearise the adjoint while the forward window runs (enable = 1; off by
+ * default). A persistent ddrs_mc_window_fwd then also computes, on a second
+ * stream of the workspace (default priority) ordered against `stream` by
+ * events, the per-step coefficients and local backward that the following
+ * ddrs_mc_window_adj would compute first, reading the forward's history as it
+ * is published; ddrs_mc_window_adj with the same parameters and q' skips that
+ * phase. Worth enabling only when an adjoint follows every forward. grid: the
+ * one-wave workgroups of the concurrent launch (0: 16 per compute unit). Results
+ * are bit-identical either way. No reference counterpart (the reference's
+ * backward in src/routing/mmc_op.rs:1704-1857 runs after the forward).
+ * Currently disabled: enable = 1 returns DDRS_ERR_INVALID (a 350000-reach,
+ * 720-step window with the concurrent launch did not finish; the cause is not
+ * known); enable = 0 is accepted. */
+ddrs_status ddrs_workspace_set_linearize_in_forward(ddrs_workspace_t ws, int32_t enable,
+                                                   int32_t grid);
 /* Per-chunk trace of the persistent forward (which = 0) and adjoint-sweep
  * (which = 1) launches: for each chunk, 8 values: GPU wall clock (100 MHz
  * ticks) when it was claimed, when its first and last steps were solved; the
  * number of polls that found an input missing; and ticks spent in prepare, in
  * the first poll of each step, waiting for late inputs, and in solve. h_rec
  * holds cap chunk records (8 each); n_chunks gets the chunk count of the last
- * launch. No reference counterpart. */
+ * launch. which = 2: the items (64 positions x 32 steps) of the last
+ * linearisation run beside a forward window, 8 values each: wall clock when
+ * taken and when done, and 1 if the concurrent launch did it (0: the launch
+ * after the forward); then one record whose first value counts the concurrent
+ * launch's workgroups that left without an item. No reference counterpart. */
 ddrs_status ddrs_set_trace(ddrs_workspace_t ws, int32_t enable);
 ddrs_status ddrs_get_trace(ddrs_workspace_t ws, int32_t which, uint64_t* h_rec, int32_t cap,
                            int32_t* n_chunks);
