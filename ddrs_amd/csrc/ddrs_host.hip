@@ -93,7 +93,6 @@ struct ddrs_workspace {
     const float* qps_src = nullptr;          // q' it was permuted from (rows qps_rows)
     int qps_rows = 0;
     float4 *rec = nullptr, *recj = nullptr;  // [window][P]
-    unsigned char* recm = nullptr;           // [window][P]
     float* gbuf = nullptr;                   // [window][P]
     int* err = nullptr;
     uint32_t epoch = 0;
@@ -106,7 +105,7 @@ struct ddrs_workspace {
     // Linearising the adjoint while the forward window runs (opt-in): a
     // second stream of default priority, events ordering it against the
     // caller's stream, the work-item counter, and the segment whose records
-    // rec/recm/recj are current (-1: none) with the parameters they used.
+    // rec/recj are current (-1: none) with the parameters they used.
     bool lin_in_fwd = false;
     int lin_grid = 0, ncu = 0;
     hipStream_t lin_stream = nullptr;
@@ -598,7 +597,6 @@ ddrs_status ddrs_workspace_create(ddrs_plan_t plan, int32_t window, int32_t max_
     A(&ws->amsg, WR * 16);
     A(&ws->rec, WR * sizeof(float4));
     A(&ws->recj, WR * sizeof(float4));
-    A(&ws->recm, WR);
     A(&ws->qps, static_cast<size_t>(std::max(max_steps, 1)) * R * 4);
     A(&ws->gqs, static_cast<size_t>(std::max(max_steps, 1)) * R * 4);
     A(&ws->gbuf, WR * 4);
@@ -758,7 +756,6 @@ static LinArgs lin_args(ddrs_workspace* ws, const ddrs_params* p, int t0, int ma
     L.ck_q = ck_q;
     L.ck_i = ck_i;
     L.rec = ws->rec;
-    L.recm = ws->recm;
     L.recj = ws->recj;
     L.err = ws->err;
     return L;
@@ -789,7 +786,6 @@ static void run_adj_segment(ddrs_workspace* ws, const ddrs_params* p, const AdjS
     W.t0 = g.t0;
     W.gqs = g.gQ ? ws->gqs : nullptr;
     W.rec = ws->rec;
-    W.recm = ws->recm;
     W.amsg = ws->amsg;
     W.st_lam = ws->st_lam;
     W.gbuf = ws->gbuf;

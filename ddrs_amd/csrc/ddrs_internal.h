@@ -111,8 +111,7 @@ struct LinArgs {
     const uint64_t* hist;  // forward history of the segment
     const float* ck_q;     // [P] out[t0] (schedule order)
     const float* ck_i;     // [P] N out[t0]
-    float4* rec;           // [window][P] {-c1, c2, d gq / d gb, 0}
-    unsigned char* recm;   // [window][P] mask of dL/d out[t]
+    float4* rec;           // [window][P] {-c1, c2, d gq / d gb, mask of dL/d out[t]}
     float4* recj;          // [window][P] d(gn, gqsp, gp, gq') / d gb
     // With items == null the grid holds one workgroup per 256 positions and 32
     // steps. Otherwise one-wave workgroups take items (64 positions, 32 steps)
@@ -147,7 +146,6 @@ struct SweepArgs {
     const float* gqs;      // [T-1][P] dL/dQ (direct) in schedule order, or null
     uint32_t epoch;
     const float4* rec;
-    const unsigned char* recm;
     uint64_t* amsg;        // [window][P][2] tagged (P, G)
     float* st_lam;         // [P] gradient into x of the step (masked dL/d out[t+1])
     float* gbuf;           // [window][P] gb of every reach-step

@@ -60,6 +60,28 @@ __device__ __forceinline__ bool word_ready(uint64_t w, uint32_t epoch) {
 }
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Marks a loaded value as used here. The compiler waits for a load at the
+// first use of its register, and the wave's memory counter cannot single out
+// one load: once a loop whose loads it cannot count (the polling loop) or a
+// store has been issued after it, that wait becomes a wait for everything in
+// flight. Prefetched values are therefore touched where everything issued
+// before has arrived anyway, which leaves their later use free of waits.
+__device__ __forceinline__ void touch(float v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void touch(int v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void touch(const float4& v) {
+    asm volatile("" ::"v"(v.x), "v"(v.y), "v"(v.z), "v"(v.w));
+}
+
+// Loads `bytes` (4 or 16) per lane from g straight into LDS at lds + lane *
+// bytes (lds the same for the whole wave); the data passes through no
+// register. The compiler waits for the load before LDS reads of that data.
+__device__ __forceinline__ void load_to_lds(const void* g, void* lds, int bytes) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (bytes == 16) __builtin_amdgcn_global_load_lds(g, lds, 16, 0, 0);
+    else __builtin_amdgcn_global_load_lds(g, lds, 4, 0, 0);
+#endif
+}
+
 // 100 MHz wall clock
 __device__ __forceinline__ uint64_t wall_ticks() { return __builtin_amdgcn_s_memrealtime(); }
 constexpr uint64_t kWatchdogTicks = 200000000ull;  // 2 s at 100 MHz
@@ -82,8 +104,10 @@ __device__ __forceinline__ int claim_chunk(const Claim& cl, int C, int* lds) {
 }
 
 // For the launch's steps in time order (Dir = +1) or reverse (Dir = -1):
-//   issue(s)           starts the loads of the cross-chunk inputs of step s
+//   issue(s, again)    starts the loads of the cross-chunk inputs of step s
 //                      (device-coherent, so each costs a full round trip);
+//                      `again` when the inputs are read again while waiting
+//                      (anything else issue(s) loads need not be reloaded);
 //   prepare(s, first)  everything of step s that does not depend on other
 //                      chunks (coefficients, right-hand side, into LDS);
 //                      `first` asks for the chunk's step-invariant data too;
@@ -92,10 +116,15 @@ __device__ __forceinline__ int claim_chunk(const Claim& cl, int C, int* lds) {
 //   probe(s)           while waiting: polls one representative input per
 //                      producing chunk (the one it publishes last); true when
 //                      all of them belong to step s, so the inputs are read again;
-//   solve(s, last)     the level-by-level solve inside the chunk, publishing,
+//   solve(s, last, h)  the level-by-level solve inside the chunk, publishing,
 //                      and the step's epilogue; `last` is the launch's final step.
-// The inputs are requested before prepare so that their round trip can
-// overlap the coefficient arithmetic. A waiting chunk polls only the
+//                      It calls h() after its arithmetic and before its first
+//                      store: h issues the next step's loads.
+// The next step's inputs are requested before this step's stores: the wave's
+// memory counter is in order, so a wait for loads issued after the stores
+// would also wait for the stores to be acknowledged (a round trip to the
+// coherence point), while loads issued before them are waited for alone. The
+// requests also overlap the next step's coefficient arithmetic. A waiting chunk polls only the
 // representatives, after a growing pause (0.2 to 1.7 us), so that waiting
 // chunks do not flood the memory system. All five are called by the whole wave. Returns early (error flag set) when
 // the inputs of a step do not arrive within the watchdog time or another
@@ -110,10 +139,9 @@ __device__ __forceinline__ void chunk_loop(int nsteps, int* err, unsigned long l
     for (int done = 0; done < nsteps; ++done) {
         const int s = (Dir > 0) ? done : (nsteps - 1 - done);
         const uint64_t ta = trace ? wall_ticks() : 0;
-        if (done > 0) issue(s);  // the first step stages its links in prepare
         prepare(s, done == 0);
         __syncthreads();
-        if (done == 0) issue(s);
+        if (done == 0) issue(s, false);  // the first step stages its links in prepare
         const uint64_t tb = trace ? wall_ticks() : 0;
         const bool ok = check(s);
         const uint64_t tc = trace ? wall_ticks() : 0;
@@ -132,7 +160,7 @@ __device__ __forceinline__ void chunk_loop(int nsteps, int* err, unsigned long l
                     return;
                 }
                 if (!probe(s)) continue;
-                issue(s);
+                issue(s, true);
                 if (check(s)) break;
             }
         }
@@ -140,7 +168,10 @@ __device__ __forceinline__ void chunk_loop(int nsteps, int* err, unsigned long l
         // The solve is a chain of dependent LDS round trips; it issues ahead of
         // the other waves on the SIMD, whose arithmetic fills the gaps.
         __builtin_amdgcn_s_setprio(3);
-        solve(s, done == nsteps - 1);
+        auto issue_next = [&]() {
+            if (done + 1 < nsteps) issue((Dir > 0) ? s + 1 : s - 1, false);
+        };
+        solve(s, done == nsteps - 1, issue_next);
         __builtin_amdgcn_s_setprio(0);
         if (trace) {
             const uint64_t te = wall_ticks();
@@ -281,7 +312,7 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
                 if (s + 1 < a.nsteps)
                     qp_next[r] = a.qps[static_cast<size_t>(t + 1) * P + p];
                 Coef co;
-                PowEval pe;
+                PowEvalInl pe;
                 mc_coef<PH>(a.P, rs[r], q, co, pe);
                 pb[j] = ((co.c2 * it) + (co.c3 * q)) + (co.c4 * qp);             // S25
                 pnc1[j] = -co.c1;
@@ -294,7 +325,7 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
     };
 
     uint64_t pw[kIssueSlots];
-    auto issue = [&](int s) {
+    auto issue = [&](int s, bool) {
         const uint64_t* row = a.hist + static_cast<size_t>(a.s_base + s) * P;
 #pragma unroll
         for (int k = 0; k < kIssueSlots; ++k) {
@@ -313,11 +344,23 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
                 else sval[e] = word_value(pw[k]);
             }
         }
-        for (int e = lane + 64 * kIssueSlots; e < nlinks; e += 64) {
-            if (skind[e] < 0) {
-                const uint64_t wv = poll(row + spos[e]);
-                if (!word_ready(wv, a.epoch)) ok = false;
-                else sval[e] = word_value(wv);
+        // links beyond the issued ones: all loads first, then the checks, so
+        // that their round trips overlap
+        if (nlinks > 64 * kIssueSlots) {
+            constexpr int kMore = Wd::kStageUp / 64 - kIssueSlots;
+            uint64_t wx[kMore];
+#pragma unroll
+            for (int k = 0; k < kMore; ++k) {
+                const int e = lane + 64 * (kIssueSlots + k);
+                wx[k] = (e < nlinks && skind[e] < 0) ? poll(row + spos[e]) : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < kMore; ++k) {
+                const int e = lane + 64 * (kIssueSlots + k);
+                if (e < nlinks && skind[e] < 0) {
+                    if (!word_ready(wx[k], a.epoch)) ok = false;
+                    else sval[e] = word_value(wx[k]);
+                }
             }
         }
         const bool all = __all(ok);
@@ -334,10 +377,13 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
     };
 
     // S26..S28 and the next step's S24 (N Q_{t+1}) from LDS only.
-    auto solve = [&](int s, bool last) {
+    auto solve = [&](int s, bool last, auto&& before_stores) {
         const int sg = a.s_base + s;
         const int t = a.t0 + sg;
         uint64_t* row = a.hist + static_cast<size_t>(sg) * P;
+        // q' of the next step, loaded in prepare, has arrived by now (see touch())
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) touch(qp_next[r]);
         // per-lane operands of the solve, read once per step
         int my[RPL], e0[RPL], nl[RPL], code[RPL][kFastLinks];
         float x[RPL], inext[RPL], b[RPL], nc1[RPL], w[RPL][kFastLinks];
@@ -389,9 +435,9 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
             }
             __syncthreads();
         }
+        before_stores();
         // Publish after the solve: other chunks read a step's values only once
-        // its solve is done, and a fixed number of stores per step lets the
-        // next step's loads wait for their own data alone.
+        // its solve is done.
 #pragma unroll
         for (int r = 0; r < RPL; ++r)
             if (my[r] >= 0) publish(row + p0 + r * 64 + lane, x[r], a.epoch);
@@ -544,8 +590,7 @@ __device__ __forceinline__ void lin_item(const LinArgs& a, int sb, int p) {
         mc_coef<PH>(a.P, rc, q, co, pe);
         const LocalGrads g = mc_local_bwd<PH, true>(a.P, co, rc, q, qp, it, 1.0f, sx, 0.0f);
         const size_t o = static_cast<size_t>(sg) * P + p;
-        a.rec[o] = make_float4(-co.c1, co.c2, g.gq, 0.0f);
-        a.recm[o] = (!a.mask_lam || q > lb) ? 1 : 0;
+        a.rec[o] = make_float4(-co.c1, co.c2, g.gq, (!a.mask_lam || q > lb) ? 1.0f : 0.0f);
         a.recj[o] = make_float4(g.gn, g.gqsp, g.gp, (!a.mask_qp || qp_raw > lb) ? g.gqp : 0.0f);
         q = fmaxr(xs, lb);
         it = inext;
@@ -621,9 +666,12 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
     __shared__ float stP[Wd::kStageDn], stG[Wd::kStageDn], sadj[Wd::kStageDn];
     __shared__ int spos[Wd::kStageDn];
     __shared__ short skind[Wd::kStageDn];
+    // the step's records {-c1, c2, dgq/dgb, mask of dL/d out[t]} and direct
+    // dL/d out[t], loaded straight into LDS
     __shared__ float4 prec[CW];
-    __shared__ float slam[CW], pdq[CW];
-    __shared__ unsigned char pm[CW], pvalid[CW];
+    __shared__ float pdq[CW];
+    __shared__ float slam[CW];
+    __shared__ unsigned char pvalid[CW];
     __shared__ short pe0[CW], pe1[CW], psub[CW];
     __shared__ short srep[kRepMax];
     __shared__ int lkc[kFastLinksDn][CW];
@@ -642,78 +690,100 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
     const int nlinks = sc.dn_ptr[p0 + CW] - base;
     const int nsub = a.ch.nsubr[c];
 
-    // The records of the next (earlier) step are loaded one step ahead, into
-    // named registers (an indexed array here would live in scratch memory).
-    float4 rn0 = make_float4(0.f, 0.f, 0.f, 0.f), rn1 = rn0;
-    int mn0 = 0, mn1 = 0;
-    // dL/d out[t] taken directly by the loss, also a step ahead
-    float dq0 = 0.f, dq1 = 0.f;
-    auto direct = [&](int sg, int p) -> float {
-        return a.gqs ? a.gqs[static_cast<size_t>(a.t0 + sg) * P + p] : 0.0f;
-    };
+    // A step's records and its cross-chunk inputs of the first link slot are
+    // loaded by issue(s), which the previous step calls before its stores.
+    // The records go from memory straight to LDS: a register that carries a
+    // load from one step to the next makes the compiler wait for everything
+    // in flight (the previous step's stores included) before reusing it.
+    uint64_t pwP = 0, pwG = 0;  // P and G words of link `lane`
 
-    auto prepare = [&](int s, bool first) {
-        bool ok_diag = true;
-        if (first) {
-            stage_links<RPL>(c, base, base + nlinks, sc.dn_pos, sc.dn_adj, spos, sadj, skind);
-            if (lane < nrep) srep[lane] = a.ch.rep_dn[rep0 + lane];
-        }
-        const size_t row = static_cast<size_t>(a.s_base + s) * P;
+    // Step-invariant data only: the per-step work of the sweep is in issue
+    // and solve.
+    auto prepare = [&](int, bool first) {
+        if (!first) return;
+        stage_links<RPL>(c, base, base + nlinks, sc.dn_pos, sc.dn_adj, spos, sadj, skind);
+        if (lane < nrep) srep[lane] = a.ch.rep_dn[rep0 + lane];
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
             const int j = r * 64 + lane;
             const int p = p0 + j;
-            if (first) pvalid[j] = sc.perm[p] >= 0;
+            pvalid[j] = sc.perm[p] >= 0;
             if (pvalid[j]) {
-                if (first) {
-                    const int ea = sc.dn_ptr[p], eb = sc.dn_ptr[p + 1];
-                    pe0[j] = static_cast<short>(ea - base);
-                    pe1[j] = static_cast<short>(eb - base);
-                    psub[j] = a.ch.subr[p];
-                    slam[j] = a.st_lam[p];
-                    (r == 0 ? rn0 : rn1) = a.rec[row + p];
-                    (r == 0 ? mn0 : mn1) = a.recm[row + p];
-                    (r == 0 ? dq0 : dq1) = direct(a.s_base + s, p);
+                const int ea = sc.dn_ptr[p], eb = sc.dn_ptr[p + 1];
+                pe0[j] = static_cast<short>(ea - base);
+                pe1[j] = static_cast<short>(eb - base);
+                psub[j] = a.ch.subr[p];
+                slam[j] = a.st_lam[p];
 #pragma unroll
-                    for (int k = 0; k < kFastLinksDn; ++k) {
-                        if (ea + k < eb) {
-                            lkc[k][j] = link_code<RPL>(c, base, ea + k, sc.dn_pos[ea + k]);
-                            lka[k][j] = sc.dn_adj[ea + k];
-                        }
+                for (int k = 0; k < kFastLinksDn; ++k) {
+                    if (ea + k < eb) {
+                        lkc[k][j] = link_code<RPL>(c, base, ea + k, sc.dn_pos[ea + k]);
+                        lka[k][j] = sc.dn_adj[ea + k];
                     }
                 }
-                const float4 rcur = (r == 0) ? rn0 : rn1;
-                prec[j] = rcur;
-                pm[j] = static_cast<unsigned char>((r == 0) ? mn0 : mn1);
-                pdq[j] = (r == 0) ? dq0 : dq1;
-                ok_diag = ok_diag && (1.0f + rcur.x * 0.0f == 1.0f);
-                if (s > 0) {
-                    (r == 0 ? rn0 : rn1) = a.rec[row - P + p];
-                    (r == 0 ? mn0 : mn1) = a.recm[row - P + p];
-                    (r == 0 ? dq0 : dq1) = direct(a.s_base + s - 1, p);
-                }
-            } else if (first) {
+            } else {
                 psub[j] = -1;
             }
         }
-        unit = __all(ok_diag);
     };
 
-    // The sweep's prepare is short, so its polls are issued when checked.
-    auto issue = [&](int) {};
+    // Every lane loads, whether or not it needs the value (an empty position
+    // reads its unused record slot, a lane without a cross-chunk link the
+    // words of position 0): a load under a lane condition merges with the old
+    // value afterwards, and that merge waits for the load at once.
+    auto issue = [&](int s, bool again) {
+        const size_t row = static_cast<size_t>(a.s_base + s) * P;
+        if (!again) {
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                const size_t p = static_cast<size_t>(p0 + r * 64 + lane);
+                load_to_lds(a.rec + row + p, &prec[r * 64], 16);
+                if (a.gqs)
+                    load_to_lds(a.gqs + static_cast<size_t>(a.t0 + a.s_base + s) * P + p,
+                                &pdq[r * 64], 4);
+            }
+        }
+        const uint64_t* mrow = a.amsg + row * 2;
+        const int d = (lane < nlinks && skind[lane] < 0) ? spos[lane] : 0;
+        pwP = poll(mrow + 2 * d);
+        pwG = poll(mrow + 2 * d + 1);
+    };
     auto check = [&](int s) -> bool {
-        const uint64_t* row = a.amsg + static_cast<size_t>(a.s_base + s) * P * 2;
+        const uint64_t* mrow = a.amsg + static_cast<size_t>(a.s_base + s) * P * 2;
         bool ok = true;
-        for (int e = lane; e < nlinks; e += 64) {
-            if (skind[e] < 0) {
-                const int d = spos[e];
-                const uint64_t w0 = poll(row + 2 * d);
-                const uint64_t w1 = poll(row + 2 * d + 1);
-                if (!word_ready(w0, a.epoch) || !word_ready(w1, a.epoch)) {
-                    ok = false;
-                } else {
-                    stP[e] = word_value(w0);
-                    stG[e] = word_value(w1);
+        if (lane < nlinks && skind[lane] < 0) {
+            if (!word_ready(pwP, a.epoch) || !word_ready(pwG, a.epoch)) {
+                ok = false;
+            } else {
+                stP[lane] = word_value(pwP);
+                stG[lane] = word_value(pwG);
+            }
+        }
+        // links beyond the first slot: all loads first, then the checks
+        if (nlinks > 64) {
+            constexpr int kMore = Wd::kStageDn / 64 - 1;
+            uint64_t wP[kMore], wG[kMore];
+#pragma unroll
+            for (int k = 0; k < kMore; ++k) {
+                const int e = lane + 64 * (k + 1);
+                wP[k] = 0;
+                wG[k] = 0;
+                if (e < nlinks && skind[e] < 0) {
+                    const int d = spos[e];
+                    wP[k] = poll(mrow + 2 * d);
+                    wG[k] = poll(mrow + 2 * d + 1);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kMore; ++k) {
+                const int e = lane + 64 * (k + 1);
+                if (e < nlinks && skind[e] < 0) {
+                    if (!word_ready(wP[k], a.epoch) || !word_ready(wG[k], a.epoch)) {
+                        ok = false;
+                    } else {
+                        stP[e] = word_value(wP[k]);
+                        stG[e] = word_value(wG[k]);
+                    }
                 }
             }
         }
@@ -731,9 +801,20 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
         return __all(ok);
     };
 
-    auto solve = [&](int s, bool last) {
+    auto solve = [&](int s, bool last, auto&& before_stores) {
         const int sg = a.s_base + s;
         uint64_t* mrow = a.amsg + static_cast<size_t>(sg) * P * 2;
+        // the step's records arrived in LDS before the inputs that were just
+        // waited for (loads complete in order)
+        {
+            bool ok_diag = true;
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                const int j = r * 64 + lane;
+                if (pvalid[j]) ok_diag = ok_diag && (1.0f + prec[j].x * 0.0f == 1.0f);
+            }
+            unit = __all(ok_diag);
+        }
         // per-lane operands of the solve, read once per step
         int my[RPL], e0[RPL], nl[RPL], code[RPL][kFastLinksDn];
         float gbv[RPL], lamv[RPL], gx[RPL], w[RPL][kFastLinksDn];
@@ -757,7 +838,7 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
             for (int r = 0; r < RPL; ++r) {
                 if (my[r] == l) {
                     const int j = r * 64 + lane;
-                    const float4 rc = prec[j];  // {-c1, c2, dgq/dgb, -}
+                    const float4 rc = prec[j];  // {-c1, c2, dgq/dgb, mask}
                     float sumP = 0.0f, sumG = 0.0f;
                     if (nl[r] <= kFastLinksDn) {
 #pragma unroll
@@ -787,11 +868,13 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
                     ldsP[j] = Pm;
                     ldsG[j] = Gm;
                     gbv[r] = gb;
-                    lamv[r] = pm[j] ? pdq[j] + (gb * rc.z + sumG) : 0.0f;
+                    const float dq = a.gqs ? pdq[j] : 0.0f;
+                    lamv[r] = (rc.w != 0.0f) ? dq + (gb * rc.z + sumG) : 0.0f;
                 }
             }
             __syncthreads();
         }
+        before_stores();
         // published after the solve, as in the forward
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
@@ -868,7 +951,7 @@ __global__ __launch_bounds__(64) void k_tri_fwd(TriArgs a) {
         __syncthreads();
         return all;
     };
-    auto solve = [&](int, bool) {
+    auto solve = [&](int, bool, auto&&) {
         const int p = c * 64 + lane;
         const bool valid = sc.perm[p] >= 0;
         int my = valid ? a.ch.sub[p] : -1;
@@ -892,7 +975,7 @@ __global__ __launch_bounds__(64) void k_tri_fwd(TriArgs a) {
         }
         if (valid) a.x[sc.perm[p]] = a.clamp ? fmaxr(x, a.lb) : x;
     };
-    auto issue = [&](int) {};
+    auto issue = [&](int, bool) {};
     auto probe = [&](int) { return true; };
     chunk_loop<+1>(1, a.err, nullptr, issue, prepare, ready, probe, solve);
 }
@@ -921,7 +1004,7 @@ __global__ __launch_bounds__(64) void k_tri_adj(TriArgs a) {
         __syncthreads();
         return all;
     };
-    auto solve = [&](int, bool) {
+    auto solve = [&](int, bool, auto&&) {
         const int p = c * 64 + lane;
         const bool valid = sc.perm[p] >= 0;
         int my = valid ? a.ch.subr[p] : -1;
@@ -947,7 +1030,7 @@ __global__ __launch_bounds__(64) void k_tri_adj(TriArgs a) {
         }
         if (valid) a.x[sc.perm[p]] = g;
     };
-    auto issue = [&](int) {};
+    auto issue = [&](int, bool) {};
     auto probe = [&](int) { return true; };
     chunk_loop<-1>(1, a.err, nullptr, issue, prepare, ready, probe, solve);
 }

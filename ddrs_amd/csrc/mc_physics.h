@@ -104,6 +104,28 @@ struct PowEval {
     }
 };
 
+// The same evaluations inlined: no call, so the caller's outstanding memory
+// operations are not waited for at each power (a call waits for all of them).
+struct PowEvalInl {
+    PowVals v;
+    __device__ __forceinline__ PowLn p5(float x, float y) {
+        const PowLn r = pow_ln_f32_inl(x, y);
+        v.pw5 = r.pw;
+        v.ln5 = r.ln;
+        return r;
+    }
+    __device__ __forceinline__ PowLn p7(float x, float y) {
+        const PowLn r = pow_ln_f32_inl(x, y);
+        v.pw7 = r.pw;
+        v.ln7 = r.ln;
+        return r;
+    }
+    __device__ __forceinline__ float p15(float x) {
+        v.pw15 = pow_f32_inl(x, 2.0f / 3.0f);
+        return v.pw15;
+    }
+};
+
 // S1..S23 for out[t] = q. ln_ratio and ln_depth are log_f32 of ratio and depth
 // (needed by B6 and B7), taken from the same ln evaluation as the powers.
 template <int PH, class Pow>
