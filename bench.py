@@ -10,8 +10,8 @@ gauge reaches) and the persistent adjoint window, producing dL/dn and
 dL/dq_spatial for every reach. Inputs are resident in HBM before timing.
 
 Also reported: the same step in the plain per-timestep-launch form, the
-roofline of the dominant kernel (the persistent adjoint window) from HIP events
-on its own stream, gradient checks, and the CPU oracle timed on a bounded
+roofline of the dominant kernel (the persistent forward window) from HIP events
+on the stream it runs on, gradient checks, and the CPU oracle timed on a bounded
 sample of the same network.
 """
 from __future__ import annotations
@@ -31,11 +31,14 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes per reach-timestep of each window kernel:
 #   forward window: q' read 4, Q written 4, x history written 8 and read back
 #     8 by the downstream chunks (one link per reach in a tree);
-#   linearise: x history of the step and the previous step with their
-#     upstream links 4 x 8, q' 4, dL/dQ 4; records written 16 + 16 + 1;
-#   sweep: records read 16 + 1, (P, G) published 16 and read 16, gb written 4;
-#   finish: records read 16, gb read 4, dL/dq' written 4.
-KERNEL_BYTES = {"fwd": 24, "lin": 73, "sweep": 53, "fin": 24}
+#   linearise: x history of the reach and of its upstream link 2 x 8 (the
+#     previous step's words are the ones read one step earlier), q' 4;
+#     records written 16 + 16;
+#   sweep: records read 16, direct dL/dQ read 4, (P, G) published 16 and read
+#     16 by the upstream chunk, gb written 4;
+#   finish: records read 16, gb read 4 (this bench's loss asks for no dL/dq',
+#     which would add 4 written).
+KERNEL_BYTES = {"fwd": 24, "lin": 52, "sweep": 56, "fin": 20}
 KERNEL_NAMES = {"fwd": "k_fwd_window (persistent forward)", "lin": "k_adj_lin (linearise)",
                 "sweep": "k_adj_sweep (persistent reverse sweep)", "fin": "k_adj_fin"}
 CONFIGS = {
@@ -56,7 +59,8 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=48)
+    ap.add_argument("--cpu-steps", type=int, default=240,
+                    help="timesteps of the single-thread oracle sample (about 12 s on C2)")
     ap.add_argument("--sublevels", type=int, default=0,
                     help="largest chain of links in one schedule block (0: library default)")
     ap.add_argument("--block", type=int, default=0, help="schedule block, 64 or 128 (0: default)")
