@@ -268,6 +268,58 @@ ddrs_status ddrs_compress(int32_t n_edges_in, const int32_t* h_rows_in,
                           int32_t* n_edges, int32_t* h_gauge);
 
 /* ---------------------------------------------------------------------------
+ * KAN parameter head (device, f32): per-reach attributes -> routing parameters.
+ * Replaces KanHead::forward (src/nn/kan_head.rs:215-246: input Linear, the
+ * KanLayer blocks of rskan v0.1.3 / pykan, output Linear, sigmoid, [P, N]) and
+ * its autodiff backward. All pointers are device pointers; weight matrices are
+ * [out][in] row-major (the fixture's layout, src/nn/kan_head.rs:299-360).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    int32_t F; /* input attributes */
+    int32_t H; /* hidden width (<= 64) */
+    int32_t P; /* output parameters (<= 16) */
+    int32_t L; /* KanLayer(H, H) blocks (<= 8) */
+    int32_t G; /* knots per input: grid + 2k + 1 */
+    int32_t k; /* spline order (<= 5); coefficients per edge C = G - 1 - k */
+} ddrs_kan_dims;
+
+typedef struct {
+    const float* input_weight;  /* [H][F] */
+    const float* input_bias;    /* [H] */
+    const float* grid;          /* [L][H][G] knots, ascending per input */
+    const float* coef;          /* [L][H in][H out][C] */
+    const float* scale_base;    /* [L][H in][H out] */
+    const float* scale_sp;      /* [L][H in][H out] */
+    const float* mask;          /* [L][H in][H out] */
+    const float* output_weight; /* [P][H] */
+    const float* output_bias;   /* [P] */
+} ddrs_kan_params;
+
+/* Gradients of the trained parameters (grid and mask are not trained), same
+ * layouts; overwritten by ddrs_kan_head_bwd. */
+typedef struct {
+    float* input_weight;
+    float* input_bias;
+    float* coef;
+    float* scale_base;
+    float* scale_sp;
+    float* output_weight;
+    float* output_bias;
+} ddrs_kan_grads;
+
+/* d_out[p][r] = sigmoid(head(d_inputs[r][:]))[p] for r < n (reference
+ * src/nn/kan_head.rs:215). */
+ddrs_status ddrs_kan_head_fwd(const ddrs_kan_dims* dims, const ddrs_kan_params* params,
+                              int32_t n, const float* d_inputs, float* d_out, void* stream);
+/* Reverse mode of ddrs_kan_head_fwd for d_grad_out [P][n]: every parameter
+ * gradient into `grads`, and dL/d inputs into d_grad_inputs [n][F] unless it is
+ * NULL (burn's autodiff of the same graph in the reference). Gradient sums use
+ * float atomics: their order, and so the last bits, vary between calls. */
+ddrs_status ddrs_kan_head_bwd(const ddrs_kan_dims* dims, const ddrs_kan_params* params,
+                              int32_t n, const float* d_inputs, const float* d_grad_out,
+                              const ddrs_kan_grads* grads, float* d_grad_inputs, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Synthetic workloads (host), seeded and portable.
  * ------------------------------------------------------------------------- */
 /* Reach i < n-1 drains to a reach drawn uniformly from (i, min(i+window, n-1)];

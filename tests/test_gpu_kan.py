@@ -1,0 +1,218 @@
+"""GPU parity of the KAN parameter head (ddrs_kan_head_fwd / ddrs_kan_head_bwd
+through the C ABI, via ddrs_amd.kan) against the reference fixture and the
+CPU oracle. Needs an MI355X.
+
+Tolerances:
+* reference fixture (H=21, two blocks, grid 50, k=2, N=64): outputs within
+  1e-6 and the gradients of sum(all outputs) within 1e-5 absolute, the bars of
+  the reference's tests/kan_head_fixture_forward.rs:69 and
+  tests/kan_head_fixture_backward.rs:77;
+* random heads at larger N, against the FP64 oracle: outputs within
+  max(1e-6, 3x the f32 oracle's own error); each gradient tensor within
+  max(3x the f32 oracle's error, 2e-5 max|g|) in max-abs. The device sums
+  gradients with float atomics in no fixed order, so its rounding differs from
+  the oracle's pairwise sums;
+* at the workload's size (N = 350000 reaches), size-independent properties:
+  rows are independent (a permutation of the inputs permutes the outputs
+  bit-exactly), and the parameter gradient of a batch is the sum of its
+  halves' (within 2e-5 max|g|).
+"""
+import ctypes
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import kan_oracle as ko
+
+pytestmark = pytest.mark.gpu
+
+if not torch.cuda.is_available():  # collected on the CPU too, skipped there
+    pytest.skip("needs a GPU", allow_module_level=True)
+
+from ddrs_amd import _native as NAT  # noqa: E402
+from ddrs_amd import kan as K  # noqa: E402
+
+DEV = "cuda:0"
+FIXTURE = os.path.join(os.path.dirname(__file__), "golden", "kan_head_seed42.npz")
+NAMES = ["n", "q_spatial", "p_spatial"]
+BLOCK_GRADS = ("coef", "scale_base", "scale_sp")
+DENSE_GRADS = ("input_weight", "input_bias", "output_weight", "output_bias")
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _fixture():
+    with np.load(FIXTURE, allow_pickle=False) as d:
+        return {k: d[k] for k in d.files}
+
+
+def _head_from_oracle_params(p, names, k):
+    t = {n: torch.from_numpy(np.asarray(p[n], np.float64)) for n in DENSE_GRADS}
+    for n in ("grid", "coef", "scale_base", "scale_sp", "mask"):
+        t[n] = torch.from_numpy(np.stack([np.asarray(b[n], np.float64) for b in p["blocks"]]))
+    return K.KanHead(names, k, t, DEV)
+
+
+def _grads(head):
+    return {n: getattr(head, n).grad.detach().cpu().numpy().astype(np.float64)
+            for n in K._TRAINED}
+
+
+def _run(head, x_np, w_np):
+    """Outputs [P, N], parameter gradients and dL/dx for L = sum(w * out)."""
+    head.zero_grad(set_to_none=True)
+    x = torch.tensor(x_np, dtype=torch.float32, device=DEV, requires_grad=True)
+    out = head.forward_packed(x)
+    (out * torch.tensor(w_np, dtype=torch.float32, device=DEV)).sum().backward()
+    torch.cuda.synchronize()
+    return (out.detach().cpu().numpy().astype(np.float64), _grads(head),
+            x.grad.detach().cpu().numpy().astype(np.float64))
+
+
+def _oracle_grads_flat(g):
+    flat = {n: g[n] for n in DENSE_GRADS}
+    for n in BLOCK_GRADS:
+        flat[n] = np.stack([b[n] for b in g["blocks"]])
+    return flat
+
+
+def test_fixture_forward_and_backward():
+    say("kan fixture H=21 L=2 grid=50 k=2 N=64")
+    d = _fixture()
+    head = K.KanHead.from_npz(FIXTURE, NAMES, k=2, device=DEV)
+    x = torch.tensor(d["inputs"], device=DEV)
+    out = head(x)
+    assert list(out) == NAMES
+    for name in NAMES:
+        got = out[name].detach().cpu().numpy()
+        np.testing.assert_allclose(got, d[f"expected_{name}"], rtol=0, atol=1e-6, err_msg=name)
+    sum(v.sum() for v in out.values()).backward()
+    torch.cuda.synchronize()
+    g = _grads(head)
+    for n in DENSE_GRADS:
+        np.testing.assert_allclose(g[n], d[f"grad_{n}"], rtol=0, atol=1e-5, err_msg=n)
+    for l in range(2):
+        for n in BLOCK_GRADS:
+            np.testing.assert_allclose(g[n][l], d[f"grad_block_{l}_{n}"], rtol=0, atol=1e-5,
+                                       err_msg=f"block {l} {n}")
+
+
+def _random_case(seed, F, H, P, L, num, k, N, spread=1.5):
+    rng = np.random.default_rng(seed)
+    G = num + 2 * k + 1
+    h = 2.0 / num
+    knots = np.arange(-k, num + k + 1) * h - 1.0
+    p = dict(input_weight=rng.normal(0, 0.6, (H, F)), input_bias=rng.normal(0, 0.1, H),
+             output_weight=rng.normal(0, 0.5, (P, H)), output_bias=rng.normal(0, 0.1, P),
+             blocks=[])
+    for _ in range(L):
+        p["blocks"].append(dict(grid=np.tile(knots, (H, 1)),
+                                coef=rng.normal(0, 0.3, (H, H, G - 1 - k)),
+                                scale_base=rng.uniform(-0.3, 0.3, (H, H)),
+                                scale_sp=rng.uniform(0.1, 0.4, (H, H)),
+                                mask=(rng.uniform(size=(H, H)) > 0.1).astype(float)))
+    # f32-representable parameters, so the oracle and the device see the same values
+    p = {kk: (np.asarray(v, np.float32).astype(np.float64) if kk != "blocks" else
+              [{a: np.asarray(b, np.float32).astype(np.float64) for a, b in blk.items()}
+               for blk in v]) for kk, v in p.items()}
+    x = rng.normal(0, spread, (N, F)).astype(np.float32).astype(np.float64)
+    w = rng.normal(0, 1, (N, P)).astype(np.float32).astype(np.float64)
+    return p, x, w
+
+
+def _check_against_oracle(p, x, w, k, label):
+    names = [f"p{j}" for j in range(p["output_weight"].shape[0])]
+    head = _head_from_oracle_params(p, names, k)
+    out, g, gx = _run(head, x, w.T.copy())
+    o64, c64 = ko.kan_head_fwd(x, p, k, np.float64)
+    o32, c32 = ko.kan_head_fwd(x, p, k, np.float32)
+    e32 = np.abs(o32 - o64).max()
+    eg = np.abs(out.T - o64).max()
+    say(label, f"out err {eg:.2e} (f32 oracle {e32:.2e})")
+    assert eg <= max(1e-6, 3 * e32), label
+    g64 = ko.kan_head_bwd(w, p, c64, np.float64)
+    g32 = ko.kan_head_bwd(w, p, c32, np.float32)
+    f64, f32 = _oracle_grads_flat(g64), _oracle_grads_flat(g32)
+    f64["inputs"], f32["inputs"], g["inputs"] = g64["inputs"], g32["inputs"], gx
+    for n, ref in f64.items():
+        scale = np.abs(ref).max()
+        err = np.abs(g[n] - ref).max()
+        err32 = np.abs(f32[n] - ref).max()
+        say(label, f"{n}: err {err:.2e}, f32 oracle {err32:.2e}, max|g| {scale:.2e}")
+        assert err <= max(3 * err32, 2e-5 * scale), f"{label} {n}"
+
+
+@pytest.mark.parametrize("case", [
+    # (seed, F, H, P, L, grid, k, N): the fixture's shape at a larger batch (the
+    # coefficient gradients exceed LDS and go to global memory), grid 5 / k 3
+    # (they fit in LDS), ragged and single-row batches, one block, a wider head
+    (1, 10, 21, 3, 2, 50, 2, 4096),
+    (2, 10, 21, 3, 2, 5, 3, 3000),
+    (3, 10, 21, 3, 2, 50, 2, 1),
+    (4, 10, 21, 3, 2, 5, 3, 130),
+    (5, 7, 16, 2, 1, 8, 1, 777),
+    (6, 16, 32, 8, 2, 5, 3, 1500),
+])
+def test_random_heads_match_oracle(case):
+    seed, F, H, P, L, num, k, N = case
+    p, x, w = _random_case(seed, F, H, P, L, num, k, N)
+    _check_against_oracle(p, x, w, k, f"kan random {case}")
+
+
+def test_inputs_outside_the_grid():
+    # hidden activations far outside [-1, 1]: the splines vanish, silu remains
+    p, x, w = _random_case(11, 10, 21, 3, 2, 50, 2, 512, spread=40.0)
+    _check_against_oracle(p, x, w, 2, "kan outside grid")
+
+
+def test_empty_batch_and_errors():
+    say("kan empty batch and argument errors")
+    p, _, _ = _random_case(12, 10, 21, 3, 2, 5, 3, 1)
+    head = _head_from_oracle_params(p, ["a", "b", "c"], 3)
+    x = torch.zeros(0, 10, device=DEV, requires_grad=True)
+    out = head.forward_packed(x)
+    assert tuple(out.shape) == (3, 0)
+    out.sum().backward()
+    torch.cuda.synchronize()
+    for n, gv in _grads(head).items():
+        assert not gv.any(), n
+    with pytest.raises(ValueError):
+        head.forward_packed(torch.zeros(4, 9, device=DEV))
+    dims = K.KanDims(10, 65, 3, 2, 12, 3)
+    prm = K.KanParams()
+    st = NAT.lib.ddrs_kan_head_fwd(ctypes.byref(dims), ctypes.byref(prm), 4, None, None, None)
+    assert st == 3  # DDRS_ERR_CAPACITY
+    assert b"H <= 64" in NAT.lib.ddrs_last_error()
+    dims = K.KanDims(10, 21, 3, 2, 3, 3)       # G <= k + 1: no basis functions
+    st = NAT.lib.ddrs_kan_head_fwd(ctypes.byref(dims), ctypes.byref(prm), 4, None, None, None)
+    assert st == 1
+
+
+def test_workload_size_properties():
+    """N = 350000 reaches, the fixture's head shape (grid 50, k=2)."""
+    N = 350_000
+    say(f"kan properties N={N}")
+    p, x, w = _random_case(21, 10, 21, 3, 2, 50, 2, N)
+    head = _head_from_oracle_params(p, ["a", "b", "c"], 2)
+    xt = torch.tensor(x, dtype=torch.float32, device=DEV)
+    perm = torch.randperm(N, generator=torch.Generator().manual_seed(0)).to(DEV)
+    with torch.no_grad():
+        out = head.forward_packed(xt)
+        out_p = head.forward_packed(xt[perm].contiguous())
+    assert torch.equal(out[:, perm], out_p)
+    _, g_all, gx_all = _run(head, x, w.T.copy())
+    h = N // 2
+    _, g_a, gx_a = _run(head, x[:h], w[:h].T.copy())
+    _, g_b, gx_b = _run(head, x[h:], w[h:].T.copy())
+    for n in K._TRAINED:
+        scale = np.abs(g_all[n]).max()
+        err = np.abs(g_all[n] - (g_a[n] + g_b[n])).max()
+        say(f"{n}: split err {err:.2e} max|g| {scale:.2e}")
+        assert err <= 2e-5 * scale, n
+    np.testing.assert_array_equal(gx_all, np.concatenate([gx_a, gx_b]))
