@@ -74,8 +74,10 @@ _SIGS = {
     "ddrs_gen_attributes": (c_int, [c_int, c_u64, P, P, P, P]),
     "ddrs_gen_forcing": (c_int, [c_int, c_int, c_u64, c_int, P]),
     "ddrs_gen_forcing_device": (c_int, [c_int, c_int, c_u64, c_int, P, P]),
-    "ddrs_kan_head_fwd": (c_int, [P, P, c_int, P, P, P]),
-    "ddrs_kan_head_bwd": (c_int, [P, P, c_int, P, P, P, P, P]),
+    "ddrs_kan_workspace_bytes": (c_int, [P, c_int, ctypes.POINTER(c_size),
+                                         ctypes.POINTER(c_size)]),
+    "ddrs_kan_head_fwd": (c_int, [P, P, c_int, P, P, P, c_size, P]),
+    "ddrs_kan_head_bwd": (c_int, [P, P, c_int, P, P, P, P, P, c_size, P]),
 }
 
 EXPORTED = tuple(_SIGS)

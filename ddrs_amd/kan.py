@@ -197,6 +197,13 @@ def _params(dims: KanDims, grid, mask, t: Dict[str, torch.Tensor]) -> KanParams:
                      _ptr(t["output_weight"]), _ptr(t["output_bias"]))
 
 
+def _workspace(dims: KanDims, n: int, device, backward: bool) -> torch.Tensor:
+    fwd, bwd = N.c_size(), N.c_size()
+    N.check(N.lib.ddrs_kan_workspace_bytes(ctypes.byref(dims), n, ctypes.byref(fwd),
+                                           ctypes.byref(bwd)), "ddrs_kan_workspace_bytes")
+    return torch.empty((bwd if backward else fwd).value, dtype=torch.uint8, device=device)
+
+
 class _KanHeadFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, inputs, dims, grid, mask, *trained):
@@ -209,8 +216,10 @@ class _KanHeadFn(torch.autograd.Function):
         n = x.shape[0]
         out = torch.empty(dims.P, n, device=x.device, dtype=torch.float32)
         prm = _params(dims, grid, mask, t)
+        ws = _workspace(dims, n, x.device, backward=False)
         N.check(N.lib.ddrs_kan_head_fwd(ctypes.byref(dims), ctypes.byref(prm), n, _ptr(x),
-                                        _ptr(out), _stream()), "ddrs_kan_head_fwd")
+                                        _ptr(out), _ptr(ws), ws.numel(), _stream()),
+                "ddrs_kan_head_fwd")
         ctx.dims = dims
         ctx.save_for_backward(x, grid, mask, *[t[n] for n in _TRAINED])
         ctx.in_dtype = inputs.dtype
@@ -226,9 +235,10 @@ class _KanHeadFn(torch.autograd.Function):
         gx = torch.empty_like(x) if ctx.needs_input_grad[0] else None
         prm = _params(dims, grid, mask, t)
         gr = KanGrads(*[_ptr(grads[n]) for n in _TRAINED])
+        ws = _workspace(dims, x.shape[0], x.device, backward=True)
         N.check(N.lib.ddrs_kan_head_bwd(ctypes.byref(dims), ctypes.byref(prm), x.shape[0],
-                                        _ptr(x), _ptr(g), ctypes.byref(gr), _ptr(gx),
-                                        _stream()), "ddrs_kan_head_bwd")
+                                        _ptr(x), _ptr(g), ctypes.byref(gr), _ptr(gx), _ptr(ws),
+                                        ws.numel(), _stream()), "ddrs_kan_head_bwd")
         if gx is not None:
             gx = gx.to(ctx.in_dtype)
         return (gx, None, None, None, *[grads[n] for n in _TRAINED])

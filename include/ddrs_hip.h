@@ -275,12 +275,12 @@ ddrs_status ddrs_compress(int32_t n_edges_in, const int32_t* h_rows_in,
  * [out][in] row-major (the fixture's layout, src/nn/kan_head.rs:299-360).
  * ------------------------------------------------------------------------- */
 typedef struct {
-    int32_t F; /* input attributes */
-    int32_t H; /* hidden width (<= 64) */
+    int32_t F; /* input attributes (<= 63) */
+    int32_t H; /* hidden width (<= 32) */
     int32_t P; /* output parameters (<= 16) */
     int32_t L; /* KanLayer(H, H) blocks (<= 8) */
     int32_t G; /* knots per input: grid + 2k + 1 */
-    int32_t k; /* spline order (<= 5); coefficients per edge C = G - 1 - k */
+    int32_t k; /* spline order (<= 5); coefficients per edge C = G - 1 - k <= 63 */
 } ddrs_kan_dims;
 
 typedef struct {
@@ -307,17 +307,24 @@ typedef struct {
     float* output_bias;
 } ddrs_kan_grads;
 
+/* Workspace the calls below need for n reaches (device memory the caller
+ * allocates; its contents need not persist between calls). No reference
+ * counterpart (burn allocates inside its ops). */
+ddrs_status ddrs_kan_workspace_bytes(const ddrs_kan_dims* dims, int32_t n, size_t* fwd_bytes,
+                                     size_t* bwd_bytes);
 /* d_out[p][r] = sigmoid(head(d_inputs[r][:]))[p] for r < n (reference
- * src/nn/kan_head.rs:215). */
+ * KanHead::forward, src/nn/kan_head.rs:215). */
 ddrs_status ddrs_kan_head_fwd(const ddrs_kan_dims* dims, const ddrs_kan_params* params,
-                              int32_t n, const float* d_inputs, float* d_out, void* stream);
+                              int32_t n, const float* d_inputs, float* d_out, void* d_workspace,
+                              size_t workspace_bytes, void* stream);
 /* Reverse mode of ddrs_kan_head_fwd for d_grad_out [P][n]: every parameter
  * gradient into `grads`, and dL/d inputs into d_grad_inputs [n][F] unless it is
- * NULL (burn's autodiff of the same graph in the reference). Gradient sums use
- * float atomics: their order, and so the last bits, vary between calls. */
+ * NULL (burn's autodiff of the same graph in the reference). Deterministic:
+ * the sums over reaches run in a fixed order. */
 ddrs_status ddrs_kan_head_bwd(const ddrs_kan_dims* dims, const ddrs_kan_params* params,
                               int32_t n, const float* d_inputs, const float* d_grad_out,
-                              const ddrs_kan_grads* grads, float* d_grad_inputs, void* stream);
+                              const ddrs_kan_grads* grads, float* d_grad_inputs,
+                              void* d_workspace, size_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Synthetic workloads (host), seeded and portable.
