@@ -13,6 +13,12 @@ Also reported: the same step in the plain per-timestep-launch form, the
 roofline of the dominant kernel (the persistent forward window) from HIP events
 on the stream it runs on, gradient checks, and the CPU oracle timed on a bounded
 sample of the same network.
+
+"kan_head" reports the KAN parameter head (the producer of n and q_spatial in
+the batched and training workloads): 10 attributes ~ N(0, 1) -> 21 hidden ->
+two KAN blocks (grid 5, order 3) -> 2 sigmoid outputs, forward + backward over
+350,000 reaches, against the fp32 peak and beside its CPU oracle.
+--kan-only runs just that part.
 """
 from __future__ import annotations
 
@@ -28,6 +34,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+# KAN head of the batched and training workloads, evaluated per reach
+KAN_SHAPE = dict(F=10, H=21, P=2, L=2, grid=5, k=3, N=350_000)
 # Algorithmic HBM bytes per reach-timestep of each window kernel:
 #   forward window: q' read 4, Q written 4, x history written 8 and read back
 #     8 by the downstream chunks (one link per reach in a tree);
@@ -64,6 +73,8 @@ def main():
     ap.add_argument("--sublevels", type=int, default=0,
                     help="largest chain of links in one schedule block (0: library default)")
     ap.add_argument("--block", type=int, default=0, help="schedule block, 64 or 128 (0: default)")
+    ap.add_argument("--kan-only", action="store_true", help="only the KAN head measurement")
+    ap.add_argument("--no-kan", action="store_true", help="skip the KAN head measurement")
     ap.add_argument("--pmc-calibration", action="store_true",
                     help="first copy 1 GiB device to device once, as the known-size reference "
                          "for FETCH_SIZE / WRITE_SIZE counters")
@@ -78,6 +89,11 @@ def main():
     from ddrs_amd import _native as NAT
     from ddrs_amd import network as NW
     from ddrs_amd.sparse import Network, SparseAdjacency
+
+    if args.kan_only:
+        kan = kan_bench(torch, args.steps, args.warmup, not args.no_cpu)
+        print(json.dumps({"metric": "reaches/s KAN head forward+backward", **kan}), flush=True)
+        return
 
     if args.pmc_calibration:
         a = torch.empty(2**28, dtype=torch.float32, device="cuda:0")
@@ -238,6 +254,8 @@ def main():
         "grad_check": grad_check(torch, D, NW, Network, SparseAdjacency),
         "cpu_baseline": cpu,
     }
+    if not args.no_kan:
+        out["kan_head"] = kan_bench(torch, args.steps, args.warmup, not args.no_cpu)
     print(json.dumps(out), flush=True)
 
 
@@ -280,6 +298,124 @@ def grad_check(torch, D, NW, Network, SparseAdjacency):
             worst = max(worst, abs(g[key][i] - fd) / max(abs(fd), 1e-12))
     return {"max_rel_err_vs_fp64_fd": worst, "reaches": 8, "params": ["n", "q_spatial"],
             "network": "2000-reach tree, 24 steps"}
+
+
+def kan_flops(F, H, P, L, k):
+    """Algorithmic fp32 flops per reach (a fused multiply-add counts 2) of the
+    KAN head: (forward, forward + backward). The backward kernels recompute the
+    forward; that recomputation is not counted."""
+    basis = 3 * k * (k + 1)            # k(k+1)/2 Cox-de Boor terms of about 6 flops
+    edge = 2 * (k + 1) + 4             # spline dot, scale_base*silu + scale_sp*spline, sum
+    fwd = 2 * F * H + L * (H * (basis + 4) + H * H * edge) + 2 * H * P + 4 * P
+    # reverse: sigmoid and output layer, per KAN layer the basis derivatives and
+    # dy/dx (spline derivative dot and the two scale terms), the input layer
+    rev = 2 * H * P + 4 * P + L * (H * (2 * basis + 6) + H * H * (2 * (k + 1) + 6)) + 2 * H * F
+    # parameter sums over reaches: per layer and input, the basis again and
+    # (k+1 basis values + silu) x H output gradients; the two linear layers
+    sums = L * H * (basis + 4 + 2 * (k + 2) * H) + 2 * (H + 1) * P + 2 * (F + 1) * H
+    return fwd, fwd + rev + sums
+
+
+def kan_bench(torch, steps, warmup, with_cpu):
+    """Forward and backward of the KAN head through the C ABI, HIP events on
+    the stream they run on; the CPU oracle on a bounded sample."""
+    import ctypes
+    from ddrs_amd import _native as NAT
+    from ddrs_amd import kan as K
+    s = KAN_SHAPE
+    F, H, P, L, k, N = s["F"], s["H"], s["P"], s["L"], s["k"], s["N"]
+    dev = torch.device("cuda:0")
+    head = K.KanHeadConfig([f"attr{i}" for i in range(F)], ["n", "q_spatial"], 42, H, L,
+                           s["grid"], k).init(dev)
+    gen = torch.Generator(device=dev).manual_seed(7)
+    x = torch.randn(N, F, generator=gen, device=dev)
+    gout = torch.randn(P, N, generator=gen, device=dev)
+    out = torch.empty(P, N, device=dev)
+    gx = torch.empty_like(x)
+    grads = {n: torch.empty_like(getattr(head, n)) for n in K._TRAINED}
+    fb, bb = NAT.c_size(), NAT.c_size()
+    NAT.check(NAT.lib.ddrs_kan_workspace_bytes(ctypes.byref(head.dims), N, ctypes.byref(fb),
+                                               ctypes.byref(bb)))
+    ws = torch.empty(bb.value, dtype=torch.uint8, device=dev)
+    prm = K._params(head.dims, head.grid, head.mask,
+                    {n: getattr(head, n).detach() for n in K._TRAINED})
+    gr = K.KanGrads(*[grads[n].data_ptr() for n in K._TRAINED])
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+
+    def fwd():
+        NAT.check(NAT.lib.ddrs_kan_head_fwd(ctypes.byref(head.dims), ctypes.byref(prm), N,
+                                            x.data_ptr(), out.data_ptr(), ws.data_ptr(),
+                                            ws.numel(), sp), "ddrs_kan_head_fwd")
+
+    def bwd():
+        NAT.check(NAT.lib.ddrs_kan_head_bwd(ctypes.byref(head.dims), ctypes.byref(prm), N,
+                                            x.data_ptr(), gout.data_ptr(), ctypes.byref(gr),
+                                            gx.data_ptr(), ws.data_ptr(), ws.numel(), sp),
+                  "ddrs_kan_head_bwd")
+
+    for _ in range(max(warmup, 1)):
+        fwd()
+        bwd()
+    torch.cuda.synchronize()
+    reps = max(steps, 1) * 4
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3 * reps)]
+    for r in range(reps):
+        ev[3 * r].record(stream)
+        fwd()
+        ev[3 * r + 1].record(stream)
+        bwd()
+        ev[3 * r + 2].record(stream)
+    torch.cuda.synchronize()
+    fwd_ms = sum(ev[3 * r].elapsed_time(ev[3 * r + 1]) for r in range(reps)) / reps
+    bwd_ms = sum(ev[3 * r + 1].elapsed_time(ev[3 * r + 2]) for r in range(reps)) / reps
+    ff, fb_flops = kan_flops(F, H, P, L, k)
+    tflops = fb_flops * N / ((fwd_ms + bwd_ms) * 1e-3) / 1e12
+    log(f"[bench] kan head N={N}: fwd {fwd_ms:.3f} ms, bwd {bwd_ms:.3f} ms, {tflops:.1f} TFLOP/s")
+    res = {
+        "value": N / ((fwd_ms + bwd_ms) * 1e-3), "unit": "reaches/s",
+        "fwd_ms": fwd_ms, "bwd_ms": bwd_ms, "calls_timed": reps, "dtype": "f32",
+        "config": {"workload": f"KAN head {F} attributes -> {H} -> {L} KAN blocks (grid "
+                               f"{s['grid']}, order {k}) -> {P} sigmoid outputs, forward + "
+                               f"backward over {N} reaches", **s},
+        "roofline": {"bound": "valu", "achieved": tflops, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": tflops / FP32_PEAK_TFLOPS, "traffic": None,
+                     "flops_per_reach_fwd": ff, "flops_per_reach_fwd_bwd": fb_flops,
+                     "fwd_TFLOPs": ff * N / (fwd_ms * 1e-3) / 1e12,
+                     "note": "fp32 vector arithmetic, no packed math; the peak counts "
+                             "packed/dual-issue fp32"},
+    }
+    if with_cpu:
+        res["cpu_baseline"] = kan_cpu_baseline(F, H, P, L, s["grid"], k)
+    return res
+
+
+def kan_cpu_baseline(F, H, P, L, grid, k, n_sample=40_000):
+    """The numpy oracle (f32, one BLAS thread) on a sample of reaches."""
+    from threadpoolctl import threadpool_limits
+    from oracle import kan_oracle as ko
+    rng = np.random.default_rng(11)
+    G = grid + 2 * k + 1
+    knots = (np.arange(-k, grid + k + 1) * (2.0 / grid) - 1.0).astype(np.float32)
+    p = dict(input_weight=rng.normal(0, 0.45, (H, F)).astype(np.float32),
+             input_bias=np.zeros(H, np.float32),
+             output_weight=rng.normal(0, 0.03, (P, H)).astype(np.float32),
+             output_bias=np.zeros(P, np.float32),
+             blocks=[dict(grid=np.tile(knots, (H, 1)),
+                          coef=rng.normal(0, 0.01, (H, H, G - 1 - k)).astype(np.float32),
+                          scale_base=rng.uniform(-0.2, 0.2, (H, H)).astype(np.float32),
+                          scale_sp=np.full((H, H), 1 / np.sqrt(H), np.float32),
+                          mask=np.ones((H, H), np.float32)) for _ in range(L)])
+    x = rng.normal(size=(n_sample, F)).astype(np.float32)
+    g = rng.normal(size=(n_sample, P)).astype(np.float32)
+    with threadpool_limits(1):
+        t0 = time.perf_counter()
+        out, cache = ko.kan_head_fwd(x, p, k, np.float32)
+        ko.kan_head_bwd(g, p, cache, np.float32)
+        dt = time.perf_counter() - t0
+    return {"value": n_sample / dt, "unit": "reaches/s", "cores": 1, "kind": "port",
+            "sample": f"oracle/kan_oracle.py forward + backward in f32 over {n_sample} "
+                      f"reaches of the same head shape, one thread ({dt:.1f} s)"}
 
 
 def cpu_baseline(rows, cols, at, R, steps, ci, storms, net, params):
