@@ -78,6 +78,17 @@ _SIGS = {
                                          ctypes.POINTER(c_size)]),
     "ddrs_kan_head_fwd": (c_int, [P, P, c_int, P, P, P, c_size, P]),
     "ddrs_kan_head_bwd": (c_int, [P, P, c_int, P, P, P, P, P, c_size, P]),
+    "ddrs_gauge_plan_create": (c_int, [c_int, c_int, c_int, P, P, c_int, ctypes.POINTER(P)]),
+    "ddrs_gauge_plan_destroy": (None, [P]),
+    "ddrs_scatter_add_by_group": (c_int, [P, P, c_int, ctypes.c_int64, ctypes.c_int64, P, P]),
+    "ddrs_scatter_add_by_group_bwd": (c_int, [P, P, c_int, ctypes.c_int64, ctypes.c_int64, P,
+                                              P]),
+    "ddrs_tau_trim_downsample": (c_int, [P, c_int, c_int, c_int, P, P]),
+    "ddrs_tau_trim_downsample_bwd": (c_int, [P, c_int, c_int, c_int, P, P]),
+    "ddrs_nse_batch_loss": (c_int, [P, P, P, P, c_int, c_int, c_float, P, P]),
+    "ddrs_nse_batch_loss_bwd": (c_int, [P, P, P, c_int, c_int, c_float, P, P, P]),
+    "ddrs_gauge_nse_loss": (c_int, [P, P, c_int, ctypes.c_int64, ctypes.c_int64, c_int, c_int,
+                                    P, P, c_float, P, P, P, P, P]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -8,7 +8,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libddrs_hip.so")
-SOURCES = ["ddrs_kernels.hip", "ddrs_host.hip", "ddrs_index.hip", "ddrs_gen.hip", "ddrs_kan.hip"]
+SOURCES = ["ddrs_kernels.hip", "ddrs_host.hip", "ddrs_index.hip", "ddrs_gen.hip", "ddrs_kan.hip",
+           "ddrs_loss.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -327,6 +327,63 @@ ddrs_status ddrs_kan_head_bwd(const ddrs_kan_dims* dims, const ddrs_kan_params* 
                               void* d_workspace, size_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Gauge readout and training loss (device, f32, deterministic).
+ * Replaces scatter_add_by_group (src/training/forward.rs:18-34),
+ * tau_trim_and_downsample (src/training/loss.rs:33-95), nse_batch_loss
+ * (src/training/loss.rs:226-237) and the micro-batch objective of
+ * src/training/driver.rs:110-201 (warmup slice, NaN-gauge filter). A discharge
+ * series is addressed by strides: element (reach n, hour t) at
+ * d_runoff[n * stride_n + t * stride_t]; the reference's (N, T) is
+ * (stride_n, stride_t) = (T, 1), this library's time-major [T][N] is (1, N).
+ * ------------------------------------------------------------------------- */
+typedef struct ddrs_gauge_plan* ddrs_gauge_plan_t;
+/* The outflow map of a batch: entry k adds reach h_flat_indices[k] into gauge
+ * h_group_ids[k] (RoutingTensors::flat_indices / group_ids, data/dataset.rs:
+ * 139-141, built from outflow_idx at :160-230). Host arrays, any order; sums
+ * run in increasing k as the reference's scatter does. Scratch for series of
+ * up to max_t_hours hours is allocated here, none in the calls below. At most
+ * 65535 gauges and 65535 distinct outflow reaches. */
+ddrs_status ddrs_gauge_plan_create(int32_t n_reaches, int32_t num_gauges, int32_t K,
+                                   const int32_t* h_flat_indices, const int32_t* h_group_ids,
+                                   int32_t max_t_hours, ddrs_gauge_plan_t* out);
+void ddrs_gauge_plan_destroy(ddrs_gauge_plan_t plan);
+/* d_out[g][t] = sum over entries k of gauge g of runoff(flat_k, t), t < T. */
+ddrs_status ddrs_scatter_add_by_group(ddrs_gauge_plan_t plan, const float* d_runoff, int32_t T,
+                                      int64_t stride_n, int64_t stride_t, float* d_out,
+                                      void* stream);
+/* Backward: grunoff(n, t) = sum over entries k with flat_k = n of d_gout[group_k][t], written
+ * for every reach n of the map (the scatter's select backward); no other entry is written. */
+ddrs_status ddrs_scatter_add_by_group_bwd(ddrs_gauge_plan_t plan, const float* d_gout,
+                                          int32_t T, int64_t stride_n, int64_t stride_t,
+                                          float* d_grunoff, void* stream);
+/* d_out [G][T_days] = hours [tau, T_hours - (24 - tau)) of d_in [G][T_hours] pooled with the
+ * area weights of area_pool_weights; T_days = (T_hours - 24) / 24; 0 <= tau < 24. */
+ddrs_status ddrs_tau_trim_downsample(const float* d_in, int32_t G, int32_t T_hours, int32_t tau,
+                                     float* d_out, void* stream);
+/* Backward into every entry of d_gin [G][T_hours] (zero outside the window). */
+ddrs_status ddrs_tau_trim_downsample_bwd(const float* d_gout, int32_t G, int32_t T_hours,
+                                         int32_t tau, float* d_gin, void* stream);
+/* *d_loss = mean over [G][T] of (p - o)^2 / (sigma_g + eps)^2 (G <= the plan's gauges; the
+ * plan lends its scratch). Backward: d_gp = gloss * dloss/dp, gloss = *d_gloss or 1. */
+ddrs_status ddrs_nse_batch_loss(ddrs_gauge_plan_t plan, const float* d_p, const float* d_o,
+                                const float* d_sigma, int32_t G, int32_t T, float eps,
+                                float* d_loss, void* stream);
+ddrs_status ddrs_nse_batch_loss_bwd(const float* d_p, const float* d_o, const float* d_sigma,
+                                    int32_t G, int32_t T, float eps, const float* d_gloss,
+                                    float* d_gp, void* stream);
+/* The objective of one micro-batch with loss.kind nse-batch (driver.rs:110-201):
+ * gauge readout -> tau trim + daily pooling -> days [warmup, T_days) -> gauges with any NaN
+ * in d_obs [G][T_days] there dropped -> mean of (p - o)^2 / (sigma + eps)^2. Writes *d_loss
+ * (0 when no gauge is kept), *d_n_valid (kept gauges x days, loss_denominator, if not NULL),
+ * d_hourly [G][T_hours] (the readout, if not NULL) and, if d_grunoff is not NULL, dL/d runoff
+ * at the map's reaches (other entries not written). */
+ddrs_status ddrs_gauge_nse_loss(ddrs_gauge_plan_t plan, const float* d_runoff, int32_t T_hours,
+                                int64_t stride_n, int64_t stride_t, int32_t tau, int32_t warmup,
+                                const float* d_obs, const float* d_sigma, float eps,
+                                float* d_loss, int32_t* d_n_valid, float* d_grunoff,
+                                float* d_hourly, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Synthetic workloads (host), seeded and portable.
  * ------------------------------------------------------------------------- */
 /* Reach i < n-1 drains to a reach drawn uniformly from (i, min(i+window, n-1)];

@@ -1,0 +1,132 @@
+"""CPU oracle of the gauge readout and training loss (test infrastructure only).
+
+Only tests/, the smoke check and bench.py's CPU baseline use this module, and
+only to check the HIP kernels; the product path never calls it.
+
+Restates, in numpy:
+- scatter_add_by_group (reference src/training/forward.rs:18-34):
+  out[g, t] = sum over k with group_ids[k] == g of runoff[flat_indices[k], t],
+  added in increasing k;
+- area_pool_weights / tau_trim_and_downsample (src/training/loss.rs:33-95):
+  hours [tau, T - (24 - tau)) pooled into T_days = (T - 24) // 24 bins of width
+  s = L / M with fractional overlaps W[i, j] = (min(j+1, (i+1)s) - max(j, i s)) / s,
+  all in float32 as the reference computes them;
+- the driver's warmup slice and NaN-gauge filter (src/training/driver.rs:150-195)
+  and nse_batch_loss (src/training/loss.rs:226-237):
+  mean over kept gauges and post-warmup days of (p - o)^2 / (sigma + eps)^2.
+
+Pinned by the reference's own tests (tests/test_loss_oracle.py): the
+scatter fixture of src/training/forward.rs:1030-1058, the hand-computed
+nse_batch_loss of loss.rs:491-507, the pooling cases of loss.rs:609-727, and
+torch's adaptive_avg_pool1d where the window is a whole number of days.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def scatter_add_by_group(runoff, flat_indices, group_ids, num_gauges, dtype=np.float32):
+    """runoff [N, T] -> [G, T], sums in increasing k (float32: the reference's
+    rounding)."""
+    runoff = np.asarray(runoff, dtype)
+    out = np.zeros((num_gauges, runoff.shape[1]), dtype)
+    for k in range(len(flat_indices)):
+        out[group_ids[k]] += runoff[flat_indices[k]]
+    return out
+
+
+def scatter_add_by_group_bwd(gout, flat_indices, group_ids, n_reaches):
+    """dL/d runoff [N, T] for dL/d out = gout [G, T]."""
+    gout = np.asarray(gout, np.float32)
+    g = np.zeros((n_reaches, gout.shape[1]), np.float32)
+    for k in range(len(flat_indices)):
+        g[flat_indices[k]] += gout[group_ids[k]]
+    return g
+
+
+def area_pool_weights(l, m):
+    """W [M, L] of loss.rs:70-95 in float32."""
+    assert 0 < m <= l
+    s = np.float32(l) / np.float32(m)
+    W = np.zeros((m, l), np.float32)
+    for i in range(m):
+        left = np.float32(i) * s
+        right = np.float32(i + 1) * s
+        j_lo = int(np.floor(left))
+        j_hi = min(int(np.ceil(right)), l)
+        for j in range(j_lo, j_hi):
+            cl = max(np.float32(j), left)
+            cr = min(np.float32(j + 1), right)
+            W[i, j] = np.float32(cr - cl) / s
+    return W
+
+
+def trim_window(t_hours, tau):
+    assert 0 <= tau < 24
+    start, end = tau, t_hours - (24 - tau)
+    assert start < end
+    L = end - start
+    M = L // 24
+    assert M > 0
+    return start, end, L, M
+
+
+def tau_trim_and_downsample(pred_hourly, tau, dtype=np.float64):
+    """[G, T_hours] -> [G, T_days]: W applied in `dtype` (float64 for the
+    reference value of the float32 weights, float32 to time it)."""
+    x = np.asarray(pred_hourly)
+    if x.dtype != np.float64:
+        x = x.astype(np.float32)
+    start, end, L, M = trim_window(x.shape[1], tau)
+    W = area_pool_weights(L, M).astype(dtype)
+    return x[:, start:end].astype(dtype) @ W.T
+
+
+def tau_trim_and_downsample_bwd(gout, t_hours, tau):
+    """dL/d pred_hourly [G, T_hours] (zero outside the trimmed window)."""
+    gout = np.asarray(gout, np.float64)
+    start, end, L, M = trim_window(t_hours, tau)
+    W = area_pool_weights(L, M).astype(np.float64)
+    g = np.zeros((gout.shape[0], t_hours))
+    g[:, start:end] = gout @ W
+    return g
+
+
+def kept_gauges(obs, warmup):
+    """Gauges whose post-warmup observations hold no NaN (driver.rs:168-176)."""
+    obs = np.asarray(obs)
+    return ~np.isnan(obs[:, warmup:]).any(axis=1)
+
+
+def nse_batch_loss(p, o, sigma, eps):
+    """Mean over (gauge, day) of (p - o)^2 / (sigma + eps)^2, in float64."""
+    p, o = np.asarray(p, np.float64), np.asarray(o, np.float64)
+    d = (np.asarray(sigma, np.float64) + eps)[:, None]
+    return float((((p - o) ** 2) / (d * d)).mean())
+
+
+def gauge_nse_loss(runoff, flat_indices, group_ids, num_gauges, tau, warmup, obs, sigma, eps,
+                   dtype=np.float64):
+    """The training objective from the routed discharge: (loss, n_valid,
+    dL/d runoff). runoff [N, T_hours]; obs [G, T_days] with NaN for missing
+    days; sigma [G]. With no gauge kept the loss is 0 and n_valid 0. The
+    gauge sums run in `dtype`, the rest in float64."""
+    hourly = scatter_add_by_group(runoff, flat_indices, group_ids, num_gauges, dtype)
+    daily = tau_trim_and_downsample(hourly, tau)
+    keep = kept_gauges(obs, warmup)
+    t_post = daily.shape[1] - warmup
+    n_valid = int(keep.sum()) * t_post
+    g_daily = np.zeros_like(daily)
+    if n_valid == 0:
+        loss = 0.0
+    else:
+        p = daily[keep, warmup:]
+        o = np.asarray(obs, np.float64)[keep, warmup:]
+        d = (np.asarray(sigma, np.float64)[keep] + eps)[:, None]
+        loss = float((((p - o) ** 2) / (d * d)).sum() / n_valid)
+        g_daily[np.ix_(keep, np.arange(warmup, daily.shape[1]))] = 2.0 * (p - o) / (d * d) / n_valid
+    g_hourly = tau_trim_and_downsample_bwd(g_daily, hourly.shape[1], tau)
+    g_runoff = np.zeros(np.asarray(runoff).shape)
+    for k in range(len(flat_indices)):
+        g_runoff[flat_indices[k]] += g_hourly[group_ids[k]]
+    return loss, n_valid, g_runoff
