@@ -7,7 +7,10 @@ hourly steps, fp32 lateral inflow LogNormal(ln 0.5, 1) with storm pulses,
 per-reach Manning n and q_spatial as leaves. One bench step = the batched
 persistent forward window, the loss gradient (MSE against a fixed target at 64
 gauge reaches) and the persistent adjoint window, producing dL/dn and
-dL/dq_spatial for every reach. Inputs are resident in HBM before timing.
+dL/dq_spatial for every reach. The window calls read out the 64 gauges in the
+kernels (the forward writes Q only there, the adjoint takes dL/dQ only there);
+results are bit-identical to the dense calls. Inputs are resident in HBM
+before timing.
 
 Also reported: the same step in the plain per-timestep-launch form, the
 roofline of the dominant kernel (the persistent forward window) from HIP events
@@ -19,6 +22,11 @@ the batched and training workloads): 10 attributes ~ N(0, 1) -> 21 hidden ->
 two KAN blocks (grid 5, order 3) -> 2 sigmoid outputs, forward + backward over
 350,000 reaches, against the fp32 peak and beside its CPU oracle.
 --kan-only runs just that part.
+
+"gauge_loss" reports the fused micro-batch objective of the training workload
+(gauge readout, tau trim and daily pooling, warmup slice, NaN-gauge filter,
+batch NSE, and its gradient into the routed discharge) for 2,000 gauges over a
+time-major 8,760-hour x 350,000-reach discharge. --loss-only runs just that.
 """
 from __future__ import annotations
 
@@ -37,6 +45,8 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 # KAN head of the batched and training workloads, evaluated per reach
 KAN_SHAPE = dict(F=10, H=21, P=2, L=2, grid=5, k=3, N=350_000)
+# gauge readout + loss of the training workload
+LOSS_SHAPE = dict(R=350_000, T=8760, G=2000, tau=9, warmup=30)
 # Algorithmic HBM bytes per reach-timestep of each window kernel:
 #   forward window: q' read 4, Q written 4, x history written 8 and read back
 #     8 by the downstream chunks (one link per reach in a tree);
@@ -75,6 +85,8 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="schedule block, 64 or 128 (0: default)")
     ap.add_argument("--kan-only", action="store_true", help="only the KAN head measurement")
     ap.add_argument("--no-kan", action="store_true", help="skip the KAN head measurement")
+    ap.add_argument("--loss-only", action="store_true", help="only the gauge loss measurement")
+    ap.add_argument("--no-loss", action="store_true", help="skip the gauge loss measurement")
     ap.add_argument("--pmc-calibration", action="store_true",
                     help="first copy 1 GiB device to device once, as the known-size reference "
                          "for FETCH_SIZE / WRITE_SIZE counters")
@@ -93,6 +105,11 @@ def main():
     if args.kan_only:
         kan = kan_bench(torch, args.steps, args.warmup, not args.no_cpu)
         print(json.dumps({"metric": "reaches/s KAN head forward+backward", **kan}), flush=True)
+        return
+    if args.loss_only:
+        gl = loss_bench(torch, args.steps, args.warmup, not args.no_cpu)
+        print(json.dumps({"metric": "gauge-hours/s fused readout + loss + gradient", **gl}),
+              flush=True)
         return
 
     if args.pmc_calibration:
@@ -131,25 +148,27 @@ def main():
     hot = net.workspace()
     NAT.check(NAT.lib.ddrs_hotstart(hot.handle, cp, ptr(qprime[0]), ptr(q0), sptr))
     hot.check()
-    Q = torch.empty((T, R), dtype=torch.float32, device=dev)
-    gQ = torch.zeros((T, R), dtype=torch.float32, device=dev)
     gn, gqsp = torch.empty(R, device=dev), torch.empty(R, device=dev)
     rng = np.random.default_rng(4000 + ci)
-    gauges = torch.as_tensor(np.sort(np.concatenate([[R - 1], rng.choice(R - 1, 63, replace=False)])),
-                             device=dev, dtype=torch.long)
+    gauges_np = np.sort(np.concatenate([[R - 1], rng.choice(R - 1, 63, replace=False)]))
+    ws.set_gauges(gauges_np)
+    K = gauges_np.size
+    Q = torch.empty((T, K), dtype=torch.float32, device=dev)   # discharge at the gauges
+    gQ = torch.empty((T, K), dtype=torch.float32, device=dev)
     # fixed observation target: the routed discharge at the gauges scaled by 0.9
-    NAT.check(NAT.lib.ddrs_mc_window_fwd(ws.handle, cp, T, ptr(qprime), ptr(q0), ptr(Q),
-                                         NAT.FORM_PERSISTENT, sptr))
+    NAT.check(NAT.lib.ddrs_mc_window_fwd_gauged(ws.handle, cp, T, ptr(qprime), ptr(q0), ptr(Q),
+                                                NAT.FORM_PERSISTENT, sptr))
     ws.check()
-    obs = (0.9 * Q[:, gauges]).clone()
-    scale = 2.0 / (T * gauges.numel())
+    obs = (0.9 * Q).clone()
+    scale = 2.0 / (T * K)
 
     def step(form):
-        NAT.check(NAT.lib.ddrs_mc_window_fwd(ws.handle, cp, T, ptr(qprime), ptr(q0), ptr(Q), form,
-                                             sptr))
-        gQ[:, gauges] = scale * (Q[:, gauges] - obs)
-        NAT.check(NAT.lib.ddrs_mc_window_adj(ws.handle, cp, T, ptr(qprime), ptr(q0), ptr(gQ),
-                                             ptr(gn), ptr(gqsp), None, None, None, form, sptr))
+        NAT.check(NAT.lib.ddrs_mc_window_fwd_gauged(ws.handle, cp, T, ptr(qprime), ptr(q0),
+                                                    ptr(Q), form, sptr))
+        torch.mul(torch.sub(Q, obs), scale, out=gQ)
+        NAT.check(NAT.lib.ddrs_mc_window_adj_gauged(ws.handle, cp, T, ptr(qprime), ptr(q0),
+                                                    ptr(gQ), ptr(gn), ptr(gqsp), None, None, None,
+                                                    form, sptr))
 
     for _ in range(args.warmup):
         step(NAT.FORM_PERSISTENT)
@@ -256,6 +275,10 @@ def main():
     }
     if not args.no_kan:
         out["kan_head"] = kan_bench(torch, args.steps, args.warmup, not args.no_cpu)
+    if not args.no_loss:
+        del Q, gQ
+        torch.cuda.empty_cache()
+        out["gauge_loss"] = loss_bench(torch, args.steps, args.warmup, not args.no_cpu)
     print(json.dumps(out), flush=True)
 
 
@@ -416,6 +439,89 @@ def kan_cpu_baseline(F, H, P, L, grid, k, n_sample=40_000):
     return {"value": n_sample / dt, "unit": "reaches/s", "cores": 1, "kind": "port",
             "sample": f"oracle/kan_oracle.py forward + backward in f32 over {n_sample} "
                       f"reaches of the same head shape, one thread ({dt:.1f} s)"}
+
+
+def loss_bench(torch, steps, warmup, with_cpu):
+    """The fused objective (ddrs_gauge_nse_loss) through the C ABI, HIP events
+    on its stream; the numpy oracle on a bounded sample of gauges."""
+    from ddrs_amd import _native as NAT
+    from ddrs_amd import loss as LS
+    s = LOSS_SHAPE
+    R, T, G, tau, wu = s["R"], s["T"], s["G"], s["tau"], s["warmup"]
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(21)
+    # one outflow reach per gauge, every tenth gauge with a second one
+    first = rng.choice(R, G, replace=False)
+    extra = rng.choice(R, G // 10, replace=False)
+    flat = np.concatenate([first, extra]).astype(np.int32)
+    group = np.concatenate([np.arange(G), np.arange(0, G, 10)]).astype(np.int32)
+    plan = LS.GaugePlan(R, flat, group, G, T)
+    gen = torch.Generator(device=dev).manual_seed(5)
+    Q = torch.rand(T, R, generator=gen, device=dev) * 10 + 0.1      # time-major [T][R]
+    gQ = torch.zeros_like(Q)
+    days = (T - 24) // 24
+    obs = torch.rand(G, days, generator=gen, device=dev) * 10 + 0.1
+    obs[::7, days // 2] = float("nan")                                 # every 7th gauge dropped
+    sigma = torch.rand(G, generator=gen, device=dev) + 0.5
+    loss = torch.empty((), device=dev)
+    nv = torch.empty((), dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+
+    def call():
+        NAT.check(NAT.lib.ddrs_gauge_nse_loss(plan.handle, Q.data_ptr(), T, 1, R, tau, wu,
+                                              obs.data_ptr(), sigma.data_ptr(), 0.1,
+                                              loss.data_ptr(), nv.data_ptr(), gQ.data_ptr(),
+                                              None, sp), "ddrs_gauge_nse_loss")
+
+    for _ in range(max(warmup, 1)):
+        call()
+    torch.cuda.synchronize()
+    reps = max(steps, 1) * 4
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * reps)]
+    for r in range(reps):
+        ev[2 * r].record(stream)
+        call()
+        ev[2 * r + 1].record(stream)
+    torch.cuda.synchronize()
+    ms = sum(ev[2 * r].elapsed_time(ev[2 * r + 1]) for r in range(reps)) / reps
+    K = flat.size
+    # algorithmic HBM bytes: discharge of the map's reaches read and their gradient
+    # written (K x T x 4 each), the hourly readout written and read back and its
+    # gradient the same (G x T x 4 x 4), observations read (G x days x 4)
+    alg = 2 * K * T * 4 + 4 * G * T * 4 + G * days * 4
+    gbs = alg / (ms * 1e-3) / 1e9
+    log(f"[bench] gauge loss G={G} T={T}: {ms:.3f} ms, {gbs:.0f} GB/s algorithmic")
+    res = {
+        "value": G * T / (ms * 1e-3), "unit": "gauge-hours/s", "ms_per_call": ms,
+        "calls_timed": reps, "loss": float(loss), "n_valid": int(nv), "dtype": "f32",
+        "config": {"workload": f"fused readout + tau trim/daily pooling + warmup + NaN filter "
+                               f"+ batch NSE + gradient: {G} gauges ({K} outflow entries) over "
+                               f"a time-major {T} x {R} discharge", **s},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": None, "bytes_per_call": alg,
+                     "note": "4-byte reads and writes at scattered reaches of a time-major "
+                             "series: each touches its own cache line"},
+    }
+    if with_cpu:
+        from oracle import loss_oracle as lo
+        gs = 200
+        sel = np.isin(group, np.arange(gs))
+        Qs = Q[:, torch.as_tensor(np.unique(flat[sel]).astype(np.int64), device=dev)]
+        remap = {r: i for i, r in enumerate(np.unique(flat[sel]))}
+        f2 = [remap[r] for r in flat[sel]]
+        runoff = Qs.T.contiguous().cpu().numpy()
+        o_np, s_np = obs[:gs].cpu().numpy(), sigma[:gs].cpu().numpy()
+        t0 = time.perf_counter()
+        lo.gauge_nse_loss(runoff, f2, group[sel], gs, tau, wu, o_np, s_np, 0.1, np.float32)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": gs * T / dt, "unit": "gauge-hours/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"oracle/loss_oracle.py gauge_nse_loss on the first "
+                                         f"{gs} gauges of the same workload ({dt:.2f} s)"}
+    del Q, gQ
+    torch.cuda.empty_cache()
+    return res
 
 
 def cpu_baseline(rows, cols, at, R, steps, ci, storms, net, params):

@@ -56,6 +56,11 @@ _SIGS = {
     "ddrs_mc_window_fwd": (c_int, [P, ctypes.POINTER(DdrsParams), c_int, P, P, P, c_int, P]),
     "ddrs_mc_window_adj": (c_int, [P, ctypes.POINTER(DdrsParams), c_int, P, P, P, P, P, P, P, P,
                                    c_int, P]),
+    "ddrs_workspace_set_gauges": (c_int, [P, c_int, P]),
+    "ddrs_mc_window_fwd_gauged": (c_int, [P, ctypes.POINTER(DdrsParams), c_int, P, P, P, c_int,
+                                          P]),
+    "ddrs_mc_window_adj_gauged": (c_int, [P, ctypes.POINTER(DdrsParams), c_int, P, P, P, P, P,
+                                          P, P, P, c_int, P]),
     "ddrs_tri_solve_fwd": (c_int, [P, P, P, P, P]),
     "ddrs_tri_solve_adj": (c_int, [P, P, P, P, P, P, P]),
     "ddrs_hotstart": (c_int, [P, ctypes.POINTER(DdrsParams), P, P, P]),

@@ -121,6 +121,13 @@ struct ddrs_workspace {
     // of the last concurrent linearisation (null: off)
     unsigned long long* trace[3] = {nullptr, nullptr, nullptr};
     int trace_n[3] = {0, 0, 0};
+    // gauge readout (ddrs_workspace_set_gauges): slot of each position
+    // (schedule order, -1: none), reach of each slot, a dense [R] row for the
+    // last step's dL/dQ
+    int gK = 0;
+    int* gslot = nullptr;
+    int* greach = nullptr;
+    float* gdense = nullptr;
 };
 
 namespace {
@@ -768,6 +775,7 @@ struct AdjSegment {
     int seg = -1;
     int t0 = 0, len = 0;
     const float* gQ = nullptr;
+    const float* gQg = nullptr;  // gauge-sparse dL/dQ [T][gK] (gQ null)
     float* gqp_out = nullptr;
     int mask_qp = 1, mask_lam = 1;
     const float* ck_q = nullptr;
@@ -785,6 +793,11 @@ static void run_adj_segment(ddrs_workspace* ws, const ddrs_params* p, const AdjS
     W.ch = ws->plan->ch[ws->rpl_sweep == 2 ? 1 : 0].dev;
     W.t0 = g.t0;
     W.gqs = g.gQ ? ws->gqs : nullptr;
+    if (!g.gQ && g.gQg) {
+        W.gqs = g.gQg;
+        W.gslot = ws->gslot;
+        W.gK = ws->gK;
+    }
     W.rec = ws->rec;
     W.amsg = ws->amsg;
     W.st_lam = ws->st_lam;
@@ -916,12 +929,17 @@ float* ck_i(ddrs_workspace* ws, int seg) { return ck_q(ws, seg) + ws->plan->P; }
 // current for `seg` until the next adjoint or forward of the workspace.
 ddrs_status run_fwd_segment(ddrs_workspace* ws, const ddrs_params* params, int T, int seg,
                             const float* qprime, float* Q_out, ddrs_form form, hipStream_t s,
-                            bool timed, bool lin) {
+                            bool timed, bool lin, float* Qg_out = nullptr) {
     const int t0 = seg * ws->window;
     const int len = seg_len(ws, T, seg);
     FwdArgs a = fwd_args(ws, params);
     a.t0 = t0;
     a.Q_out = Q_out;
+    if (Qg_out && ws->gK > 0) {
+        a.Qg = Qg_out;
+        a.gslot = ws->gslot;
+        a.gK = ws->gK;
+    }
     ws->lin_seg = -1;
     if (form == DDRS_FORM_PERSISTENT) {
         a.s_base = 0;
@@ -983,9 +1001,9 @@ ddrs_status run_fwd_segment(ddrs_workspace* ws, const ddrs_params* params, int T
 
 extern "C" {
 
-ddrs_status ddrs_mc_window_fwd(ddrs_workspace_t ws, const ddrs_params* params, int32_t T,
-                               const float* d_q_prime, const float* d_q0, float* d_Q_out,
-                               ddrs_form form, void* stream) {
+static ddrs_status window_fwd(ddrs_workspace_t ws, const ddrs_params* params, int32_t T,
+                              const float* d_q_prime, const float* d_q0, float* d_Q_out,
+                              float* d_Qg_out, ddrs_form form, void* stream) {
     if (!ws) return fail(DDRS_ERR_INVALID, "workspace is NULL");
     ddrs_status st = check_params(params);
     if (st != DDRS_OK) return st;
@@ -1000,6 +1018,7 @@ ddrs_status ddrs_mc_window_fwd(ddrs_workspace_t ws, const ddrs_params* params, i
     const int P = ws->plan->P;
     ws->n_ev[TK_FWD] = ws->n_ev[TK_LIN_TAIL] = 0;
     launch_state(ws->plan->sc, d_q0, 1, params->discharge_lb, ws->st_q, ws->st_i, d_Q_out, s);
+    if (d_Qg_out) launch_gauge_row0(ws->greach, ws->gK, d_q0, params->discharge_lb, d_Qg_out, s);
     const int steps = T - 1;
     const int nseg = (steps + ws->window - 1) / ws->window;
     ws->qps_src = nullptr;  // the caller may have rewritten q' in place
@@ -1007,7 +1026,8 @@ ddrs_status ddrs_mc_window_fwd(ddrs_workspace_t ws, const ddrs_params* params, i
     for (int j = 0; j < nseg; ++j) {
         HIPCHK(hipMemcpyAsync(ck_q(ws, j), ws->st_q, P * sizeof(float), hipMemcpyDeviceToDevice, s));
         HIPCHK(hipMemcpyAsync(ck_i(ws, j), ws->st_i, P * sizeof(float), hipMemcpyDeviceToDevice, s));
-        st = run_fwd_segment(ws, params, T, j, d_q_prime, d_Q_out, form, s, true, j == nseg - 1);
+        st = run_fwd_segment(ws, params, T, j, d_q_prime, d_Q_out, form, s, true, j == nseg - 1,
+                             d_Qg_out);
         if (st != DDRS_OK) return st;
     }
     ws->last_T = T;
@@ -1015,12 +1035,26 @@ ddrs_status ddrs_mc_window_fwd(ddrs_workspace_t ws, const ddrs_params* params, i
     return DDRS_OK;
 }
 
-ddrs_status ddrs_mc_window_adj(ddrs_workspace_t ws, const ddrs_params* params, int32_t T,
-                               const float* d_q_prime, const float* d_q0,
-                               const float* d_grad_Q, float* d_grad_n,
-                               float* d_grad_q_spatial, float* d_grad_p_spatial,
-                               float* d_grad_q_prime, float* d_grad_q0, ddrs_form form,
-                               void* stream) {
+ddrs_status ddrs_mc_window_fwd(ddrs_workspace_t ws, const ddrs_params* params, int32_t T,
+                               const float* d_q_prime, const float* d_q0, float* d_Q_out,
+                               ddrs_form form, void* stream) {
+    return window_fwd(ws, params, T, d_q_prime, d_q0, d_Q_out, nullptr, form, stream);
+}
+
+ddrs_status ddrs_mc_window_fwd_gauged(ddrs_workspace_t ws, const ddrs_params* params, int32_t T,
+                                      const float* d_q_prime, const float* d_q0, float* d_Qg_out,
+                                      ddrs_form form, void* stream) {
+    if (!ws) return fail(DDRS_ERR_INVALID, "workspace is NULL");
+    if (ws->gK <= 0) return fail(DDRS_ERR_INVALID, "no gauges set (ddrs_workspace_set_gauges)");
+    if (!d_Qg_out) return fail(DDRS_ERR_INVALID, "NULL gauge output");
+    return window_fwd(ws, params, T, d_q_prime, d_q0, nullptr, d_Qg_out, form, stream);
+}
+
+static ddrs_status window_adj(ddrs_workspace_t ws, const ddrs_params* params, int32_t T,
+                              const float* d_q_prime, const float* d_q0, const float* d_grad_Q,
+                              const float* d_grad_Qg, float* d_grad_n, float* d_grad_q_spatial,
+                              float* d_grad_p_spatial, float* d_grad_q_prime, float* d_grad_q0,
+                              ddrs_form form, void* stream) {
     if (!ws) return fail(DDRS_ERR_INVALID, "workspace is NULL");
     ddrs_status st = check_params(params);
     if (st != DDRS_OK) return st;
@@ -1039,6 +1073,13 @@ ddrs_status ddrs_mc_window_adj(ddrs_workspace_t ws, const ddrs_params* params, i
         ws->lin_seg = -1;  // records of another q' or other parameters
     if (d_grad_Q) launch_permute_rows(sc, d_grad_Q, 0, steps, ws->gqs, s);
     const float* g_last = d_grad_Q ? d_grad_Q + static_cast<size_t>(T - 1) * R : nullptr;
+    if (d_grad_Qg) {
+        // the last step's dL/dQ as a dense row for the initial gradient
+        launch_zero(static_cast<size_t>(R), ws->gdense, s);
+        launch_gauge_scatter_row(ws->greach, ws->gK, d_grad_Qg + static_cast<size_t>(T - 1) * ws->gK,
+                                 ws->gdense, s);
+        g_last = ws->gdense;
+    }
     if (d_grad_q_prime) launch_zero(static_cast<size_t>(R), d_grad_q_prime + static_cast<size_t>(T - 1) * R, s);
     if (nseg == 0) {
         // T == 1: out[0] = max(Q0, lb) only
@@ -1064,6 +1105,7 @@ ddrs_status ddrs_mc_window_adj(ddrs_workspace_t ws, const ddrs_params* params, i
         g.t0 = j * ws->window;
         g.len = len;
         g.gQ = d_grad_Q;
+        g.gQg = d_grad_Qg;
         g.gqp_out = d_grad_q_prime;
         g.mask_qp = 1;
         g.mask_lam = 1;
@@ -1077,6 +1119,62 @@ ddrs_status ddrs_mc_window_adj(ddrs_workspace_t ws, const ddrs_params* params, i
     if (d_grad_q_spatial) launch_unpermute(sc, ws->acc_qsp, d_grad_q_spatial, s);
     if (d_grad_p_spatial) launch_unpermute(sc, ws->acc_p, d_grad_p_spatial, s);
     HIPCHK(hipGetLastError());
+    return DDRS_OK;
+}
+
+ddrs_status ddrs_mc_window_adj(ddrs_workspace_t ws, const ddrs_params* params, int32_t T,
+                               const float* d_q_prime, const float* d_q0,
+                               const float* d_grad_Q, float* d_grad_n,
+                               float* d_grad_q_spatial, float* d_grad_p_spatial,
+                               float* d_grad_q_prime, float* d_grad_q0, ddrs_form form,
+                               void* stream) {
+    return window_adj(ws, params, T, d_q_prime, d_q0, d_grad_Q, nullptr, d_grad_n,
+                      d_grad_q_spatial, d_grad_p_spatial, d_grad_q_prime, d_grad_q0, form, stream);
+}
+
+ddrs_status ddrs_mc_window_adj_gauged(ddrs_workspace_t ws, const ddrs_params* params, int32_t T,
+                                      const float* d_q_prime, const float* d_q0,
+                                      const float* d_grad_Qg, float* d_grad_n,
+                                      float* d_grad_q_spatial, float* d_grad_p_spatial,
+                                      float* d_grad_q_prime, float* d_grad_q0, ddrs_form form,
+                                      void* stream) {
+    if (!ws) return fail(DDRS_ERR_INVALID, "workspace is NULL");
+    if (ws->gK <= 0) return fail(DDRS_ERR_INVALID, "no gauges set (ddrs_workspace_set_gauges)");
+    return window_adj(ws, params, T, d_q_prime, d_q0, nullptr, d_grad_Qg, d_grad_n,
+                      d_grad_q_spatial, d_grad_p_spatial, d_grad_q_prime, d_grad_q0, form, stream);
+}
+
+ddrs_status ddrs_workspace_set_gauges(ddrs_workspace_t ws, int32_t K, const int32_t* h_reaches) {
+    if (!ws) return fail(DDRS_ERR_INVALID, "workspace is NULL");
+    const ddrs_plan* pl = ws->plan;
+    const int R = pl->sc.R, P = pl->sc.P;
+    if (K < 0 || (K > 0 && !h_reaches)) return fail(DDRS_ERR_INVALID, "bad gauge list");
+    std::vector<int> slot_of_reach(R, -1);
+    for (int k = 0; k < K; ++k) {
+        const int r = h_reaches[k];
+        if (r < 0 || r >= R) return fail(DDRS_ERR_INVALID, "gauge reach outside [0, n)");
+        if (slot_of_reach[r] >= 0) return fail(DDRS_ERR_INVALID, "gauge reach listed twice");
+        slot_of_reach[r] = k;
+    }
+    if (!ws->gslot) {
+        ddrs_status st = dalloc(ws, reinterpret_cast<void**>(&ws->gslot), static_cast<size_t>(P) * 4);
+        if (st != DDRS_OK) return st;
+        st = dalloc(ws, reinterpret_cast<void**>(&ws->gdense), static_cast<size_t>(R) * 4);
+        if (st != DDRS_OK) return st;
+    }
+    if (K > ws->gK || !ws->greach) {
+        // the old list, if any, stays allocated with the workspace
+        ddrs_status st = dalloc(ws, reinterpret_cast<void**>(&ws->greach),
+                                static_cast<size_t>(std::max(K, 1)) * 4);
+        if (st != DDRS_OK) return st;
+    }
+    std::vector<int> slot(P, -1);
+    for (int p = 0; p < P; ++p)
+        if (pl->perm[p] >= 0) slot[p] = slot_of_reach[pl->perm[p]];
+    HIPCHK(hipMemcpy(ws->gslot, slot.data(), static_cast<size_t>(P) * 4, hipMemcpyHostToDevice));
+    if (K > 0)
+        HIPCHK(hipMemcpy(ws->greach, h_reaches, static_cast<size_t>(K) * 4, hipMemcpyHostToDevice));
+    ws->gK = K;
     return DDRS_OK;
 }
 

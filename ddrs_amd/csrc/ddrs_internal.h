@@ -92,6 +92,11 @@ struct FwdArgs {
     const float* qps;      // [T-1][P] q' in schedule order
     float* Q_out;          // [T][R] reference order or null
     float* X_out;          // [R] pre-clamp solve (single step) or null
+    // Gauge readout (null: off): Qg[t][slot] for the positions whose gslot
+    // (schedule order, -1: none) is a slot in [0, gK).
+    float* Qg;
+    const int* gslot;
+    int gK;
     uint64_t* hist;        // [window][P] tagged x, schedule order
     float* st_q;           // [P] out[t] of the current step
     float* st_i;           // [P] N out[t]
@@ -143,7 +148,12 @@ struct SweepArgs {
     Claim claim;
     Trace trace;
     int t0, s_base, nsteps;
-    const float* gqs;      // [T-1][P] dL/dQ (direct) in schedule order, or null
+    // dL/dQ (direct), or null: with gK == 0 [T-1][P] in schedule order; with
+    // gK > 0 gauge-sparse [T-1][gK], read at the slot gslot[p] of the position
+    // (zero where gslot is -1)
+    const float* gqs;
+    const int* gslot;
+    int gK;
     uint32_t epoch;
     const float4* rec;
     uint64_t* amsg;        // [window][P][2] tagged (P, G)
@@ -205,6 +215,12 @@ void launch_unpermute(const Sched& sc, const float* src, float* dst, hipStream_t
 void launch_grad_q0(const Sched& sc, const float* lam, const float* q0, float lb, float* g,
                     hipStream_t s);
 void launch_zero(size_t n, float* x, hipStream_t s);
+// Qg[s] = max(q0[reach[s]], lb) for s < K (row 0 of a gauge readout)
+void launch_gauge_row0(const int* reach, int K, const float* q0, float lb, float* Qg,
+                       hipStream_t s);
+// dense[reach[s]] = g[s] for s < K (other entries untouched)
+void launch_gauge_scatter_row(const int* reach, int K, const float* g, float* dense,
+                              hipStream_t s);
 // out[k][p] = q[(t_first + k) * R + perm[p]] for k < rows (0 at empty positions)
 void launch_permute_rows(const Sched& sc, const float* q, int t_first, int rows, float* out,
                          hipStream_t s);

@@ -252,6 +252,7 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
     __shared__ float pb[CW], pnc1[CW], sq[CW], si[CW];
     __shared__ short pe0[CW], pe1[CW], psub[CW];
     __shared__ int porig[CW];
+    __shared__ int pgs[CW];  // gauge slot of the position, -1: none
     __shared__ short srep[kRepMax];
     __shared__ int lkc[kFastLinks][CW];
     __shared__ float lka[kFastLinks][CW];
@@ -273,6 +274,8 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
     // step is loaded one step ahead.
     Reach rs[RPL];
     float qp_next[RPL];
+    // whether this chunk holds a gauge of the readout (wave-uniform)
+    bool hg = false;
 
     // S1..S25 for window step s: coefficients and right-hand side.
     auto prepare = [&](int s, bool first) {
@@ -280,6 +283,16 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
         if (first) {
             stage_links<RPL>(c, base, base + nlinks, sc.up_pos, sc.up_adj, spos, sadj, skind);
             if (lane < nrep) srep[lane] = a.ch.rep_up[rep0 + lane];
+            if (a.Qg) {
+                int any = 0;
+#pragma unroll
+                for (int r = 0; r < RPL; ++r) {
+                    const int sl = a.gslot[p0 + r * 64 + lane];
+                    pgs[r * 64 + lane] = sl;
+                    any |= sl >= 0;
+                }
+                hg = __any(any) != 0;
+            }
         }
         const int t = a.t0 + a.s_base + s;
 #pragma unroll
@@ -454,6 +467,10 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
                 }
                 if (a.Q_out) a.Q_out[static_cast<size_t>(t + 1) * R + porig[j]] = qn;
                 if (a.X_out) a.X_out[porig[j]] = x[r];
+                if (hg) {
+                    const int sl = pgs[j];
+                    if (sl >= 0) a.Qg[static_cast<size_t>(t + 1) * a.gK + sl] = qn;
+                }
             }
         }
     };
@@ -670,6 +687,7 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
     // dL/d out[t], loaded straight into LDS
     __shared__ float4 prec[CW];
     __shared__ float pdq[CW];
+    __shared__ int pgs[CW];  // gauge slot of the position, -1: none
     __shared__ float slam[CW];
     __shared__ unsigned char pvalid[CW];
     __shared__ short pe0[CW], pe1[CW], psub[CW];
@@ -703,6 +721,10 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
         if (!first) return;
         stage_links<RPL>(c, base, base + nlinks, sc.dn_pos, sc.dn_adj, spos, sadj, skind);
         if (lane < nrep) srep[lane] = a.ch.rep_dn[rep0 + lane];
+        if (a.gK) {
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) pgs[r * 64 + lane] = a.gslot[p0 + r * 64 + lane];
+        }
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
             const int j = r * 64 + lane;
@@ -738,9 +760,14 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
             for (int r = 0; r < RPL; ++r) {
                 const size_t p = static_cast<size_t>(p0 + r * 64 + lane);
                 load_to_lds(a.rec + row + p, &prec[r * 64], 16);
-                if (a.gqs)
-                    load_to_lds(a.gqs + static_cast<size_t>(a.t0 + a.s_base + s) * P + p,
+                if (a.gqs) {
+                    // dense rows of P, or gauge rows of gK read at the position's slot
+                    // (every lane loads: slot 0 stands in where there is no gauge)
+                    const size_t idx = a.gK ? static_cast<size_t>(max(pgs[r * 64 + lane], 0)) : p;
+                    load_to_lds(a.gqs + static_cast<size_t>(a.t0 + a.s_base + s) *
+                                            static_cast<size_t>(a.gK ? a.gK : P) + idx,
                                 &pdq[r * 64], 4);
+                }
             }
         }
         const uint64_t* mrow = a.amsg + row * 2;
@@ -868,7 +895,7 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
                     ldsP[j] = Pm;
                     ldsG[j] = Gm;
                     gbv[r] = gb;
-                    const float dq = a.gqs ? pdq[j] : 0.0f;
+                    const float dq = (a.gqs && (!a.gK || pgs[j] >= 0)) ? pdq[j] : 0.0f;
                     lamv[r] = (rc.w != 0.0f) ? dq + (gb * rc.z + sumG) : 0.0f;
                 }
             }
@@ -1128,6 +1155,14 @@ __global__ void k_permute_rows(Sched sc, const float* q, int t_first, int rows, 
     }
 }
 
+__global__ void k_gauge_row0(const int* reach, int K, const float* q0, float lb, float* Qg) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < K) Qg[i] = fmaxr(q0[reach[i]], lb);
+}
+__global__ void k_gauge_scatter_row(const int* reach, int K, const float* g, float* dense) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < K) dense[reach[i]] = g[i];
+}
 __global__ void k_zero_f(size_t n, float* x) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) x[i] = 0.0f;
@@ -1246,6 +1281,14 @@ void launch_permute_rows(const Sched& sc, const float* q, int t_first, int rows,
 }
 void launch_zero(size_t n, float* x, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_zero_f, g1(n), dim3(256), 0, s, n, x);
+}
+void launch_gauge_row0(const int* reach, int K, const float* q0, float lb, float* Qg,
+                       hipStream_t s) {
+    if (K > 0) hipLaunchKernelGGL(k_gauge_row0, g1(K), dim3(256), 0, s, reach, K, q0, lb, Qg);
+}
+void launch_gauge_scatter_row(const int* reach, int K, const float* g, float* dense,
+                              hipStream_t s) {
+    if (K > 0) hipLaunchKernelGGL(k_gauge_scatter_row, g1(K), dim3(256), 0, s, reach, K, g, dense);
 }
 void launch_zero_u64(size_t n, unsigned long long* x, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_zero_u64, g1(n), dim3(256), 0, s, n, x);

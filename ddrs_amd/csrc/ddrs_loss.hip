@@ -93,6 +93,85 @@ __global__ void k_gauge_scatter_bwd(const int* uptr, const int* ureach, const in
     grunoff[static_cast<long long>(ureach[u]) * sn + static_cast<long long>(t) * st] = s;
 }
 
+// Row-wise forms for a time-major series (stride_t != 1): one workgroup per
+// hour, threads across gauges (or reaches), so the reads of one hour share the
+// row's pages instead of striding a whole row apart. The gauge series is kept
+// hour-major, hourly[t][g].
+__global__ void k_gauge_gather_rows(const int* gptr, const int* gflat, const float* runoff, int G,
+                                    long long sn, long long st, float* hourly_tg) {
+    const int t = blockIdx.x;
+    const float* row = runoff + static_cast<long long>(t) * st;
+    for (int g = threadIdx.x; g < G; g += blockDim.x) {
+        float s = 0.0f;
+        for (int k = gptr[g]; k < gptr[g + 1]; ++k) s += row[static_cast<long long>(gflat[k]) * sn];
+        hourly_tg[static_cast<size_t>(t) * G + g] = s;
+    }
+}
+
+__global__ void k_gauge_scatter_bwd_rows(const int* uptr, const int* ureach, const int* ugroup,
+                                         const float* g_tg, int G, int U, long long sn,
+                                         long long st, float* grunoff) {
+    const int t = blockIdx.x;
+    const float* gr = g_tg + static_cast<size_t>(t) * G;
+    float* row = grunoff + static_cast<long long>(t) * st;
+    for (int u = threadIdx.x; u < U; u += blockDim.x) {
+        float s = 0.0f;
+        for (int k = uptr[u]; k < uptr[u + 1]; ++k) s += gr[ugroup[k]];
+        row[static_cast<long long>(ureach[u]) * sn] = s;
+    }
+}
+
+// [R][C] -> [C][R] through a 32 x 33 LDS tile.
+__global__ void k_transpose(const float* in, int R, int C, float* out) {
+    __shared__ float tile[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    for (int dr = threadIdx.y; dr < 32; dr += blockDim.y) {
+        const int r = r0 + dr, c = c0 + threadIdx.x;
+        if (r < R && c < C) tile[dr][threadIdx.x] = in[static_cast<size_t>(r) * C + c];
+    }
+    __syncthreads();
+    for (int dc = threadIdx.y; dc < 32; dc += blockDim.y) {
+        const int c = c0 + dc, r = r0 + threadIdx.x;
+        if (r < R && c < C) out[static_cast<size_t>(c) * R + r] = tile[threadIdx.x][dc];
+    }
+}
+
+void transpose(const float* in, int R, int C, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_transpose, dim3((C + 31) / 32, (R + 31) / 32), dim3(32, 8), 0, s, in, R, C,
+                       out);
+}
+
+// daily[g][i] from an hour-major series in_tg[t][g]: threads across gauges.
+__global__ void k_area_pool_tg(const float* in_tg, int G, int start, int L, int M, float* out) {
+    const int g = blockIdx.y * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x;
+    if (g >= G) return;
+    const float s = static_cast<float>(L) / static_cast<float>(M);
+    const int j_lo = static_cast<int>(floorf(static_cast<float>(i) * s));
+    const int j_hi = min(static_cast<int>(ceilf(static_cast<float>(i + 1) * s)), L);
+    float acc = 0.0f;
+    for (int j = j_lo; j < j_hi; ++j)
+        acc = fmaf(in_tg[static_cast<size_t>(start + j) * G + g], area_w(i, j, s, L), acc);
+    out[static_cast<size_t>(g) * M + i] = acc;
+}
+
+// gin_tg[h][g] = sum_i gout[g][i] W(i, h - start); zero outside the window.
+__global__ void k_area_pool_bwd_tg(const float* gout, int G, int start, int L, int M,
+                                   float* gin_tg) {
+    const int g = blockIdx.y * blockDim.x + threadIdx.x;
+    const int h = blockIdx.x;
+    if (g >= G) return;
+    const int j = h - start;
+    float acc = 0.0f;
+    if (j >= 0 && j < L) {
+        const float s = static_cast<float>(L) / static_cast<float>(M);
+        const int i0 = static_cast<int>(static_cast<float>(j) / s);
+        for (int i = max(i0 - 1, 0); i <= min(i0 + 1, M - 1); ++i)
+            acc = fmaf(gout[static_cast<size_t>(g) * M + i], area_w(i, j, s, L), acc);
+    }
+    gin_tg[static_cast<size_t>(h) * G + g] = acc;
+}
+
 // daily[g][i] = sum_j in[g][start + j] W(i, j), j ascending.
 __global__ void k_area_pool(const float* in, int T, int start, int L, int M, float* out) {
     const int g = blockIdx.y;
@@ -315,9 +394,18 @@ ddrs_status ddrs_scatter_add_by_group(ddrs_gauge_plan_t p, const float* d_runoff
     if (!p || T < 0 || (T > 0 && (!d_runoff || !d_out)))
         return lfail(DDRS_ERR_INVALID, "scatter_add_by_group: bad arguments");
     if (T == 0) return DDRS_OK;
-    hipLaunchKernelGGL(k_gauge_gather, grid2(T, p->G), dim3(256), 0,
-                       static_cast<hipStream_t>(stream), p->gptr, p->gflat, d_runoff, T,
-                       static_cast<long long>(stride_n), static_cast<long long>(stride_t), d_out);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (stride_t == 1) {
+        hipLaunchKernelGGL(k_gauge_gather, grid2(T, p->G), dim3(256), 0, s, p->gptr, p->gflat,
+                           d_runoff, T, static_cast<long long>(stride_n), 1LL, d_out);
+    } else {
+        if (T > p->max_t)
+            return lfail(DDRS_ERR_CAPACITY, "scatter_add_by_group: T exceeds the plan's max_t_hours");
+        hipLaunchKernelGGL(k_gauge_gather_rows, dim3(T), dim3(256), 0, s, p->gptr, p->gflat,
+                           d_runoff, p->G, static_cast<long long>(stride_n),
+                           static_cast<long long>(stride_t), p->hourly);
+        transpose(p->hourly, T, p->G, d_out, s);
+    }
     return hip_check("scatter_add_by_group");
 }
 
@@ -327,10 +415,20 @@ ddrs_status ddrs_scatter_add_by_group_bwd(ddrs_gauge_plan_t p, const float* d_go
     if (!p || T < 0 || (T > 0 && (!d_gout || !d_grunoff)))
         return lfail(DDRS_ERR_INVALID, "scatter_add_by_group_bwd: bad arguments");
     if (T == 0 || p->U == 0) return DDRS_OK;
-    hipLaunchKernelGGL(k_gauge_scatter_bwd, grid2(T, p->U), dim3(256), 0,
-                       static_cast<hipStream_t>(stream), p->uptr, p->ureach, p->ugroup, d_gout, T,
-                       static_cast<long long>(stride_n), static_cast<long long>(stride_t),
-                       d_grunoff);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (stride_t == 1) {
+        hipLaunchKernelGGL(k_gauge_scatter_bwd, grid2(T, p->U), dim3(256), 0, s, p->uptr,
+                           p->ureach, p->ugroup, d_gout, T, static_cast<long long>(stride_n), 1LL,
+                           d_grunoff);
+    } else {
+        if (T > p->max_t)
+            return lfail(DDRS_ERR_CAPACITY,
+                         "scatter_add_by_group_bwd: T exceeds the plan's max_t_hours");
+        transpose(d_gout, p->G, T, p->hourly, s);
+        hipLaunchKernelGGL(k_gauge_scatter_bwd_rows, dim3(T), dim3(256), 0, s, p->uptr, p->ureach,
+                           p->ugroup, p->hourly, p->G, p->U, static_cast<long long>(stride_n),
+                           static_cast<long long>(stride_t), d_grunoff);
+    }
     return hip_check("scatter_add_by_group_bwd");
 }
 
@@ -400,12 +498,21 @@ ddrs_status ddrs_gauge_nse_loss(ddrs_gauge_plan_t p, const float* d_runoff, int3
         return lfail(DDRS_ERR_INVALID, "gauge_nse_loss: warmup must be in [0, days)");
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int G = p->G;
-    float* hourly = d_hourly ? d_hourly : p->hourly;
-    hipLaunchKernelGGL(k_gauge_gather, grid2(T_hours, G), dim3(256), 0, s, p->gptr, p->gflat,
-                       d_runoff, T_hours, static_cast<long long>(stride_n),
-                       static_cast<long long>(stride_t), hourly);
-    hipLaunchKernelGGL(k_area_pool, grid2(M, G), dim3(256), 0, s, hourly, T_hours, start, L, M,
-                       p->daily);
+    const bool rows = stride_t != 1;  // time-major: hour-major gauge series
+    if (rows) {
+        hipLaunchKernelGGL(k_gauge_gather_rows, dim3(T_hours), dim3(256), 0, s, p->gptr, p->gflat,
+                           d_runoff, G, static_cast<long long>(stride_n),
+                           static_cast<long long>(stride_t), p->hourly);
+        if (d_hourly) transpose(p->hourly, T_hours, G, d_hourly, s);
+        hipLaunchKernelGGL(k_area_pool_tg, dim3(M, (G + 255) / 256), dim3(256), 0, s, p->hourly,
+                           G, start, L, M, p->daily);
+    } else {
+        float* hourly = d_hourly ? d_hourly : p->hourly;
+        hipLaunchKernelGGL(k_gauge_gather, grid2(T_hours, G), dim3(256), 0, s, p->gptr, p->gflat,
+                           d_runoff, T_hours, static_cast<long long>(stride_n), 1LL, hourly);
+        hipLaunchKernelGGL(k_area_pool, grid2(M, G), dim3(256), 0, s, hourly, T_hours, start, L,
+                           M, p->daily);
+    }
     hipLaunchKernelGGL(k_nse_partial, dim3(G), dim3(256), 0, s, p->daily, d_obs, d_sigma, M,
                        warmup, eps, 1, p->part, p->valid);
     hipLaunchKernelGGL(k_nse_final, dim3(1), dim3(256), 0, s, p->part, p->valid, G, M - warmup,
@@ -413,12 +520,20 @@ ddrs_status ddrs_gauge_nse_loss(ddrs_gauge_plan_t p, const float* d_runoff, int3
     if (d_grunoff && p->U > 0) {
         hipLaunchKernelGGL(k_nse_bwd, grid2(M, G), dim3(256), 0, s, p->daily, d_obs, d_sigma, G,
                            M, warmup, eps, nullptr, p->valid, p->n_valid, p->gdaily);
-        hipLaunchKernelGGL(k_area_pool_bwd, grid2(T_hours, G), dim3(256), 0, s, p->gdaily,
-                           T_hours, start, L, M, p->hourly);
-        hipLaunchKernelGGL(k_gauge_scatter_bwd, grid2(T_hours, p->U), dim3(256), 0, s, p->uptr,
-                           p->ureach, p->ugroup, p->hourly, T_hours,
-                           static_cast<long long>(stride_n), static_cast<long long>(stride_t),
-                           d_grunoff);
+        if (rows) {
+            hipLaunchKernelGGL(k_area_pool_bwd_tg, dim3(T_hours, (G + 255) / 256), dim3(256), 0, s,
+                               p->gdaily, G, start, L, M, p->hourly);
+            hipLaunchKernelGGL(k_gauge_scatter_bwd_rows, dim3(T_hours), dim3(256), 0, s, p->uptr,
+                               p->ureach, p->ugroup, p->hourly, G, p->U,
+                               static_cast<long long>(stride_n), static_cast<long long>(stride_t),
+                               d_grunoff);
+        } else {
+            hipLaunchKernelGGL(k_area_pool_bwd, grid2(T_hours, G), dim3(256), 0, s, p->gdaily,
+                               T_hours, start, L, M, p->hourly);
+            hipLaunchKernelGGL(k_gauge_scatter_bwd, grid2(T_hours, p->U), dim3(256), 0, s,
+                               p->uptr, p->ureach, p->ugroup, p->hourly, T_hours,
+                               static_cast<long long>(stride_n), 1LL, d_grunoff);
+        }
     }
     return hip_check("gauge_nse_loss");
 }

@@ -154,6 +154,16 @@ class Workspace:
         N.check(N.lib.ddrs_workspace_set_chunk_width(self.handle, int(fwd), int(sweep)),
                 "ddrs_workspace_set_chunk_width")
 
+    def set_gauges(self, reaches) -> None:
+        """Gauge readout of the _gauged window calls: distinct reach ids
+        (reference order); their forward writes Q only at these reaches,
+        [T, K], and their adjoint takes dL/dQ only there. Empty clears."""
+        r = np.ascontiguousarray(reaches, np.int32)
+        N.check(N.lib.ddrs_workspace_set_gauges(self.handle, int(r.size),
+                                                r.ctypes.data if r.size else None),
+                "ddrs_workspace_set_gauges")
+        self.gauges = r
+
     def set_linearize_in_forward(self, enable: bool, grid: int = 0):
         """Linearise the adjoint while a persistent forward window runs, so
         that the adjoint that follows skips that phase (bit-identical results;

@@ -172,6 +172,29 @@ ddrs_status ddrs_mc_window_adj(ddrs_workspace_t ws, const ddrs_params* params, i
                                float* d_grad_q_prime, float* d_grad_q0, ddrs_form form,
                                void* stream);
 
+/* Gauge readout (no reference counterpart as an entry point; it fuses the
+ * per-gauge select of scatter_add_by_group, src/training/forward.rs:18-34,
+ * into the window). After ddrs_workspace_set_gauges(ws, K, h_reaches) (host
+ * array of K distinct reach ids, reference order; K = 0 clears), the _gauged
+ * window calls read and write only the gauge reaches:
+ *   ddrs_mc_window_fwd_gauged writes d_Qg_out [T][K] = Q[t][h_reaches[k]]
+ *     (bit-identical to the dense call's Q at those reaches) instead of Q [T][n];
+ *   ddrs_mc_window_adj_gauged takes dL/dQ at the gauges, d_grad_Qg [T][K], and
+ *     returns what ddrs_mc_window_adj returns for the dense dL/dQ that is zero
+ *     elsewhere (bit-identical).
+ * The full discharge is then neither written by the forward nor permuted by
+ * the adjoint. */
+ddrs_status ddrs_workspace_set_gauges(ddrs_workspace_t ws, int32_t K, const int32_t* h_reaches);
+ddrs_status ddrs_mc_window_fwd_gauged(ddrs_workspace_t ws, const ddrs_params* params, int32_t T,
+                                      const float* d_q_prime, const float* d_q0, float* d_Qg_out,
+                                      ddrs_form form, void* stream);
+ddrs_status ddrs_mc_window_adj_gauged(ddrs_workspace_t ws, const ddrs_params* params, int32_t T,
+                                      const float* d_q_prime, const float* d_q0,
+                                      const float* d_grad_Qg, float* d_grad_n,
+                                      float* d_grad_q_spatial, float* d_grad_p_spatial,
+                                      float* d_grad_q_prime, float* d_grad_q0, ddrs_form form,
+                                      void* stream);
+
 /* ---------------------------------------------------------------------------
  * Sparse seam: triangular_csr_solve (sparse/mod.rs:483-531) forward and the
  * CsrSolveOp backward (425-472; dispatch.rs:102-138, 214-244) for arbitrary

@@ -398,6 +398,43 @@ def _window_raw(c, ws, gQ, form=NAT.FORM_PERSISTENT):
     return res
 
 
+@pytest.mark.parametrize("form", [NAT.FORM_PERSISTENT, NAT.FORM_PLAIN])
+def test_gauged_window_identical(form):
+    """The gauge readout (forward writes Q only at K gauge reaches, adjoint
+    takes dL/dQ only there) gives the dense call's bits: Qg equals Q at the
+    gauges, and every gradient equals the dense adjoint's for the dL/dQ that
+    is zero off the gauges. Over checkpoint segments (window 13 < T-1)."""
+    say("gauged window", form)
+    c = Case(6000, 60, seed=41, storms=True)
+    rng = np.random.default_rng(42)
+    gauges = np.sort(np.concatenate([[c.R - 1], rng.choice(c.R - 1, 29, replace=False)]))
+    gQ = np.zeros((c.T, c.R), np.float32)
+    gQ[:, gauges] = rng.normal(0, 1, (c.T, gauges.size)).astype(np.float32)
+    ws = c.net.workspace(13, c.T - 1)
+    dense = _window_raw(c, ws, gQ, form)
+    ws.set_gauges(gauges)
+    params = to_params(c.P).to_c()
+    s = torch.cuda.current_stream().cuda_stream
+    qp, q0 = t(c.qprime), t(c.q0())
+    Qg = torch.empty((c.T, gauges.size), device="cuda")
+    gQg = t(gQ[:, gauges].copy())
+    outs = [torch.empty(c.R, device="cuda") for _ in range(4)]
+    gqp = torch.empty((c.T, c.R), device="cuda")
+    P = lambda x: x.data_ptr()
+    NAT.check(NAT.lib.ddrs_mc_window_fwd_gauged(ws.handle, params, c.T, P(qp), P(q0), P(Qg),
+                                                form, s))
+    NAT.check(NAT.lib.ddrs_mc_window_adj_gauged(ws.handle, params, c.T, P(qp), P(q0), P(gQg),
+                                                P(outs[0]), P(outs[1]), P(outs[2]), P(gqp),
+                                                P(outs[3]), form, s))
+    ws.check()
+    np.testing.assert_array_equal(Qg.cpu().numpy(), dense["Q"][:, gauges])
+    for k, v in zip(["n", "q_spatial", "p_spatial", "q0"], outs):
+        np.testing.assert_array_equal(v.cpu().numpy(), dense[k], err_msg=k)
+    np.testing.assert_array_equal(gqp.cpu().numpy(), dense["q_prime"])
+    with pytest.raises(NAT.DdrsError):
+        ws.set_gauges([1, 1])
+
+
 @pytest.mark.skip(reason="linearising in the forward window is disabled in the library: a "
                          "350000-reach, 720-step window with the concurrent launch did not "
                          "finish on the GPU and the cause is not known")
