@@ -83,6 +83,9 @@ _SIGS = {
                                          ctypes.POINTER(c_size)]),
     "ddrs_kan_head_fwd": (c_int, [P, P, c_int, P, P, P, c_size, P]),
     "ddrs_kan_head_bwd": (c_int, [P, P, c_int, P, P, P, P, P, c_size, P]),
+    "ddrs_zarr_array_info": (c_int, [ctypes.c_char_p, P, P, P, c_int, P]),
+    "ddrs_zarr_read_array": (c_int, [ctypes.c_char_p, c_int, P, ctypes.c_int64]),
+    "ddrs_zarr_group_attr": (c_int, [ctypes.c_char_p, ctypes.c_char_p, P, P, c_int]),
     "ddrs_gauge_plan_create": (c_int, [c_int, c_int, c_int, P, P, c_int, ctypes.POINTER(P)]),
     "ddrs_gauge_plan_destroy": (None, [P]),
     "ddrs_scatter_add_by_group": (c_int, [P, P, c_int, ctypes.c_int64, ctypes.c_int64, P, P]),

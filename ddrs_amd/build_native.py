@@ -9,7 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libddrs_hip.so")
 SOURCES = ["ddrs_kernels.hip", "ddrs_host.hip", "ddrs_index.hip", "ddrs_gen.hip", "ddrs_kan.hip",
-           "ddrs_loss.hip"]
+           "ddrs_loss.hip", "ddrs_zarr.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -25,6 +25,7 @@ FLAGS = [
     "-Wall",
     "-Wno-unused-result",
     "-Wno-unused-value",
+    "-ldl",
 ]
 
 

@@ -407,6 +407,33 @@ ddrs_status ddrs_gauge_nse_loss(ddrs_gauge_plan_t plan, const float* d_runoff, i
                                 float* d_hourly, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Network stores on disk (host): zarr v3 arrays and group attributes, as the
+ * reference reads its binsparse-COO stores with the zarrs crate
+ * (src/data/store/zarr.rs:47-300: order, indices_0/1, length_m, slope,
+ * parent_order / parent_offset; gage_idx / gage_catchment attributes).
+ * Supported: regular chunk grids (C order), "bytes" (little or big endian) +
+ * optional "zstd" (decoded by the system's libzstd.so.1), default or v2 chunk
+ * keys; a missing chunk reads as the fill value. Anything else is an error.
+ * ------------------------------------------------------------------------- */
+typedef enum {
+    DDRS_ZARR_I8 = 1, DDRS_ZARR_U8 = 2, DDRS_ZARR_I16 = 3, DDRS_ZARR_U16 = 4,
+    DDRS_ZARR_I32 = 5, DDRS_ZARR_U32 = 6, DDRS_ZARR_I64 = 7, DDRS_ZARR_U64 = 8,
+    DDRS_ZARR_F32 = 9, DDRS_ZARR_F64 = 10
+} ddrs_zarr_dtype;
+/* Data type, rank, shape (first shape_cap dims) and element count of the
+ * array stored in directory array_dir (its zarr.json). Any output may be NULL. */
+ddrs_status ddrs_zarr_array_info(const char* array_dir, int32_t* dtype, int32_t* ndim,
+                                 int64_t* shape, int32_t shape_cap, int64_t* count);
+/* The whole array, C order, into h_out (capacity elements); dtype must be the
+ * stored type (no conversion, as the reference's typed reads). */
+ddrs_status ddrs_zarr_read_array(const char* array_dir, int32_t dtype, void* h_out,
+                                 int64_t capacity);
+/* Attribute `key` of the group in group_dir: its number (NaN if not numeric)
+ * and its text (strings, or a number's JSON text), either output optional. */
+ddrs_status ddrs_zarr_group_attr(const char* group_dir, const char* key, double* h_num,
+                                 char* h_text, int32_t text_cap);
+
+/* ---------------------------------------------------------------------------
  * Synthetic workloads (host), seeded and portable.
  * ------------------------------------------------------------------------- */
 /* Reach i < n-1 drains to a reach drawn uniformly from (i, min(i+window, n-1)];

@@ -101,14 +101,15 @@ class Case:
     """A seeded routing case: network, statics, forcing, parameters."""
 
     def __init__(self, R, T, seed, window=256, storms=False, params=None, rows=None, cols=None,
-                 vals=None):
+                 vals=None, length=None, slope=None):
         if rows is None:
             rows, cols = NW.gen_tree(R, window, 1000 + seed)
         self.R, self.T = R, T
         self.rows, self.cols = rows, cols
         self.vals = np.ones(rows.shape[0], np.float32) if vals is None else vals
         at = NW.gen_attributes(R, 2000 + seed)
-        self.length, self.slope_raw = at["length"], at["slope"]
+        self.length = at["length"] if length is None else np.asarray(length, np.float32)
+        self.slope_raw = at["slope"] if slope is None else np.asarray(slope, np.float32)
         self.n, self.qsp = at["n"], at["q_spatial"]
         self.p = np.full(R, 21.0, np.float32)
         self.qprime = NW.gen_forcing(R, T, 3000 + seed, storms)
@@ -251,6 +252,33 @@ def test_window_adjoint_vs_oracle():
                         dtype=np.float64)
     for name in ["n", "q_spatial", "p_spatial", "q_prime", "q0"]:
         grad_close(name, got[name], ref[name], ref64[name])
+
+
+def test_juniata_real_network(golden_dir):
+    """The reference's Juniata network read from its zarr store (213 MERIT
+    reaches, 212 edges, real length_m and slope; slopes of 0 clamp to 1e-3):
+    a 30-day hourly window with storm forcing, discharge parity as stated in
+    the module docstring, and the adjoint of a loss at the outlet gauge
+    (gage_idx 212) within the gradient bar."""
+    from ddrs_amd import store as ST
+    say("juniata 213 x 720")
+    conus = ST.ConusAdjacencyStore.open(
+        os.path.join(golden_dir, "juniata", "juniata_conus_adjacency.zarr"))
+    gage = ST.GagesAdjacencyStore.open(
+        os.path.join(golden_dir, "juniata", "juniata_gages_adjacency.zarr"), ["01567000"]).get("01567000")
+    c = Case(conus.n, 24 * 30, seed=51, storms=True, rows=conus.indices_0, cols=conus.indices_1,
+             length=conus.length_m, slope=conus.slope)
+    ref, ref64 = c.oracle_Q(), c.oracle_Q(np.float64)
+    gQ = np.zeros((c.T, c.R), np.float32)
+    gQ[:, gage.gage_idx] = np.random.default_rng(52).uniform(0.5, 1.5, c.T).astype(np.float32)
+    got = c.gpu_window(D.FORM_PERSISTENT, gQ=gQ)
+    check_discharge("juniata", got["Q"], ref, ref64)
+    q0 = c.q0()
+    radj = O.route_adj(c.P, c.pat, c.n, c.qsp, c.p, c.length, c.slope_raw, c.qprime, gQ, q0=q0)
+    radj64 = O.route_adj(c.P, c.pat, c.n, c.qsp, c.p, c.length, c.slope_raw, c.qprime, gQ, q0=q0,
+                         dtype=np.float64)
+    for name in ["n", "q_spatial", "p_spatial", "q_prime", "q0"]:
+        grad_close(f"juniata {name}", got[name], radj[name], radj64[name])
 
 
 def test_plain_persistent_and_checkpointed_are_identical():
