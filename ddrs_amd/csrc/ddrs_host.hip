@@ -86,6 +86,7 @@ struct ddrs_workspace {
     float *acc_n = nullptr, *acc_qsp = nullptr, *acc_p = nullptr;
     float* ck = nullptr;  // [max_seg][2][P]
     uint64_t *hist = nullptr, *amsg = nullptr, *slot = nullptr;
+    float2* ihist = nullptr;  // [window][P] the forward's upstream sums per reach-step
     unsigned long long* claim_ctr = nullptr;  // chunk claims of persistent launches
     unsigned long long claims = 0;            // claims made so far
     float* qps = nullptr;                    // [max_steps][P] q' in schedule order
@@ -601,6 +602,7 @@ ddrs_status ddrs_workspace_create(ddrs_plan_t plan, int32_t window, int32_t max_
     A(&ws->acc_p, R * 4);
     A(&ws->ck, static_cast<size_t>(ws->max_seg) * 2 * R * 4);
     A(&ws->hist, WR * 8);
+    A(&ws->ihist, WR * sizeof(float2));
     A(&ws->amsg, WR * 16);
     A(&ws->rec, WR * sizeof(float4));
     A(&ws->recj, WR * sizeof(float4));
@@ -715,6 +717,7 @@ static FwdArgs fwd_args(ddrs_workspace* ws, const ddrs_params* p) {
     a.st = ws->st;
     a.P = to_phys(p);
     a.hist = ws->hist;
+    a.ihist = ws->ihist;
     a.qps = ws->qps;
     a.st_q = ws->st_q;
     a.st_i = ws->st_i;
@@ -776,6 +779,7 @@ struct AdjSegment {
     int t0 = 0, len = 0;
     const float* gQ = nullptr;
     const float* gQg = nullptr;  // gauge-sparse dL/dQ [T][gK] (gQ null)
+    bool fwd_sums = true;        // the history rows came from the window forward
     float* gqp_out = nullptr;
     int mask_qp = 1, mask_lam = 1;
     const float* ck_q = nullptr;
@@ -786,6 +790,7 @@ static void run_adj_segment(ddrs_workspace* ws, const ddrs_params* p, const AdjS
                             ddrs_form form, hipStream_t s) {
     const Sched& sc = ws->plan->sc;
     LinArgs L = lin_args(ws, p, g.t0, g.mask_qp, g.mask_lam, g.ck_q, g.ck_i);
+    L.ihist = g.fwd_sums ? ws->ihist : nullptr;
     const bool have_rec = form == DDRS_FORM_PERSISTENT && g.seg >= 0 && ws->lin_seg == g.seg;
     ws->lin_seg = -1;
     SweepArgs W{};
@@ -895,6 +900,7 @@ ddrs_status ddrs_mc_step_bwd(ddrs_workspace_t ws, const ddrs_params* params, con
     g.mask_lam = 0;                // grad_q_t is the gradient of the raw q_t input
     g.ck_q = ws->st_q;
     g.ck_i = ws->st_i;
+    g.fwd_sums = false;            // the history row is the caller's solve
     run_adj_segment(ws, params, g, DDRS_FORM_PERSISTENT, s);
     HIPCHK(hipGetLastError());
     if (d_grad_n) launch_unpermute(sc, ws->acc_n, d_grad_n, s);

@@ -98,6 +98,9 @@ struct FwdArgs {
     const int* gslot;
     int gK;
     uint64_t* hist;        // [window][P] tagged x, schedule order
+    // [window][P] upstream sums of each reach-step, or null: (sum_k adj_k x[up k],
+    // sum_k adj_k max(x[up k], lb)) over the reach's upstream links in CSR order
+    float2* ihist;
     float* st_q;           // [P] out[t] of the current step
     float* st_i;           // [P] N out[t]
     int* err;
@@ -114,6 +117,9 @@ struct LinArgs {
     int mask_qp;           // 1: gradient w.r.t. raw q' through max(q', lb)
     int mask_lam;          // 1: dL/d out[t] is passed on through out[t] = max(x, lb)
     const uint64_t* hist;  // forward history of the segment
+    // the forward's upstream sums of the segment (FwdArgs::ihist), or null:
+    // then they are gathered from the upstream positions' history words
+    const float2* ihist;
     const float* ck_q;     // [P] out[t0] (schedule order)
     const float* ck_i;     // [P] N out[t0]
     float4* rec;           // [window][P] {-c1, c2, d gq / d gb, mask of dL/d out[t]}

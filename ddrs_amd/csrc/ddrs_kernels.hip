@@ -399,13 +399,14 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
         for (int r = 0; r < RPL; ++r) touch(qp_next[r]);
         // per-lane operands of the solve, read once per step
         int my[RPL], e0[RPL], nl[RPL], code[RPL][kFastLinks];
-        float x[RPL], inext[RPL], b[RPL], nc1[RPL], w[RPL][kFastLinks];
+        float x[RPL], inext[RPL], sxn[RPL], b[RPL], nc1[RPL], w[RPL][kFastLinks];
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
             const int j = r * 64 + lane;
             my[r] = psub[j];
             x[r] = 0.0f;
             inext[r] = 0.0f;
+            sxn[r] = 0.0f;
             b[r] = pb[j];
             nc1[r] = pnc1[j];
             e0[r] = pe0[j];
@@ -421,7 +422,7 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
             for (int r = 0; r < RPL; ++r) {
                 if (my[r] == l) {
                     const int j = r * 64 + lane;
-                    float sum = 0.0f, in = 0.0f;
+                    float sum = 0.0f, in = 0.0f, sx = 0.0f;
                     if (nl[r] <= kFastLinks) {
 #pragma unroll
                         for (int k = 0; k < kFastLinks; ++k) {
@@ -430,6 +431,7 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
                                 const float v = (cd >= 0) ? lds_x[cd] : sval[-1 - cd];
                                 sum = sum + (nc1[r] * w[r][k]) * v;
                                 in = in + fmaxr(v, a.P.q_lb) * w[r][k];
+                                sx = sx + v * w[r][k];
                             }
                         }
                     } else {
@@ -439,10 +441,12 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
                             const float adj = sadj[e];
                             sum = sum + (nc1[r] * adj) * v;
                             in = in + fmaxr(v, a.P.q_lb) * adj;
+                            sx = sx + v * adj;
                         }
                     }
                     x[r] = unit ? (b[r] - sum) : div_diag(b[r] - sum, 1.0f + nc1[r] * 0.0f);
                     inext[r] = in;
+                    sxn[r] = sx;
                     lds_x[j] = x[r];
                 }
             }
@@ -454,6 +458,13 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
 #pragma unroll
         for (int r = 0; r < RPL; ++r)
             if (my[r] >= 0) publish(row + p0 + r * 64 + lane, x[r], a.epoch);
+        if (a.ihist) {
+#pragma unroll
+            for (int r = 0; r < RPL; ++r)
+                if (my[r] >= 0)
+                    a.ihist[static_cast<size_t>(sg) * P + p0 + r * 64 + lane] =
+                        make_float2(sxn[r], inext[r]);
+        }
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
             if (my[r] >= 0) {
@@ -575,11 +586,13 @@ __device__ __forceinline__ void lin_item(const LinArgs& a, int sb, int p) {
     }
     // state entering the first step: out[t] and N out[t]
     float q, it;
+    const bool have_sums = !Coh && a.ihist != nullptr;
     if (s_first > 0) {
         const uint64_t* hq = a.hist + static_cast<size_t>(s_first - 1) * P;
         bool ok = true;
         float xq = rd(hq + p, ok), sx_unused;
-        inflow(hq, sx_unused, it, ok);
+        if (have_sums) it = a.ihist[static_cast<size_t>(s_first - 1) * P + p].y;
+        else inflow(hq, sx_unused, it, ok);
         if (Coh && !ok) {
             xq = wt(hq + p);
             inflow_wait(hq, sx_unused, it);
@@ -594,7 +607,13 @@ __device__ __forceinline__ void lin_item(const LinArgs& a, int sb, int p) {
         const uint64_t* hx = a.hist + static_cast<size_t>(sg) * P;
         bool ok = true;
         float sx, inext;
-        inflow(hx, sx, inext, ok);
+        if (have_sums) {
+            const float2 u = a.ihist[static_cast<size_t>(sg) * P + p];
+            sx = u.x;
+            inext = u.y;
+        } else {
+            inflow(hx, sx, inext, ok);
+        }
         float xs = rd(hx + p, ok);
         if (Coh && !ok) {
             inflow_wait(hx, sx, inext);
