@@ -65,6 +65,11 @@ KERNEL_NAMES = {"fwd": "k_fwd_window (persistent forward)", "lin": "k_adj_lin (l
 CONFIGS = {
     "c1": dict(R=5000, T=240, window=256, storms=False),
     "c2": dict(R=350000, T=720, window=256, storms=True),
+    # 512 basins of log-uniform size in [10, 50000] side by side, 2160 hourly
+    # steps, the adjoint over checkpoint segments of 256 steps, the loss at
+    # every basin outlet
+    "c3": dict(basins=512, size_lo=10, size_hi=50000, T=2160, window=256, ckpt=256,
+               storms=True),
 }
 
 
@@ -121,12 +126,21 @@ def main():
         torch.cuda.synchronize()
         del a, b
     cfg = CONFIGS[args.config]
-    R, T = cfg["R"], cfg["T"]
-    ci = 1 if args.config == "c1" else 2
+    T = cfg["T"]
+    ci = {"c1": 1, "c2": 2, "c3": 3}[args.config]
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    outlets = None
+    if "basins" in cfg:
+        rs = np.random.default_rng(7000 + ci)
+        sizes = np.exp(rs.uniform(np.log(cfg["size_lo"]), np.log(cfg["size_hi"]),
+                                  cfg["basins"])).astype(np.int64)
+        rows, cols, outlets = NW.gen_forest(sizes, cfg["window"], 1000 + ci)
+        R = int(sizes.sum())
+    else:
+        R = cfg["R"]
+        rows, cols = NW.gen_tree(R, cfg["window"], 1000 + ci)
     log(f"[bench] building network R={R} T={T}")
-    rows, cols = NW.gen_tree(R, cfg["window"], 1000 + ci)
     at = NW.gen_attributes(R, 2000 + ci)
     adj = SparseAdjacency.from_coo(R, rows, cols, None, at["length"], at["slope"])
     t0 = time.time()
@@ -139,7 +153,7 @@ def main():
     qprime = NW.gen_forcing_device(R, T, 3000 + ci, cfg["storms"], device=dev)
     params = D.Params()
     cp = params.to_c()
-    ws = net.workspace(T - 1, T - 1)
+    ws = net.workspace(cfg.get("ckpt") or (T - 1), T - 1)
     log(f"[bench] workspace {ws.nbytes() / 2**30:.2f} GiB")
     stream = torch.cuda.current_stream()
     sptr = stream.cuda_stream
@@ -152,7 +166,10 @@ def main():
     hot.check()
     gn, gqsp = torch.empty(R, device=dev), torch.empty(R, device=dev)
     rng = np.random.default_rng(4000 + ci)
-    gauges_np = np.sort(np.concatenate([[R - 1], rng.choice(R - 1, 63, replace=False)]))
+    if outlets is not None:
+        gauges_np = np.sort(outlets)                      # every basin outlet
+    else:
+        gauges_np = np.sort(np.concatenate([[R - 1], rng.choice(R - 1, 63, replace=False)]))
     ws.set_gauges(gauges_np)
     K = gauges_np.size
     Q = torch.empty((T, K), dtype=torch.float32, device=dev)   # discharge at the gauges
@@ -244,7 +261,9 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(rows, cols, at, R, args.cpu_steps, ci, cfg["storms"], net, params)
+        # bounded sample: about 80 M reach-steps of single-thread oracle work
+        cpu_steps = args.cpu_steps if R * args.cpu_steps <= 1e8 else max(1, int(8e7 // R))
+        cpu = cpu_baseline(rows, cols, at, R, cpu_steps, ci, cfg["storms"], net, params)
 
     out = {
         "metric": "reach-timesteps/s fwd+adjoint (batched vs plain, 1 GPU); grad rel err vs FD",
@@ -259,8 +278,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded generator; no network access)",
-        "config": {"workload": f"{args.config}: {R}-reach tree (window {cfg['window']}), "
-                               f"{T} hourly steps, forward + adjoint, MSE at 64 gauges",
+        "config": {"workload": (f"{args.config}: {cfg['basins']} basins, {R} reaches (sizes "
+                                f"log-uniform {cfg['size_lo']}-{cfg['size_hi']}, generator window "
+                                f"{cfg['window']}), {T} hourly steps, checkpoint window "
+                                f"{cfg['ckpt']}, forward + adjoint, MSE at the "
+                                f"{len(gauges_np)} basin outlets") if outlets is not None else
+                               (f"{args.config}: {R}-reach tree (window {cfg['window']}), "
+                                f"{T} hourly steps, forward + adjoint, MSE at 64 gauges"),
                    "reaches": R, "timesteps": T, "levels": net.n_levels, "form": "persistent",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": roof, "peak": HBM_PEAK_GBS, "unit": "GB/s",

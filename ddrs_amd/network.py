@@ -19,6 +19,23 @@ def gen_tree(n: int, window: int = 256, seed: int = 1001):
     return rows, cols
 
 
+def gen_forest(sizes, window: int = 256, seed: int = 1001):
+    """Independent basins side by side: basin b is gen_tree(sizes[b], window,
+    seed + b) on positions [offset_b, offset_b + sizes[b]). Returns (rows, cols,
+    outlets), outlets[b] the last position of basin b. Still lower-triangular
+    (block diagonal)."""
+    rows, cols, outlets = [], [], []
+    off = 0
+    for b, n in enumerate(int(x) for x in sizes):
+        r, c = gen_tree(n, window, seed + b)
+        rows.append(r + off)
+        cols.append(c + off)
+        off += n
+        outlets.append(off - 1)
+    cat = lambda a: np.concatenate(a).astype(np.int32) if a else np.zeros(0, np.int32)
+    return cat(rows), cat(cols), np.asarray(outlets, np.int32)
+
+
 def gen_attributes(n: int, seed: int = 2001):
     out = [np.zeros(n, np.float32) for _ in range(4)]
     N.check(N.lib.ddrs_gen_attributes(n, seed, *[_ptr(o) for o in out]), "ddrs_gen_attributes")
