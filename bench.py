@@ -70,6 +70,11 @@ CONFIGS = {
     # every basin outlet
     "c3": dict(basins=512, size_lo=10, size_hi=50000, T=2160, window=256, ckpt=256,
                storms=True),
+    # 200,000 basins, Zipf(1.8) sizes capped at 300,000 scaled to 2.94 M reaches,
+    # 8760 hourly steps, checkpoints every 168 steps, the loss at the outlets of
+    # the 2000 largest basins
+    "c4": dict(basins=200_000, zipf=1.8, size_hi=300_000, total=2_940_000, T=8760,
+               window=256, ckpt=168, storms=True, gauges=2000),
 }
 
 
@@ -127,15 +132,22 @@ def main():
         del a, b
     cfg = CONFIGS[args.config]
     T = cfg["T"]
-    ci = {"c1": 1, "c2": 2, "c3": 3}[args.config]
+    ci = {"c1": 1, "c2": 2, "c3": 3, "c4": 4}[args.config]
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     outlets = None
     if "basins" in cfg:
         rs = np.random.default_rng(7000 + ci)
-        sizes = np.exp(rs.uniform(np.log(cfg["size_lo"]), np.log(cfg["size_hi"]),
-                                  cfg["basins"])).astype(np.int64)
+        if "zipf" in cfg:
+            z = np.minimum(rs.zipf(cfg["zipf"], cfg["basins"]), cfg["size_hi"]).astype(np.float64)
+            sizes = np.maximum(1, np.floor(z * cfg["total"] / z.sum())).astype(np.int64)
+            sizes[np.argmax(sizes)] += cfg["total"] - int(sizes.sum())
+        else:
+            sizes = np.exp(rs.uniform(np.log(cfg["size_lo"]), np.log(cfg["size_hi"]),
+                                      cfg["basins"])).astype(np.int64)
         rows, cols, outlets = NW.gen_forest(sizes, cfg["window"], 1000 + ci)
+        if cfg.get("gauges"):
+            outlets = outlets[np.argsort(sizes)[::-1][: cfg["gauges"]]]
         R = int(sizes.sum())
     else:
         R = cfg["R"]
@@ -153,7 +165,8 @@ def main():
     qprime = NW.gen_forcing_device(R, T, 3000 + ci, cfg["storms"], device=dev)
     params = D.Params()
     cp = params.to_c()
-    ws = net.workspace(cfg.get("ckpt") or (T - 1), T - 1)
+    # the step reads out gauges only: no dense dL/dQ buffer
+    ws = D.Workspace(net, cfg.get("ckpt") or (T - 1), T - 1, dense_grad=False)
     log(f"[bench] workspace {ws.nbytes() / 2**30:.2f} GiB")
     stream = torch.cuda.current_stream()
     sptr = stream.cuda_stream
@@ -210,6 +223,12 @@ def main():
         log(f"[bench] step {k + 1}/{args.steps} done")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    free_b, total_b = torch.cuda.mem_get_info()
+    hbm = {"device_used_GiB": (total_b - free_b) / 2**30, "device_total_GiB": total_b / 2**30,
+           "workspace_GiB": ws.nbytes() / 2**30,
+           "note": "device memory in use after the timed steps (workspace, q', outputs, "
+                   "runtime)"}
+    log(f"[bench] HBM in use {hbm['device_used_GiB']:.1f} of {hbm['device_total_GiB']:.1f} GiB")
     ws.check()
     NAT.check(NAT.lib.ddrs_set_timing(ws.handle, 0))
     rts = R * (T - 1)
@@ -279,9 +298,11 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded generator; no network access)",
         "config": {"workload": (f"{args.config}: {cfg['basins']} basins, {R} reaches (sizes "
-                                f"log-uniform {cfg['size_lo']}-{cfg['size_hi']}, generator window "
-                                f"{cfg['window']}), {T} hourly steps, checkpoint window "
-                                f"{cfg['ckpt']}, forward + adjoint, MSE at the "
+                                + (f"Zipf({cfg['zipf']}) capped at {cfg['size_hi']} and scaled to "
+                                   f"{cfg['total']}" if "zipf" in cfg else
+                                   f"log-uniform {cfg['size_lo']}-{cfg['size_hi']}")
+                                + f", generator window {cfg['window']}), {T} hourly steps, "
+                                f"checkpoint window {cfg['ckpt']}, forward + adjoint, MSE at "
                                 f"{len(gauges_np)} basin outlets") if outlets is not None else
                                (f"{args.config}: {R}-reach tree (window {cfg['window']}), "
                                 f"{T} hourly steps, forward + adjoint, MSE at 64 gauges"),
@@ -294,6 +315,7 @@ def main():
                      "note": "latency- and VALU-bound persistent pipeline; see kernels for "
                              "every window kernel",
                      "kernels": kernels, "layout": layout},
+        "hbm": hbm,
         "plain": plain,
         "batched_over_plain": (value / plain["value"]) if plain else None,
         "grad_check": grad_check(torch, D, NW, Network, SparseAdjacency),

@@ -35,6 +35,7 @@ class DdrsParams(ctypes.Structure):
 
 FORM_PLAIN = 0
 FORM_PERSISTENT = 1
+WS_NO_DENSE_GRAD = 1
 
 # name -> (restype, argtypes); restype None means ddrs_status (int32)
 _SIGS = {
@@ -47,6 +48,7 @@ _SIGS = {
     "ddrs_plan_get_csr": (c_int, [P, P, P, P, P, P, P, P, P]),
     "ddrs_plan_get_schedule": (c_int, [P, P, P]),
     "ddrs_workspace_create": (c_int, [P, c_int, c_int, ctypes.POINTER(P)]),
+    "ddrs_workspace_create_ex": (c_int, [P, c_int, c_int, c_int, ctypes.POINTER(P)]),
     "ddrs_workspace_destroy": (None, [P]),
     "ddrs_workspace_bytes": (c_int, [P, ctypes.POINTER(c_size)]),
     "ddrs_workspace_check": (c_int, [P, P]),

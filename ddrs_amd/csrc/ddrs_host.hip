@@ -295,23 +295,47 @@ ddrs_status ddrs_plan_create_ex(int32_t n, int32_t nnz_off, const int32_t* h_row
     // a thin chain is spread over several chunks that pipeline over time
     // steps instead of one chunk walking the whole chain every step. Kernel
     // chunks are 64 or 128 consecutive positions.
+    // A block is also closed early when the next reach's links would overflow
+    // the LDS staging of its 64-position chunk (kStageUpBase upstream and
+    // kStageDnBase downstream links), as in forests of small basins, whose
+    // deeper levels hold outlets with many direct tributaries.
     std::vector<int> inv(R, -1), subl(R, 0);
     {
-        int cur = 0, fill = 0;
+        int cur = 0, fill = 0, half = 0, nup = 0, ndn = 0;
         for (int k = 0; k < R; ++k) {
             const int i = pl->order[k];
+            const int dup = pl->crow[i + 1] - pl->crow[i] - 1;
+            const int ddn = pl->trans_crow[i + 1] - pl->trans_crow[i] - 1;
+            if (dup > kStageUpBase || ddn > kStageDnBase) {
+                delete pl;
+                return fail(DDRS_ERR_CAPACITY, "reach " + std::to_string(i) + " has more than " +
+                                                   std::to_string(kStageUpBase) + " upstream or " +
+                                                   std::to_string(kStageDnBase) +
+                                                   " downstream links");
+            }
             int sl = 0;
             for (int e = pl->crow[i]; e < pl->crow[i + 1]; ++e) {
                 const int u = pl->col[e];
                 if (u != i && (inv[u] >> blk_shift) == cur) sl = std::max(sl, subl[u] + 1);
             }
-            if (fill == blk || sl >= sub_cap) {
+            // the 64-position chunk the reach would land in, and its links so far
+            const int h = (cur * blk + fill) >> 6;
+            if (h != half) {
+                half = h;
+                nup = ndn = 0;
+            }
+            if (fill == blk || sl >= sub_cap || nup + dup > kStageUpBase ||
+                ndn + ddn > kStageDnBase) {
                 ++cur;
                 fill = 0;
                 sl = 0;
+                half = (cur * blk) >> 6;
+                nup = ndn = 0;
             }
             inv[i] = cur * blk + fill;
             subl[i] = sl;
+            nup += dup;
+            ndn += ddn;
             ++fill;
         }
         pl->P = 128 * ((cur * blk / 64 + blk / 64 + 1) / 2);  // whole chunks at both widths
@@ -573,8 +597,8 @@ extern "C" {
 // ---------------------------------------------------------------------------
 // workspace
 
-ddrs_status ddrs_workspace_create(ddrs_plan_t plan, int32_t window, int32_t max_steps,
-                                  ddrs_workspace_t* out) {
+ddrs_status ddrs_workspace_create_ex(ddrs_plan_t plan, int32_t window, int32_t max_steps,
+                                     int32_t flags, ddrs_workspace_t* out) {
     if (!out) return fail(DDRS_ERR_INVALID, "out is NULL");
     *out = nullptr;
     if (!plan) return fail(DDRS_ERR_INVALID, "plan is NULL");
@@ -607,7 +631,8 @@ ddrs_status ddrs_workspace_create(ddrs_plan_t plan, int32_t window, int32_t max_
     A(&ws->rec, WR * sizeof(float4));
     A(&ws->recj, WR * sizeof(float4));
     A(&ws->qps, static_cast<size_t>(std::max(max_steps, 1)) * R * 4);
-    A(&ws->gqs, static_cast<size_t>(std::max(max_steps, 1)) * R * 4);
+    if (!(flags & DDRS_WS_NO_DENSE_GRAD))
+        A(&ws->gqs, static_cast<size_t>(std::max(max_steps, 1)) * R * 4);
     A(&ws->gbuf, WR * 4);
     A(&ws->slot, R * 8);
     A(&ws->claim_ctr, 8);
@@ -668,6 +693,11 @@ void ddrs_workspace_destroy(ddrs_workspace_t ws) {
         if (e) hipEventDestroy(e);
     if (ws->lin_stream) hipStreamDestroy(ws->lin_stream);
     delete ws;
+}
+
+ddrs_status ddrs_workspace_create(ddrs_plan_t plan, int32_t window, int32_t max_steps,
+                                  ddrs_workspace_t* out) {
+    return ddrs_workspace_create_ex(plan, window, max_steps, 0, out);
 }
 
 ddrs_status ddrs_workspace_bytes(ddrs_workspace_t ws, size_t* bytes) {
@@ -1077,6 +1107,9 @@ static ddrs_status window_adj(ddrs_workspace_t ws, const ddrs_params* params, in
     if (permute_qprime(ws, d_q_prime, steps, s) ||
         std::memcmp(&ws->lin_params, params, sizeof(ddrs_params)) != 0)
         ws->lin_seg = -1;  // records of another q' or other parameters
+    if (d_grad_Q && !ws->gqs)
+        return fail(DDRS_ERR_INVALID, "the workspace has no dense dL/dQ buffer "
+                                      "(DDRS_WS_NO_DENSE_GRAD): use the _gauged adjoint");
     if (d_grad_Q) launch_permute_rows(sc, d_grad_Q, 0, steps, ws->gqs, s);
     const float* g_last = d_grad_Q ? d_grad_Q + static_cast<size_t>(T - 1) * R : nullptr;
     if (d_grad_Qg) {

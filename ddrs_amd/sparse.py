@@ -117,13 +117,16 @@ class Network:
 
 
 class Workspace:
-    def __init__(self, net: Network, window: int, max_steps: int):
+    def __init__(self, net: Network, window: int, max_steps: int, dense_grad: bool = True):
+        """dense_grad=False leaves out the [max_steps][n] dense dL/dQ buffer
+        (only the gauged adjoint works then)."""
         self.net = net
         self.window = window
         self.max_steps = max_steps
         h = ctypes.c_void_p()
-        N.check(N.lib.ddrs_workspace_create(net.handle, window, max_steps, ctypes.byref(h)),
-                "ddrs_workspace_create")
+        flags = 0 if dense_grad else N.WS_NO_DENSE_GRAD
+        N.check(N.lib.ddrs_workspace_create_ex(net.handle, window, max_steps, flags,
+                                               ctypes.byref(h)), "ddrs_workspace_create_ex")
         self.handle = h
         self.fwd_token = None
 

@@ -105,6 +105,13 @@ ddrs_status ddrs_plan_get_schedule(ddrs_plan_t plan, int32_t* h_perm, int32_t* h
  * ------------------------------------------------------------------------- */
 ddrs_status ddrs_workspace_create(ddrs_plan_t plan, int32_t window, int32_t max_steps,
                                   ddrs_workspace_t* out);
+/* As ddrs_workspace_create with flags: DDRS_WS_NO_DENSE_GRAD leaves out the
+ * [max_steps][n] buffer of a dense dL/dQ (only the _gauged adjoint is then
+ * available; the dense one returns DDRS_ERR_INVALID). For windows whose q'
+ * alone fills a large part of HBM. No reference counterpart. */
+enum { DDRS_WS_NO_DENSE_GRAD = 1 };
+ddrs_status ddrs_workspace_create_ex(ddrs_plan_t plan, int32_t window, int32_t max_steps,
+                                     int32_t flags, ddrs_workspace_t* out);
 void ddrs_workspace_destroy(ddrs_workspace_t ws);
 /* Bytes of device memory the workspace holds. */
 ddrs_status ddrs_workspace_bytes(ddrs_workspace_t ws, size_t* bytes);

@@ -70,6 +70,31 @@ def test_schedule_is_level_major_topological():
     assert net.n_levels == lev.max() + 1
 
 
+def test_forest_of_small_basins_fits_chunk_staging():
+    """Small basins put outlets with several direct tributaries on deeper
+    levels; the schedule closes a chunk before its upstream links overflow the
+    192-link staging (a plan that used to be refused with DDRS_ERR_CAPACITY)."""
+    from ddrs_amd import network as NW
+    rng = np.random.default_rng(9)
+    sizes = np.minimum(rng.zipf(1.8, 2000), 3000).astype(np.int64)
+    rows, cols, outlets = NW.gen_forest(sizes, 256, 77)
+    # in plain level-major order some 64 consecutive reaches hold > 192 upstream links
+    indeg = np.bincount(rows, minlength=int(sizes.sum()))
+    lev = np.zeros(int(sizes.sum()), np.int64)
+    for j in np.argsort(rows, kind="stable"):
+        lev[rows[j]] = max(lev[rows[j]], lev[cols[j]] + 1)
+    lm = np.argsort(lev, kind="stable")
+    assert max(indeg[lm[k:k + 64]].sum() for k in range(0, lm.size, 64)) > 192
+    R = int(sizes.sum())
+    net, _ = _plan_csr(R, rows, cols)
+    perm, level = net.schedule()
+    pos = np.empty(R, np.int64)
+    pos[perm] = np.arange(R)
+    assert (pos[cols] < pos[rows]).all() and (level[cols] < level[rows]).all()
+    assert net.n_chunks >= int(np.ceil(rows.size / 192))
+    assert (outlets == np.cumsum(sizes) - 1).all()
+
+
 def test_plan_rejects_bad_adjacency():
     from ddrs_amd.sparse import Network, SparseAdjacency
     from ddrs_amd._native import DdrsError
