@@ -50,16 +50,16 @@ LOSS_SHAPE = dict(R=350_000, T=8760, G=2000, tau=9, warmup=30)
 # Algorithmic HBM bytes per reach-timestep of each window kernel, with the
 # bench's gauge readout (Q written and dL/dQ read only at the 64 gauges):
 #   forward window: q' read 4, x history written 8 and read back 8 by the
-#     downstream chunks (one link per reach in a tree); a dense Q output would
-#     add 4 written;
-#   linearise: x history of the reach and of its upstream link 2 x 8 (the
-#     previous step's words are the ones read one step earlier), q' 4;
+#     downstream chunks (one link per reach in a tree), the upstream sums
+#     (float2) written 8 for the linearisation; a dense Q output would add 4
+#     written;
+#   linearise: the reach's x history 8 and its upstream sums 8, q' 4;
 #     records written 16 + 16;
 #   sweep: records read 16, (P, G) published 16 and read 16 by the upstream
 #     chunk, gb written 4; a dense dL/dQ would add 4 read;
 #   finish: records read 16, gb read 4 (this bench's loss asks for no dL/dq',
 #     which would add 4 written).
-KERNEL_BYTES = {"fwd": 20, "lin": 52, "sweep": 52, "fin": 20}
+KERNEL_BYTES = {"fwd": 28, "lin": 52, "sweep": 52, "fin": 20}
 KERNEL_NAMES = {"fwd": "k_fwd_window (persistent forward)", "lin": "k_adj_lin (linearise)",
                 "sweep": "k_adj_sweep (persistent reverse sweep)", "fin": "k_adj_fin"}
 CONFIGS = {
