@@ -75,6 +75,12 @@ CONFIGS = {
     # the 2000 largest basins
     "c4": dict(basins=200_000, zipf=1.8, size_hi=300_000, total=2_940_000, T=8760,
                window=256, ckpt=168, storms=True, gauges=2000),
+    # training micro-batch: 2000 nested gauges (basins log-uniform 10-50000) on
+    # a 350,000-reach tree, the union of their basins compressed to the batch,
+    # 365 days hourly; KAN head -> routing -> daily NSE -> adjoint -> KAN
+    # backward -> AdaDelta; the routing adjoint in one segment (ckpt None)
+    "c5": dict(R=350_000, window=256, gauges=2000, size_lo=10, size_hi=50_000, T=8760,
+               ckpt=None, storms=True, tau=9, warmup=5, eps=0.1, noise=0.1),
 }
 
 
@@ -122,6 +128,10 @@ def main():
         gl = loss_bench(torch, args.steps, args.warmup, not args.no_cpu)
         print(json.dumps({"metric": "gauge-hours/s fused readout + loss + gradient", **gl}),
               flush=True)
+        return
+
+    if args.config == "c5":
+        print(json.dumps(train_bench(args, torch)), flush=True)
         return
 
     if args.pmc_calibration:
@@ -570,6 +580,177 @@ def loss_bench(torch, steps, warmup, with_cpu):
     del Q, gQ
     torch.cuda.empty_cache()
     return res
+
+
+def train_bench(args, torch):
+    """The training workload: one optimizer step of ddrs_amd.train.Trainer per
+    bench step on a fixed micro-batch (HIP KAN head, gauged routing window,
+    fused daily NSE, adjoint, KAN backward, clipping and AdaDelta on the
+    device). Reach-timesteps/s of the whole step; the routing kernels' share
+    from HIP events; the plain per-timestep-launch form of the same step once,
+    with its gradients checked bitwise against the persistent form's."""
+    import ddrs_amd as D
+    from ddrs_amd import _native as NAT
+    from ddrs_amd import kan as K
+    from ddrs_amd import loss as LS
+    from ddrs_amd import network as NW
+    from ddrs_amd import train as TR
+    from ddrs_amd.sparse import Network, SparseAdjacency
+
+    cfg = CONFIGS["c5"]
+    T, R0, ci = cfg["T"], cfg["R"], 5
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    t0 = time.time()
+    rows, cols = NW.gen_tree(R0, cfg["window"], 1000 + ci)
+    g_full = TR.nested_gauges(R0, rows, cols, cfg["gauges"], cfg["size_lo"], cfg["size_hi"],
+                              seed=7000 + ci)
+    basin = TR.upstream_sizes(R0, rows, cols)[g_full]
+    ur, uc = TR.upstream_union(R0, rows, cols, g_full)
+    active, cr, cc, cg = NW.compress(ur, uc, g_full)
+    R = int(active.size)
+    at0 = NW.gen_attributes(R0, 2000 + ci)
+    at = {k: np.ascontiguousarray(v[active]) for k, v in at0.items()}
+    adj = SparseAdjacency.from_coo(R, cr, cc, None, at["length"], at["slope"])
+    net = Network(adj)
+    log(f"[bench] c5 batch: {R} of {R0} reaches, {cg.size} gauges (basins {basin.min()}-"
+        f"{basin.max()}), {net.n_levels} levels; host setup {time.time() - t0:.1f}s")
+    gen = torch.Generator(device=dev).manual_seed(5000 + ci)
+    names = [f"attr_{i}" for i in range(KAN_SHAPE["F"])]
+    x = torch.randn(R, len(names), generator=gen, device=dev)
+    qprime = NW.gen_forcing_device(R, T, 3000 + ci, cfg["storms"], device=dev)
+    mk = lambda seed: K.KanHeadConfig(names, ["n", "q_spatial"], seed,
+                                      hidden_size=KAN_SHAPE["H"],
+                                      num_hidden_layers=KAN_SHAPE["L"], grid=KAN_SHAPE["grid"],
+                                      k=KAN_SHAPE["k"]).init(dev)
+    tau, wu, eps = cfg["tau"], cfg["warmup"], cfg["eps"]
+    # observations: a hidden head's routed discharge at the gauges, daily
+    # pooled, times LogNormal(0, 0.1) noise; sigma is each gauge's std
+    truth = TR.Trainer(mk(4242), tau=tau, warmup=wu, eps=eps, window=cfg["ckpt"], network=net)
+    probe = TR.GaugedBatch(adj, x, qprime, cg, None, None)
+    with torch.no_grad():
+        Qg = truth.predict(probe)
+        daily = LS.tau_trim_and_downsample(Qg.t().contiguous(), tau)
+        obs = daily * torch.exp(cfg["noise"] * torch.randn(daily.shape, generator=gen, device=dev))
+        sigma = obs.std(dim=1)
+    del truth, Qg, daily
+    batch = TR.GaugedBatch(adj, x, qprime, cg, obs.contiguous(), sigma.contiguous())
+    head = mk(42)
+    tr = TR.Trainer(head, tau=tau, warmup=wu, eps=eps, window=cfg["ckpt"], network=net)
+    log(f"[bench] c5 observations ready ({time.time() - t0:.1f}s)")
+    first = tr.step(batch)
+    for _ in range(max(args.warmup - 1, 0)):
+        tr.step(batch)
+    ws = next(v for k, v in net._ws.items() if k[0] == "gauged")
+    log(f"[bench] c5 workspace {ws.nbytes() / 2**30:.1f} GiB, first loss {first.loss:.5f}")
+    NAT.check(NAT.lib.ddrs_set_timing(ws.handle, 1))
+    fwd_ms = adj_ms = 0.0
+    phase_ms = np.zeros(3)
+    losses = []
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for k in range(args.steps):
+        out = tr.step(batch)
+        losses.append(out.loss)
+        a = [np.zeros(1, np.int32), np.zeros(1, np.float32), np.zeros(1, np.int32),
+             np.zeros(1, np.float32)]
+        NAT.check(NAT.lib.ddrs_get_timing(ws.handle, *[v.ctypes.data for v in a]))
+        fwd_ms += float(a[1][0])
+        adj_ms += float(a[3][0])
+        ph = [np.zeros(1, np.float32) for _ in range(3)]
+        NAT.check(NAT.lib.ddrs_get_adj_phase_timing(ws.handle, *[v.ctypes.data for v in ph]))
+        phase_ms += [float(v[0]) for v in ph]
+        log(f"[bench] c5 step {k + 1}/{args.steps}: loss {out.loss:.5f}, grad norm "
+            f"{out.grad_norm:.4g}")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t1
+    NAT.check(NAT.lib.ddrs_set_timing(ws.handle, 0))
+    free_b, total_b = torch.cuda.mem_get_info()
+    rs = R * (T - 1)
+    value = rs * args.steps / elapsed
+    ms_per_step = 1000.0 * elapsed / args.steps
+    phase_ms /= args.steps
+    launch_ms = {"fwd": fwd_ms / args.steps, "lin": phase_ms[0], "sweep": phase_ms[1],
+                 "fin": phase_ms[2]}
+    kernels = {kk: {"launch_ms": v, "bytes_per_reach_step": KERNEL_BYTES[kk],
+                    "achieved_GBs": KERNEL_BYTES[kk] * rs / (v * 1e-3) / 1e9}
+               for kk, v in launch_ms.items()}
+    dom = max(launch_ms, key=launch_ms.get)
+    routing_ms = sum(launch_ms.values())
+    log(f"[bench] c5 persistent: {ms_per_step:.1f} ms/step, {value / 1e9:.3f} G reach-ts/s; "
+        f"routing kernels {routing_ms:.1f} ms (fwd {launch_ms['fwd']:.1f}, lin "
+        f"{phase_ms[0]:.1f}, sweep {phase_ms[1]:.1f}, fin {phase_ms[2]:.1f})")
+
+    def grads_of(form, step=False):
+        tr.form = form
+        for prm in head.parameters():
+            prm.grad = None
+        loss, _ = tr.loss(batch)
+        loss.backward()
+        g = [prm.grad.clone() for prm in head.parameters()]
+        if step:
+            TR.clip_grad_norm(head.parameters(), tr.grad_clip)
+            tr.opt.step()
+        return float(loss.detach()), g
+
+    plain = None
+    if not args.no_plain:
+        lp, gp = grads_of(NAT.FORM_PERSISTENT)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        lq, gq = grads_of(NAT.FORM_PLAIN, step=True)
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t2
+        same = lp == lq and all(torch.equal(u, v) for u, v in zip(gp, gq))
+        plain = dict(value=rs / tp, ms_per_step=1000 * tp, bit_identical_to_persistent=same)
+        log(f"[bench] c5 plain: {1000 * tp:.0f} ms/step ({value * tp / rs:.1f}x slower), "
+            f"identical={same}")
+        tr.form = NAT.FORM_PERSISTENT
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(cr, cc, at, R, max(1, int(8e7 // R)), ci, cfg["storms"], net,
+                           D.Params())
+        cpu["sample"] += " (routing forward + adjoint; the KAN head and loss add <1% there)"
+    roof = kernels[dom]["achieved_GBs"]
+    return {
+        "metric": "reach-timesteps/s fwd+adjoint (batched vs plain, 1 GPU); grad rel err vs FD",
+        "value": value,
+        "unit": "reach-timesteps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generators; observations from a hidden KAN head's routed "
+                "discharge x LogNormal(0, 0.1); no network access)",
+        "config": {"workload": (f"c5: training micro-batch, {cg.size} nested gauges (basins "
+                                f"log-uniform {cfg['size_lo']}-{cfg['size_hi']} reaches) on a "
+                                f"{R0}-reach tree (window {cfg['window']}), the union of their "
+                                f"basins = {R} reaches, {T} hourly steps; KAN head (10 -> 21, "
+                                f"2 blocks, grid 5, order 3 -> n, q_spatial) -> routing "
+                                f"forward -> tau {tau} trim + daily pooling -> warmup {wu} d -> "
+                                f"batch NSE -> adjoint -> KAN backward -> clip 1.0 -> AdaDelta"),
+                   "reaches": R, "timesteps": T, "gauges": int(cg.size),
+                   "levels": net.n_levels, "checkpoint_window": cfg["ckpt"] or (T - 1),
+                   "form": "persistent", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": roof, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": roof / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": KERNEL_NAMES[dom], "launch_ms": launch_ms[dom],
+                     "bytes_per_reach_step": KERNEL_BYTES[dom], "reach_steps_per_launch": rs,
+                     "kernels": kernels, "routing_kernels_ms": routing_ms,
+                     "other_ms": ms_per_step - routing_ms,
+                     "note": "other_ms: KAN head forward and backward, hotstart, loss, "
+                             "optimizer, host synchronisation of the loss"},
+        "losses": losses,
+        "hbm": {"device_used_GiB": (total_b - free_b) / 2**30,
+                "device_total_GiB": total_b / 2**30, "workspace_GiB": ws.nbytes() / 2**30},
+        "plain": plain,
+        "batched_over_plain": (value / plain["value"]) if plain else None,
+        "cpu_baseline": cpu,
+    }
 
 
 def cpu_baseline(rows, cols, at, R, steps, ci, storms, net, params):

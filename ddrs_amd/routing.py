@@ -17,6 +17,7 @@ import math
 from dataclasses import dataclass
 from typing import Optional
 
+import numpy as np
 import torch
 
 from . import _native as N
@@ -172,6 +173,84 @@ def route_window(net: Network, params: Params, n, q_spatial, p_spatial, q_prime,
     window = max(T - 1, 1) if window is None else int(window)
     return _Window.apply(n, q_spatial, p_spatial, q_prime, q0, net, params, length, slope,
                          x_storage, form, window)
+
+
+class _WindowGauged(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, n, qsp, p, q_prime, q0, net, params, length, slope, xst, form, window,
+                gauges):
+        T = int(q_prime.shape[0])
+        ws = _gauge_workspace(net, window, max(T - 1, 1), gauges)
+        cp, keep, xk = _bind(ws, params, n, qsp, p, length, slope, xst)
+        qp = _dev_f32(q_prime, T * net.n)
+        q0d = _dev_f32(q0, net.n)
+        K = int(ws.gauges.size)
+        Qg = torch.empty((T, K), dtype=torch.float32, device=qp.device)
+        N.check(N.lib.ddrs_mc_window_fwd_gauged(ws.handle, cp, T, _ptr(qp), _ptr(q0d), _ptr(Qg),
+                                                form, _stream()), "ddrs_mc_window_fwd_gauged")
+        ws.check()
+        token = next(_tokens)
+        ws.fwd_token = token
+        ctx.token, ctx.net, ctx.params, ctx.form, ctx.window, ctx.T = token, net, params, form, window, T
+        ctx.gauges = ws.gauges
+        ctx.has_x = xk is not None
+        ctx.save_for_backward(*keep, qp, q0d, *([xk] if xk is not None else []))
+        return Qg
+
+    @staticmethod
+    def backward(ctx, gQg):
+        saved = ctx.saved_tensors
+        n, qsp, p, length, slope, qp, q0d = saved[:7]
+        xst = saved[7] if ctx.has_x else None
+        net, T = ctx.net, ctx.T
+        ws = _gauge_workspace(net, ctx.window, max(T - 1, 1), ctx.gauges)
+        cp, _, _ = _bind(ws, ctx.params, n, qsp, p, length, slope, xst)
+        if ws.fwd_token != ctx.token:
+            N.check(N.lib.ddrs_mc_window_fwd(ws.handle, cp, T, _ptr(qp), _ptr(q0d), None,
+                                             N.FORM_PERSISTENT, _stream()), "ddrs_mc_window_fwd")
+            ws.fwd_token = ctx.token
+        g = _dev_f32(gQg, T * int(ctx.gauges.size))
+        R = net.n
+        gn = torch.empty(R, dtype=torch.float32, device=g.device)
+        gqsp, gp = torch.empty_like(gn), torch.empty_like(gn)
+        need_qp, need_q0 = ctx.needs_input_grad[3], ctx.needs_input_grad[4]
+        gqp = torch.empty((T, R), dtype=torch.float32, device=g.device) if need_qp else None
+        gq0 = torch.empty_like(gn) if need_q0 else None
+        N.check(N.lib.ddrs_mc_window_adj_gauged(ws.handle, cp, T, _ptr(qp), _ptr(q0d), _ptr(g),
+                                                _ptr(gn), _ptr(gqsp), _ptr(gp), _ptr(gqp),
+                                                _ptr(gq0), ctx.form, _stream()),
+                "ddrs_mc_window_adj_gauged")
+        ws.check()
+        return gn, gqsp, gp, gqp, gq0, None, None, None, None, None, None, None, None
+
+
+def _gauge_workspace(net: Network, window: int, max_steps: int, gauges) -> "Workspace":
+    """A workspace of `net` without the dense dL/dQ buffer, its gauges set to
+    `gauges` (kept while unchanged)."""
+    from .sparse import Workspace
+    key = ("gauged", int(window), int(max_steps))
+    ws = net._ws.get(key)
+    if ws is None:
+        ws = Workspace(net, int(window), int(max_steps), dense_grad=False)
+        net._ws[key] = ws
+    g = np.ascontiguousarray(gauges, np.int32)
+    if getattr(ws, "gauges", None) is None or not np.array_equal(ws.gauges, g):
+        ws.set_gauges(g)
+    return ws
+
+
+def route_window_gauged(net: Network, params: Params, n, q_spatial, p_spatial, q_prime, q0,
+                        length, slope, gauges, x_storage=None, form: int = N.FORM_PERSISTENT,
+                        window: Optional[int] = None) -> torch.Tensor:
+    """Q at the gauge reaches only, [T, K] with K = len(gauges) (reference
+    reach ids): the same values as route_window(...)[:, gauges], computed
+    without writing the full [T, R] discharge; its backward gives the same
+    gradients as route_window's for a loss that reads Q only at the gauges
+    (dL/dq' and dL/dq0 only when those inputs require gradients)."""
+    T = int(q_prime.shape[0])
+    window = max(T - 1, 1) if window is None else int(window)
+    return _WindowGauged.apply(n, q_spatial, p_spatial, q_prime, q0, net, params, length, slope,
+                               x_storage, form, window, gauges)
 
 
 @dataclass

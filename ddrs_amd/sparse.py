@@ -108,8 +108,9 @@ class Network:
 
     def workspace(self, window: int = 1, max_steps: int = 1) -> "Workspace":
         key = (int(window), int(max_steps))
-        for (w, m), ws in self._ws.items():
-            if w >= key[0] and m >= key[1]:
+        for k, ws in self._ws.items():
+            # skip the gauge-readout workspaces (no dense dL/dQ buffer)
+            if len(k) == 2 and k[0] >= key[0] and k[1] >= key[1]:
                 return ws
         ws = Workspace(self, *key)
         self._ws[key] = ws

@@ -463,6 +463,49 @@ def test_gauged_window_identical(form):
         ws.set_gauges([1, 1])
 
 
+def test_route_window_gauged_autograd():
+    """route_window_gauged through autograd gives route_window's bits for a
+    loss that reads Q only at the gauges: its output is Q[:, gauges] and every
+    input gradient is identical. Also after another window ran on the same
+    workspace in between (the backward recomputes the forward), and with q'
+    and q0 not requiring gradients (those are not computed)."""
+    say("route_window_gauged autograd 5000 x 50")
+    c = Case(5000, 50, seed=61, storms=True)
+    rng = np.random.default_rng(62)
+    gauges = np.sort(np.concatenate([[c.R - 1], rng.choice(c.R - 1, 19, replace=False)]))
+    w = t(rng.normal(0, 1, (c.T, gauges.size)).astype(np.float32))
+    params = to_params(c.P)
+
+    def run(gauged, grad_forcing=True, other_between=False, window=11):
+        n, qsp, p = (t(v).requires_grad_(True) for v in (c.n, c.qsp, c.p))
+        qp = t(c.qprime).requires_grad_(grad_forcing)
+        q0 = t(c.q0()).requires_grad_(grad_forcing)
+        args = (c.net, params, n, qsp, p, qp, q0, t(c.length), t(c.slope_raw))
+        if gauged:
+            Qg = D.route_window_gauged(*args, gauges, window=window)
+        else:
+            Qg = D.route_window(*args, window=window)[:, torch.as_tensor(gauges, device="cuda")]
+        if other_between:
+            with torch.no_grad():
+                D.route_window_gauged(c.net, params, *(t(v) for v in (c.n * 1.1, c.qsp, c.p)),
+                                      t(c.qprime * 0.5), t(c.q0()), t(c.length),
+                                      t(c.slope_raw), gauges, window=window)
+        (Qg * w).sum().backward()
+        out = {"Qg": Qg.detach().cpu().numpy(), "n": n.grad.cpu().numpy(),
+               "q_spatial": qsp.grad.cpu().numpy(), "p_spatial": p.grad.cpu().numpy()}
+        if grad_forcing:
+            out.update(q_prime=qp.grad.cpu().numpy(), q0=q0.grad.cpu().numpy())
+        else:
+            assert qp.grad is None and q0.grad is None
+        return out
+
+    ref = run(False)
+    for kw in (dict(), dict(other_between=True), dict(grad_forcing=False)):
+        got = run(True, **kw)
+        for k in got:
+            np.testing.assert_array_equal(got[k], ref[k], err_msg=f"{kw}: {k}")
+
+
 @pytest.mark.skip(reason="linearising in the forward window is disabled in the library: a "
                          "350000-reach, 720-step window with the concurrent launch did not "
                          "finish on the GPU and the cause is not known")

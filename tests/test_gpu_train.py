@@ -1,0 +1,132 @@
+"""GPU parity of one training micro-batch (ddrs_amd.train.Trainer: KAN head ->
+denormalize -> gauged routing window -> tau trim, daily pooling and batch NSE
+-> routing adjoint -> KAN backward -> clipping -> AdaDelta) against the oracle
+chain of tests/train_case.py (KAN oracle in float64, routing oracle in f32,
+loss oracle in float64 after the f32 readout), which a CPU test pins to
+float64 finite differences. Needs an MI355X.
+
+Workload: a 4000-reach synthetic tree, 16 nested gauges of log-uniform basin
+size, the union of their basins compressed to a batch (as the reference's
+collate does), 240 hourly steps, observations from a hidden head's routed
+discharge with 10% lognormal noise and one missing day (so one gauge is
+dropped by the NaN filter).
+
+The stages meet on the device's n and q_spatial: the oracle routes the
+device head's denormalised outputs (and takes its KAN backward from its own
+float64 forward). Fed its own head's outputs instead, which differ from the
+device's by an ulp, the chain's weight gradients move by up to ~1e-3 max|g|:
+perturbing the oracle's own n and q_spatial by one ulp moves its f32 weight
+gradients by 7.5e-4 max|g| on this batch (reach states near the routing's
+clamps, summed over reaches with 1-2% cancellation). That end-to-end
+difference is printed, not asserted.
+
+Bars:
+* the loss within 1e-4 relative of the oracle chain, n_valid exact;
+* each head gradient tensor within 1e-3 max|g| of the oracle chain (the
+  routing gradient bar; the KAN and loss parts are far tighter);
+* Trainer.step equals clip_grad_norm + AdaDelta applied on the CPU to the same
+  device gradients (torch.optim.Adadelta, which the reference matches), within
+  1e-6 relative;
+* three more steps on the batch lower the loss.
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from tests import train_case as TC
+
+pytestmark = pytest.mark.gpu
+
+if not torch.cuda.is_available():  # collected on the CPU too, skipped there
+    pytest.skip("needs a GPU", allow_module_level=True)
+
+from ddrs_amd import kan as K  # noqa: E402
+from ddrs_amd import train as TR  # noqa: E402
+
+DEV = "cuda:0"
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def t(x):
+    return torch.as_tensor(np.ascontiguousarray(x, np.float32), device=DEV)
+
+
+def make_head(seed):
+    return K.KanHeadConfig([f"a{i}" for i in range(TC.F)], TC.NAMES, seed).init(DEV)
+
+
+def device_batch(b):
+    return TR.GaugedBatch(b.adj_obj, t(b.x), t(b.qprime), b.gauges, t(b.obs), t(b.sigma))
+
+
+def make_batch(seed):
+    from ddrs_amd.sparse import SparseAdjacency
+    b = TC.Batch(seed=seed)
+    b.observe(TC.oracle_params(make_head(99 + seed)), 3)
+    b.adj_obj = SparseAdjacency.from_coo(b.R, b.rows, b.cols, None, b.length, b.slope)
+    return b
+
+
+@pytest.mark.parametrize("window", [None, 50])
+def test_micro_batch_matches_oracle_chain(window):
+    say("training micro-batch vs oracle chain, window", window)
+    b = make_batch(0)
+    head = make_head(42)
+    tr = TR.Trainer(head, tau=b.tau, warmup=b.warmup, window=window)
+    batch = device_batch(b)
+    loss, nv = tr.loss(batch)
+    loss.backward()
+    with torch.no_grad():
+        out = head(batch.attributes)
+        rng = tr.params.parameter_ranges
+        nq = [(out[k] * (r[1] - r[0]) + r[0]).cpu().numpy()
+              for k, r in (("n", rng.n), ("q_spatial", rng.q_spatial))]
+    p = TC.oracle_params(head)
+    wl, wnv, wg = b.oracle_loss_and_grads(p, head.k, nq=nq)
+    _, _, own = b.oracle_loss_and_grads(p, head.k, dtype=np.float64)
+    say(f"R={b.R} gauges={b.gauges.size}: loss {loss.item():.8g} oracle {wl:.8g}; "
+        f"n_valid {int(nv)} oracle {wnv}")
+    assert int(nv) == wnv == (b.gauges.size - 1) * ((b.T - 24) // 24 - b.warmup)
+    assert abs(loss.item() - wl) <= 1e-4 * abs(wl)
+    for name, want in wg.items():
+        got = getattr(head, name).grad.double().cpu().numpy()
+        scale = max(float(np.abs(want).max()), 1e-30)
+        err = float(np.abs(got - want).max()) / scale
+        e_own = float(np.abs(got - own[name]).max()) / float(np.abs(own[name]).max())
+        print(f"  grad {name}: max err / max|g| {err:.3e} (vs the float64 chain through the "
+              f"oracle head's own outputs {e_own:.3e})")
+        assert err <= 1e-3, name
+
+
+def test_step_is_clip_plus_adadelta_and_descends():
+    say("training step = clip + AdaDelta; descent")
+    b = make_batch(1)
+    batch = device_batch(b)
+    ha, hb = make_head(42), make_head(42)
+    ta = TR.Trainer(ha, tau=b.tau, warmup=b.warmup)
+    tb = TR.Trainer(hb, tau=b.tau, warmup=b.warmup, network=ta.network(batch))
+    # reference: the device gradients, clipped and stepped on the CPU
+    loss, _ = ta.loss(batch)
+    loss.backward()
+    cpu = [torch.nn.Parameter(p.detach().cpu().clone()) for p in ha.parameters()]
+    for c, p in zip(cpu, ha.parameters()):
+        c.grad = p.grad.detach().cpu().clone()
+    norm = TR.clip_grad_norm(cpu, 1.0)
+    torch.optim.Adadelta(cpu).step()
+    out = tb.step(batch)
+    assert out.grad_norm == pytest.approx(norm, rel=1e-6)
+    assert out.loss == pytest.approx(loss.item(), rel=1e-7)
+    for c, p in zip(cpu, hb.parameters()):
+        np.testing.assert_allclose(p.detach().cpu().numpy(), c.detach().numpy(), rtol=1e-6,
+                                   atol=1e-9)
+    losses = [out.loss]
+    for _ in range(3):
+        losses.append(tb.step(batch).loss)
+    say("losses", losses)
+    assert losses[-1] < losses[0]
