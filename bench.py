@@ -612,7 +612,7 @@ def train_bench(args, torch):
     at0 = NW.gen_attributes(R0, 2000 + ci)
     at = {k: np.ascontiguousarray(v[active]) for k, v in at0.items()}
     adj = SparseAdjacency.from_coo(R, cr, cc, None, at["length"], at["slope"])
-    net = Network(adj)
+    net = Network(adj, args.sublevels, args.block)
     log(f"[bench] c5 batch: {R} of {R0} reaches, {cg.size} gauges (basins {basin.min()}-"
         f"{basin.max()}), {net.n_levels} levels; host setup {time.time() - t0:.1f}s")
     gen = torch.Generator(device=dev).manual_seed(5000 + ci)
