@@ -78,7 +78,8 @@ CONFIGS = {
     # training micro-batch: 2000 nested gauges (basins log-uniform 10-50000) on
     # a 350,000-reach tree, the union of their basins compressed to the batch,
     # 365 days hourly; KAN head -> routing -> daily NSE -> adjoint -> KAN
-    # backward -> AdaDelta; the routing adjoint in one segment (ckpt None)
+    # backward -> AdaDelta; the autotuner picks the schedule and the checkpoint
+    # window (one segment when it fits)
     "c5": dict(R=350_000, window=256, gauges=2000, size_lo=10, size_hi=50_000, T=8760,
                ckpt=None, storms=True, tau=9, warmup=5, eps=0.1, noise=0.1),
 }
@@ -101,6 +102,9 @@ def main():
     ap.add_argument("--sublevels", type=int, default=0,
                     help="largest chain of links in one schedule block (0: library default)")
     ap.add_argument("--block", type=int, default=0, help="schedule block, 64 or 128 (0: default)")
+    ap.add_argument("--tune", action="store_true",
+                    help="choose the schedule's sub-level cap and the checkpoint window with the "
+                         "autotuner (always on for c5 unless --sublevels is given)")
     ap.add_argument("--kan-only", action="store_true", help="only the KAN head measurement")
     ap.add_argument("--no-kan", action="store_true", help="skip the KAN head measurement")
     ap.add_argument("--loss-only", action="store_true", help="only the gauge loss measurement")
@@ -166,17 +170,24 @@ def main():
     at = NW.gen_attributes(R, 2000 + ci)
     adj = SparseAdjacency.from_coo(R, rows, cols, None, at["length"], at["slope"])
     t0 = time.time()
-    net = Network(adj, args.sublevels, args.block)
-    log(f"[bench] plan {time.time() - t0:.2f}s: {net.n_chunks} chunks, {net.n_levels} levels")
     f = lambda x: torch.as_tensor(np.ascontiguousarray(x, np.float32), device=dev)
     n, qsp = f(at["n"]), f(at["q_spatial"])
     p = torch.full((R,), 21.0, dtype=torch.float32, device=dev)
     length, slope = f(at["length"]), f(at["slope"])
     qprime = NW.gen_forcing_device(R, T, 3000 + ci, cfg["storms"], device=dev)
+    rng = np.random.default_rng(4000 + ci)
+    if outlets is not None:
+        gauges_np = np.sort(outlets)                      # every basin outlet
+    else:
+        gauges_np = np.sort(np.concatenate([[R - 1], rng.choice(R - 1, 63, replace=False)]))
+    net, window, schedule = schedule_for(args, adj, T, qprime, gauges_np, cfg.get("ckpt"),
+                                         tune=args.tune)
+    log(f"[bench] plan {time.time() - t0:.2f}s: {net.n_chunks} chunks, {net.n_levels} levels; "
+        f"schedule {schedule}")
     params = D.Params()
     cp = params.to_c()
     # the step reads out gauges only: no dense dL/dQ buffer
-    ws = D.Workspace(net, cfg.get("ckpt") or (T - 1), T - 1, dense_grad=False)
+    ws = D.Workspace(net, window or (T - 1), T - 1, dense_grad=False)
     log(f"[bench] workspace {ws.nbytes() / 2**30:.2f} GiB")
     stream = torch.cuda.current_stream()
     sptr = stream.cuda_stream
@@ -188,11 +199,6 @@ def main():
     NAT.check(NAT.lib.ddrs_hotstart(hot.handle, cp, ptr(qprime[0]), ptr(q0), sptr))
     hot.check()
     gn, gqsp = torch.empty(R, device=dev), torch.empty(R, device=dev)
-    rng = np.random.default_rng(4000 + ci)
-    if outlets is not None:
-        gauges_np = np.sort(outlets)                      # every basin outlet
-    else:
-        gauges_np = np.sort(np.concatenate([[R - 1], rng.choice(R - 1, 63, replace=False)]))
     ws.set_gauges(gauges_np)
     K = gauges_np.size
     Q = torch.empty((T, K), dtype=torch.float32, device=dev)   # discharge at the gauges
@@ -312,11 +318,12 @@ def main():
                                    f"{cfg['total']}" if "zipf" in cfg else
                                    f"log-uniform {cfg['size_lo']}-{cfg['size_hi']}")
                                 + f", generator window {cfg['window']}), {T} hourly steps, "
-                                f"checkpoint window {cfg['ckpt']}, forward + adjoint, MSE at "
+                                f"checkpoint window {window or (T - 1)}, forward + adjoint, MSE at "
                                 f"{len(gauges_np)} basin outlets") if outlets is not None else
                                (f"{args.config}: {R}-reach tree (window {cfg['window']}), "
                                 f"{T} hourly steps, forward + adjoint, MSE at 64 gauges"),
-                   "reaches": R, "timesteps": T, "levels": net.n_levels, "form": "persistent",
+                   "reaches": R, "timesteps": T, "levels": net.n_levels, "schedule": schedule,
+                   "form": "persistent",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": roof, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": roof / HBM_PEAK_GBS, "traffic": traffic,
@@ -582,6 +589,26 @@ def loss_bench(torch, steps, warmup, with_cpu):
     return res
 
 
+def schedule_for(args, adj, T, qprime, gauges, ckpt, tune=True):
+    """(network, checkpoint window, description): the autotuner's choice, or
+    the fixed options (--sublevels / --block, the config's checkpoint window)."""
+    from ddrs_amd import tune as TU
+    from ddrs_amd.sparse import Network
+    if args.sublevels or args.block or not tune:
+        net = Network(adj, args.sublevels, args.block)
+        return net, ckpt, {"tuned": False, "sub_levels": args.sublevels or 8,
+                           "block": args.block or 64, "window": ckpt or (T - 1)}
+    t0 = time.time()
+    ts = TU.tune_schedule(adj, T, q_prime=qprime, gauges=gauges)
+    desc = {"tuned": True, "sub_levels": ts.max_sub_levels, "block": ts.block,
+            "window": ts.window, "predicted_ms": ts.predicted_ms,
+            "tuning_s": time.time() - t0,
+            "probes": [{k: v for k, v in pr.items() if k != "probe_lengths"} for pr in ts.probes],
+            "probe_lengths": ts.probes[0]["probe_lengths"]}
+    window = ts.window if ts.window < T - 1 else None
+    return ts.network, window, desc
+
+
 def train_bench(args, torch):
     """The training workload: one optimizer step of ddrs_amd.train.Trainer per
     bench step on a fixed micro-batch (HIP KAN head, gauged routing window,
@@ -612,13 +639,14 @@ def train_bench(args, torch):
     at0 = NW.gen_attributes(R0, 2000 + ci)
     at = {k: np.ascontiguousarray(v[active]) for k, v in at0.items()}
     adj = SparseAdjacency.from_coo(R, cr, cc, None, at["length"], at["slope"])
-    net = Network(adj, args.sublevels, args.block)
-    log(f"[bench] c5 batch: {R} of {R0} reaches, {cg.size} gauges (basins {basin.min()}-"
-        f"{basin.max()}), {net.n_levels} levels; host setup {time.time() - t0:.1f}s")
     gen = torch.Generator(device=dev).manual_seed(5000 + ci)
     names = [f"attr_{i}" for i in range(KAN_SHAPE["F"])]
     x = torch.randn(R, len(names), generator=gen, device=dev)
     qprime = NW.gen_forcing_device(R, T, 3000 + ci, cfg["storms"], device=dev)
+    net, window, schedule = schedule_for(args, adj, T, qprime, cg, cfg["ckpt"])
+    log(f"[bench] c5 batch: {R} of {R0} reaches, {cg.size} gauges (basins {basin.min()}-"
+        f"{basin.max()}), {net.n_levels} levels; setup {time.time() - t0:.1f}s; "
+        f"schedule {schedule}")
     mk = lambda seed: K.KanHeadConfig(names, ["n", "q_spatial"], seed,
                                       hidden_size=KAN_SHAPE["H"],
                                       num_hidden_layers=KAN_SHAPE["L"], grid=KAN_SHAPE["grid"],
@@ -626,7 +654,7 @@ def train_bench(args, torch):
     tau, wu, eps = cfg["tau"], cfg["warmup"], cfg["eps"]
     # observations: a hidden head's routed discharge at the gauges, daily
     # pooled, times LogNormal(0, 0.1) noise; sigma is each gauge's std
-    truth = TR.Trainer(mk(4242), tau=tau, warmup=wu, eps=eps, window=cfg["ckpt"], network=net)
+    truth = TR.Trainer(mk(4242), tau=tau, warmup=wu, eps=eps, window=window, network=net)
     probe = TR.GaugedBatch(adj, x, qprime, cg, None, None)
     with torch.no_grad():
         Qg = truth.predict(probe)
@@ -636,7 +664,7 @@ def train_bench(args, torch):
     del truth, Qg, daily
     batch = TR.GaugedBatch(adj, x, qprime, cg, obs.contiguous(), sigma.contiguous())
     head = mk(42)
-    tr = TR.Trainer(head, tau=tau, warmup=wu, eps=eps, window=cfg["ckpt"], network=net)
+    tr = TR.Trainer(head, tau=tau, warmup=wu, eps=eps, window=window, network=net)
     log(f"[bench] c5 observations ready ({time.time() - t0:.1f}s)")
     first = tr.step(batch)
     for _ in range(max(args.warmup - 1, 0)):
@@ -734,8 +762,8 @@ def train_bench(args, torch):
                                 f"forward -> tau {tau} trim + daily pooling -> warmup {wu} d -> "
                                 f"batch NSE -> adjoint -> KAN backward -> clip 1.0 -> AdaDelta"),
                    "reaches": R, "timesteps": T, "gauges": int(cg.size),
-                   "levels": net.n_levels, "checkpoint_window": cfg["ckpt"] or (T - 1),
-                   "form": "persistent", "parallelism": "single GPU"},
+                   "levels": net.n_levels, "checkpoint_window": window or (T - 1),
+                   "schedule": schedule, "form": "persistent", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": roof, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": roof / HBM_PEAK_GBS, "traffic": None,
                      "kernel": KERNEL_NAMES[dom], "launch_ms": launch_ms[dom],

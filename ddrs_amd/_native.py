@@ -51,6 +51,7 @@ _SIGS = {
     "ddrs_workspace_create_ex": (c_int, [P, c_int, c_int, c_int, ctypes.POINTER(P)]),
     "ddrs_workspace_destroy": (None, [P]),
     "ddrs_workspace_bytes": (c_int, [P, ctypes.POINTER(c_size)]),
+    "ddrs_workspace_query_bytes": (c_int, [P, c_int, c_int, c_int, ctypes.POINTER(c_size)]),
     "ddrs_workspace_check": (c_int, [P, P]),
     "ddrs_mc_bind_statics": (c_int, [P, ctypes.POINTER(DdrsParams), P, P, P, P, P, P, P]),
     "ddrs_mc_step_fwd": (c_int, [P, ctypes.POINTER(DdrsParams), P, P, P, P, P]),

@@ -590,6 +590,49 @@ ddrs_status plan_to_device(ddrs_plan* pl) {
     return DDRS_OK;
 }
 
+static ddrs_status check_ws_args(ddrs_plan_t plan, int32_t window, int32_t max_steps) {
+    if (!plan) return fail(DDRS_ERR_INVALID, "plan is NULL");
+    if (window < 1 || max_steps < 0) return fail(DDRS_ERR_INVALID, "window must be >= 1");
+    if (window > 65535) return fail(DDRS_ERR_INVALID, "window must be <= 65535");
+    return DDRS_OK;
+}
+
+// The workspace's device buffers in allocation order: f(&field, bytes) for
+// each. Shared by the allocation and the size query, so the two agree.
+template <class F>
+static void ws_buffers(ddrs_workspace* ws, int32_t flags, F&& f) {
+    const size_t R = ws->plan->P;  // position-indexed buffers
+    const size_t WR = static_cast<size_t>(ws->window) * R;
+    const size_t steps = static_cast<size_t>(std::max(ws->max_steps, 1));
+    f(&ws->st, R * 3 * sizeof(float4));
+    f(&ws->st_q, R * 4);
+    f(&ws->st_i, R * 4);
+    f(&ws->st_lam, R * 4);
+    f(&ws->acc_n, R * 4);
+    f(&ws->acc_qsp, R * 4);
+    f(&ws->acc_p, R * 4);
+    f(&ws->ck, static_cast<size_t>(ws->max_seg) * 2 * R * 4);
+    f(&ws->hist, WR * 8);
+    f(&ws->ihist, WR * sizeof(float2));
+    f(&ws->amsg, WR * 16);
+    f(&ws->rec, WR * sizeof(float4));
+    f(&ws->recj, WR * sizeof(float4));
+    f(&ws->qps, steps * R * 4);
+    if (!(flags & DDRS_WS_NO_DENSE_GRAD)) f(&ws->gqs, steps * R * 4);
+    f(&ws->gbuf, WR * 4);
+    f(&ws->slot, R * 8);
+    f(&ws->claim_ctr, 8);
+    f(&ws->lin_ctr, 8);
+    f(&ws->err, sizeof(int));
+}
+
+static void ws_shape(ddrs_workspace* ws, ddrs_plan_t plan, int32_t window, int32_t max_steps) {
+    ws->plan = plan;
+    ws->window = window;
+    ws->max_steps = max_steps;
+    ws->max_seg = std::max(1, (max_steps + window - 1) / window);
+}
+
 }  // namespace
 
 extern "C" {
@@ -597,47 +640,35 @@ extern "C" {
 // ---------------------------------------------------------------------------
 // workspace
 
+ddrs_status ddrs_workspace_query_bytes(ddrs_plan_t plan, int32_t window, int32_t max_steps,
+                                       int32_t flags, size_t* bytes) {
+    if (!bytes) return fail(DDRS_ERR_INVALID, "bytes is NULL");
+    *bytes = 0;
+    ddrs_status st = check_ws_args(plan, window, max_steps);
+    if (st != DDRS_OK) return st;
+    ddrs_workspace tmp;
+    ws_shape(&tmp, plan, window, max_steps);
+    size_t total = 0;
+    ws_buffers(&tmp, flags, [&](auto**, size_t b) { total += b; });
+    *bytes = total;
+    return DDRS_OK;
+}
+
 ddrs_status ddrs_workspace_create_ex(ddrs_plan_t plan, int32_t window, int32_t max_steps,
                                      int32_t flags, ddrs_workspace_t* out) {
     if (!out) return fail(DDRS_ERR_INVALID, "out is NULL");
     *out = nullptr;
-    if (!plan) return fail(DDRS_ERR_INVALID, "plan is NULL");
-    if (window < 1 || max_steps < 0) return fail(DDRS_ERR_INVALID, "window must be >= 1");
-    if (window > 65535) return fail(DDRS_ERR_INVALID, "window must be <= 65535");
+    ddrs_status st = check_ws_args(plan, window, max_steps);
+    if (st != DDRS_OK) return st;
     ddrs_status pst = plan_to_device(plan);
     if (pst != DDRS_OK) return pst;
     auto* ws = new ddrs_workspace();
-    ws->plan = plan;
-    ws->window = window;
-    ws->max_steps = max_steps;
-    ws->max_seg = std::max(1, (max_steps + window - 1) / window);
-    const size_t R = plan->P;  // position-indexed buffers
+    ws_shape(ws, plan, window, max_steps);
+    const size_t R = plan->P;
     const size_t WR = static_cast<size_t>(window) * R;
-    ddrs_status st = DDRS_OK;
-    auto A = [&](auto** p, size_t bytes) {
+    ws_buffers(ws, flags, [&](auto** p, size_t bytes) {
         if (st == DDRS_OK) st = dalloc(ws, reinterpret_cast<void**>(p), bytes);
-    };
-    A(&ws->st, R * 3 * sizeof(float4));
-    A(&ws->st_q, R * 4);
-    A(&ws->st_i, R * 4);
-    A(&ws->st_lam, R * 4);
-    A(&ws->acc_n, R * 4);
-    A(&ws->acc_qsp, R * 4);
-    A(&ws->acc_p, R * 4);
-    A(&ws->ck, static_cast<size_t>(ws->max_seg) * 2 * R * 4);
-    A(&ws->hist, WR * 8);
-    A(&ws->ihist, WR * sizeof(float2));
-    A(&ws->amsg, WR * 16);
-    A(&ws->rec, WR * sizeof(float4));
-    A(&ws->recj, WR * sizeof(float4));
-    A(&ws->qps, static_cast<size_t>(std::max(max_steps, 1)) * R * 4);
-    if (!(flags & DDRS_WS_NO_DENSE_GRAD))
-        A(&ws->gqs, static_cast<size_t>(std::max(max_steps, 1)) * R * 4);
-    A(&ws->gbuf, WR * 4);
-    A(&ws->slot, R * 8);
-    A(&ws->claim_ctr, 8);
-    A(&ws->lin_ctr, 8);
-    A(&ws->err, sizeof(int));
+    });
     if (st != DDRS_OK) {
         ddrs_workspace_destroy(ws);
         return st;

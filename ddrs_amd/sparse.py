@@ -106,6 +106,15 @@ class Network:
         N.check(N.lib.ddrs_plan_get_schedule(self.handle, _ptr(perm), _ptr(level)))
         return perm, level
 
+    def workspace_bytes(self, window: int, max_steps: int, dense_grad: bool = True) -> int:
+        """Device bytes a Workspace(self, window, max_steps, dense_grad) would
+        hold (without its gauge buffers); allocates nothing."""
+        b = N.c_size()
+        flags = 0 if dense_grad else N.WS_NO_DENSE_GRAD
+        N.check(N.lib.ddrs_workspace_query_bytes(self.handle, int(window), int(max_steps), flags,
+                                                 ctypes.byref(b)), "ddrs_workspace_query_bytes")
+        return int(b.value)
+
     def workspace(self, window: int = 1, max_steps: int = 1) -> "Workspace":
         key = (int(window), int(max_steps))
         for k, ws in self._ws.items():

@@ -101,7 +101,7 @@ ddrs_status ddrs_plan_get_schedule(ddrs_plan_t plan, int32_t* h_perm, int32_t* h
  * activation-checkpoint interval k: forward history, adjoint messages and
  * checkpoints are sized for it). max_steps bounds the total steps T-1 of one
  * window call (checkpoint storage). 288 GB MI355X: C2 (350k x 720) with
- * window 720 uses about 6.1 GB.
+ * window 719 and the dense dL/dQ buffer uses 17.7 GiB.
  * ------------------------------------------------------------------------- */
 ddrs_status ddrs_workspace_create(ddrs_plan_t plan, int32_t window, int32_t max_steps,
                                   ddrs_workspace_t* out);
@@ -115,6 +115,12 @@ ddrs_status ddrs_workspace_create_ex(ddrs_plan_t plan, int32_t window, int32_t m
 void ddrs_workspace_destroy(ddrs_workspace_t ws);
 /* Bytes of device memory the workspace holds. */
 ddrs_status ddrs_workspace_bytes(ddrs_workspace_t ws, size_t* bytes);
+/* Bytes ddrs_workspace_create_ex(plan, window, max_steps, flags) would
+ * allocate (before the gauge buffers of ddrs_workspace_set_gauges), without
+ * allocating: for choosing a checkpoint window that fits the memory at hand.
+ * No reference counterpart (the reference keeps every step's state). */
+ddrs_status ddrs_workspace_query_bytes(ddrs_plan_t plan, int32_t window, int32_t max_steps,
+                                       int32_t flags, size_t* bytes);
 /* Synchronise `stream` and report whether a persistent kernel of this
  * workspace stopped on its progress watchdog since the last check (the
  * replacement for the reference's expect()-panics on solver failure,

@@ -174,3 +174,28 @@ def test_generators_are_deterministic():
     at = NW.gen_attributes(1000, 4)
     assert at["length"].min() >= 1000 and at["length"].max() <= 10000
     assert at["slope"].min() >= 1e-4 and at["slope"].max() <= 5e-2
+
+
+def test_workspace_size_query_and_window_choice():
+    """ddrs_workspace_query_bytes (no allocation) and the autotuner's
+    checkpoint-window choice: the longest window that fits the budget, all
+    steps when they fit, an error when one step does not."""
+    from ddrs_amd import network as NW
+    from ddrs_amd import tune as TU
+    from ddrs_amd.sparse import Network, SparseAdjacency
+    R = 20000
+    rows, cols = NW.gen_tree(R, 256, 3)
+    net = Network(SparseAdjacency.from_coo(R, rows, cols))
+    b = lambda w, m, d=True: net.workspace_bytes(w, m, d)
+    # the dense dL/dQ buffer is max_steps x positions x 4 bytes
+    P = (b(10, 100) - b(10, 100, False)) // (100 * 4)
+    assert R <= P < R + 64 * net.n_chunks
+    # per window step: 68 bytes per position (history, sums, messages, records, gb)
+    assert b(11, 100) - b(10, 100) == 68 * P
+    T = 500
+    full = b(T - 1, T - 1, False)
+    assert TU.choose_window(net, T, full) == T - 1
+    w = TU.choose_window(net, T, full - 1)
+    assert w < T - 1 and b(w, T - 1, False) <= full - 1 < b(w + 1, T - 1, False)
+    with pytest.raises(MemoryError):
+        TU.choose_window(net, T, b(1, T - 1, False) - 1)
