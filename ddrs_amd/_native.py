@@ -36,6 +36,7 @@ class DdrsParams(ctypes.Structure):
 FORM_PLAIN = 0
 FORM_PERSISTENT = 1
 WS_NO_DENSE_GRAD = 1
+LOSS_L1, LOSS_NSE_BATCH = 0, 1
 
 # name -> (restype, argtypes); restype None means ddrs_status (int32)
 _SIGS = {
@@ -100,6 +101,8 @@ _SIGS = {
     "ddrs_nse_batch_loss_bwd": (c_int, [P, P, P, c_int, c_int, c_float, P, P, P]),
     "ddrs_gauge_nse_loss": (c_int, [P, P, c_int, ctypes.c_int64, ctypes.c_int64, c_int, c_int,
                                     P, P, c_float, P, P, P, P, P]),
+    "ddrs_gauge_batch_loss": (c_int, [P, c_int, P, c_int, ctypes.c_int64, ctypes.c_int64, c_int,
+                                      c_int, P, P, c_float, P, P, P, P, P]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -215,24 +215,27 @@ __device__ double block_sum(double v, double* sh) {
     return r;
 }
 
-// Per gauge over days [t0, T): sum of (p - o)^2 / (sigma + eps)^2 and whether
-// the observations there hold no NaN (check_nan: else every gauge counts).
+// Per gauge over days [t0, T): the sum of the loss terms, (p - o)^2 /
+// (sigma + eps)^2 for nse-batch or |p - o| for L1, and whether the
+// observations there hold no NaN (check_nan: else every gauge counts).
 __global__ __launch_bounds__(256) void k_nse_partial(const float* p, const float* o,
                                                      const float* sigma, int T, int t0, float eps,
-                                                     int check_nan, double* part, int* valid) {
+                                                     int check_nan, int kind, double* part,
+                                                     int* valid) {
     __shared__ double sh[256];
     __shared__ int bad;
     const int g = blockIdx.x;
     if (threadIdx.x == 0) bad = 0;
     __syncthreads();
-    const float d = sigma[g] + eps;
+    const bool l1 = kind == DDRS_LOSS_L1;
+    const float d = l1 ? 1.0f : sigma[g] + eps;
     const float dd = d * d;
     double acc = 0.0;
     for (int t = t0 + threadIdx.x; t < T; t += blockDim.x) {
         const float ov = o[static_cast<size_t>(g) * T + t];
         if (check_nan && isnan(ov)) bad = 1;
         const float r = p[static_cast<size_t>(g) * T + t] - ov;
-        acc += static_cast<double>(r * r / dd);
+        acc += l1 ? static_cast<double>(fabsf(r)) : static_cast<double>(r * r / dd);
     }
     const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) {
@@ -261,10 +264,11 @@ __global__ __launch_bounds__(256) void k_nse_final(const double* part, const int
     }
 }
 
-// gp[g][t] = gloss * 2 (p - o) / ((sigma + eps)^2 n) for kept gauges and
-// t >= t0, else 0; n from *n_valid (or G * (T - t0) when n_valid is null).
+// gp[g][t] = gloss * 2 (p - o) / ((sigma + eps)^2 n) (nse-batch) or
+// gloss * sign(p - o) / n (L1, sign 0 at p == o) for kept gauges and t >= t0,
+// else 0; n from *n_valid (or G * (T - t0) when n_valid is null).
 __global__ void k_nse_bwd(const float* p, const float* o, const float* sigma, int G, int T,
-                          int t0, float eps, const float* gloss, const int* valid,
+                          int t0, float eps, int kind, const float* gloss, const int* valid,
                           const int* n_valid, float* gp) {
     const int g = blockIdx.y;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -273,9 +277,15 @@ __global__ void k_nse_bwd(const float* p, const float* o, const float* sigma, in
     float v = 0.0f;
     const int n = n_valid ? *n_valid : G * (T - t0);
     if (t >= t0 && n > 0 && (!valid || valid[g])) {
-        const float d = sigma[g] + eps;
         const float gl = gloss ? *gloss : 1.0f;
-        v = gl * (2.0f * (p[e] - o[e]) / (d * d)) / static_cast<float>(n);
+        const float r = p[e] - o[e];
+        if (kind == DDRS_LOSS_L1) {
+            const float sg = (r > 0.0f) ? 1.0f : ((r < 0.0f) ? -1.0f : 0.0f);
+            v = gl * sg / static_cast<float>(n);
+        } else {
+            const float d = sigma[g] + eps;
+            v = gl * (2.0f * r / (d * d)) / static_cast<float>(n);
+        }
     }
     gp[e] = v;
 }
@@ -465,7 +475,7 @@ ddrs_status ddrs_nse_batch_loss(ddrs_gauge_plan_t p, const float* d_p, const flo
         return lfail(DDRS_ERR_INVALID, "nse_batch_loss: bad arguments (G <= the plan's gauges)");
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(k_nse_partial, dim3(G), dim3(256), 0, s, d_p, d_o, d_sigma, T, 0, eps, 0,
-                       p->part, p->valid);
+                       static_cast<int>(DDRS_LOSS_NSE_BATCH), p->part, p->valid);
     hipLaunchKernelGGL(k_nse_final, dim3(1), dim3(256), 0, s, p->part, p->valid, G, T, d_loss,
                        nullptr, nullptr);
     return hip_check("nse_batch_loss");
@@ -477,7 +487,8 @@ ddrs_status ddrs_nse_batch_loss_bwd(const float* d_p, const float* d_o, const fl
     if (G < 1 || T < 1 || !d_p || !d_o || !d_sigma || !d_gp)
         return lfail(DDRS_ERR_INVALID, "nse_batch_loss_bwd: bad arguments");
     hipLaunchKernelGGL(k_nse_bwd, grid2(T, G), dim3(256), 0, static_cast<hipStream_t>(stream),
-                       d_p, d_o, d_sigma, G, T, 0, eps, d_gloss, nullptr, nullptr, d_gp);
+                       d_p, d_o, d_sigma, G, T, 0, eps, static_cast<int>(DDRS_LOSS_NSE_BATCH),
+                       d_gloss, nullptr, nullptr, d_gp);
     return hip_check("nse_batch_loss_bwd");
 }
 
@@ -486,16 +497,30 @@ ddrs_status ddrs_gauge_nse_loss(ddrs_gauge_plan_t p, const float* d_runoff, int3
                                 const float* d_obs, const float* d_sigma, float eps,
                                 float* d_loss, int32_t* d_n_valid, float* d_grunoff,
                                 float* d_hourly, void* stream) {
+    if (!d_sigma) return lfail(DDRS_ERR_INVALID, "gauge_nse_loss: NULL sigma");
+    return ddrs_gauge_batch_loss(p, DDRS_LOSS_NSE_BATCH, d_runoff, T_hours, stride_n, stride_t,
+                                 tau, warmup, d_obs, d_sigma, eps, d_loss, d_n_valid, d_grunoff,
+                                 d_hourly, stream);
+}
+
+ddrs_status ddrs_gauge_batch_loss(ddrs_gauge_plan_t p, int32_t kind, const float* d_runoff,
+                                  int32_t T_hours, int64_t stride_n, int64_t stride_t,
+                                  int32_t tau, int32_t warmup, const float* d_obs,
+                                  const float* d_sigma, float eps, float* d_loss,
+                                  int32_t* d_n_valid, float* d_grunoff, float* d_hourly,
+                                  void* stream) {
     int start, L, M;
-    if (!p || !d_runoff || !d_obs || !d_sigma || !d_loss)
-        return lfail(DDRS_ERR_INVALID, "gauge_nse_loss: NULL argument");
+    if (kind != DDRS_LOSS_L1 && kind != DDRS_LOSS_NSE_BATCH)
+        return lfail(DDRS_ERR_INVALID, "gauge_batch_loss: unknown loss kind");
+    if (!p || !d_runoff || !d_obs || !d_loss || (kind == DDRS_LOSS_NSE_BATCH && !d_sigma))
+        return lfail(DDRS_ERR_INVALID, "gauge_batch_loss: NULL argument");
     if (T_hours > p->max_t)
-        return lfail(DDRS_ERR_CAPACITY, "gauge_nse_loss: T_hours exceeds the plan's max_t_hours");
+        return lfail(DDRS_ERR_CAPACITY, "gauge_batch_loss: T_hours exceeds the plan's max_t_hours");
     if (!window_of(T_hours, tau, &start, &L, &M))
-        return lfail(DDRS_ERR_INVALID, "gauge_nse_loss: tau must be in [0, 24) and "
+        return lfail(DDRS_ERR_INVALID, "gauge_batch_loss: tau must be in [0, 24) and "
                                        "T_hours - 24 >= 24");
     if (warmup < 0 || warmup >= M)
-        return lfail(DDRS_ERR_INVALID, "gauge_nse_loss: warmup must be in [0, days)");
+        return lfail(DDRS_ERR_INVALID, "gauge_batch_loss: warmup must be in [0, days)");
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int G = p->G;
     const bool rows = stride_t != 1;  // time-major: hour-major gauge series
@@ -514,12 +539,13 @@ ddrs_status ddrs_gauge_nse_loss(ddrs_gauge_plan_t p, const float* d_runoff, int3
                            M, p->daily);
     }
     hipLaunchKernelGGL(k_nse_partial, dim3(G), dim3(256), 0, s, p->daily, d_obs, d_sigma, M,
-                       warmup, eps, 1, p->part, p->valid);
+                       warmup, eps, 1, static_cast<int>(kind), p->part, p->valid);
     hipLaunchKernelGGL(k_nse_final, dim3(1), dim3(256), 0, s, p->part, p->valid, G, M - warmup,
                        d_loss, p->n_valid, d_n_valid);
     if (d_grunoff && p->U > 0) {
         hipLaunchKernelGGL(k_nse_bwd, grid2(M, G), dim3(256), 0, s, p->daily, d_obs, d_sigma, G,
-                           M, warmup, eps, nullptr, p->valid, p->n_valid, p->gdaily);
+                           M, warmup, eps, static_cast<int>(kind), nullptr, p->valid, p->n_valid,
+                           p->gdaily);
         if (rows) {
             hipLaunchKernelGGL(k_area_pool_bwd_tg, dim3(T_hours, (G + 255) / 256), dim3(256), 0, s,
                                p->gdaily, G, start, L, M, p->hourly);
@@ -535,7 +561,7 @@ ddrs_status ddrs_gauge_nse_loss(ddrs_gauge_plan_t p, const float* d_runoff, int3
                                static_cast<long long>(stride_n), 1LL, d_grunoff);
         }
     }
-    return hip_check("gauge_nse_loss");
+    return hip_check("gauge_batch_loss");
 }
 
 }  // extern "C"

@@ -13,7 +13,7 @@ All run in the HIP kernels through the C ABI; there is no CPU path.
 from __future__ import annotations
 
 import ctypes
-from typing import Sequence, Tuple
+from typing import Optional, Sequence, Tuple
 
 import numpy as np
 import torch
@@ -157,20 +157,24 @@ def nse_batch_loss(p: torch.Tensor, o: torch.Tensor, sigma: torch.Tensor, eps: f
     return _Nse.apply(p, o, sigma, eps, plan)
 
 
+_KINDS = {"l1": N.LOSS_L1, "nse-batch": N.LOSS_NSE_BATCH}
+
+
 class _GaugeNse(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, runoff, plan, time_major, tau, warmup, obs, sigma, eps):
+    def forward(ctx, runoff, plan, time_major, tau, warmup, obs, sigma, eps, kind):
         x = _cuda_f32(runoff, "runoff")
-        o, s = _cuda_f32(obs, "obs"), _cuda_f32(sigma, "sigma")
+        o = _cuda_f32(obs, "obs")
+        s = _cuda_f32(sigma, "sigma") if sigma is not None else None
         T, sn, st = _strides(x, time_major)
         loss = torch.empty((), device=x.device)
         n_valid = torch.empty((), device=x.device, dtype=torch.int32)
         hourly = torch.empty(plan.num_gauges, T, device=x.device)
         g = torch.zeros_like(x) if ctx.needs_input_grad[0] else None
-        N.check(N.lib.ddrs_gauge_nse_loss(plan.handle, _ptr(x), T, sn, st, int(tau), int(warmup),
-                                          _ptr(o), _ptr(s), float(eps), _ptr(loss),
-                                          _ptr(n_valid), _ptr(g), _ptr(hourly), _stream()),
-                "ddrs_gauge_nse_loss")
+        N.check(N.lib.ddrs_gauge_batch_loss(plan.handle, _KINDS[kind], _ptr(x), T, sn, st,
+                                            int(tau), int(warmup), _ptr(o), _ptr(s), float(eps),
+                                            _ptr(loss), _ptr(n_valid), _ptr(g), _ptr(hourly),
+                                            _stream()), "ddrs_gauge_batch_loss")
         ctx.save_for_backward(g if g is not None else torch.empty(0, device=x.device))
         ctx.mark_non_differentiable(n_valid, hourly)
         return loss, n_valid, hourly
@@ -178,7 +182,7 @@ class _GaugeNse(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gl, _gn, _gh):
         (g,) = ctx.saved_tensors
-        return g * gl.to(torch.float32), None, None, None, None, None, None, None
+        return g * gl.to(torch.float32), None, None, None, None, None, None, None, None
 
 
 def gauge_nse_loss(runoff: torch.Tensor, plan: GaugePlan, tau: int, warmup: int,
@@ -190,4 +194,17 @@ def gauge_nse_loss(runoff: torch.Tensor, plan: GaugePlan, tau: int, warmup: int,
     n_valid is loss_denominator (kept gauges x post-warmup days; 0 and loss 0
     when every gauge has a NaN), hourly the [G, T] readout. obs is
     [G, T_days] with NaN for missing days, sigma [G]."""
-    return _GaugeNse.apply(runoff, plan, time_major, tau, warmup, obs, sigma, eps)
+    return _GaugeNse.apply(runoff, plan, time_major, tau, warmup, obs, sigma, eps, "nse-batch")
+
+
+def gauge_batch_loss(runoff: torch.Tensor, plan: GaugePlan, tau: int, warmup: int,
+                     obs: torch.Tensor, sigma: Optional[torch.Tensor] = None, eps: float = 0.1,
+                     kind: str = "l1", time_major: bool = True):
+    """gauge_nse_loss for a loss.kind of batch_loss (src/training/loss.rs:179-208):
+    "l1" (the reference's default, mean |p - o|; sigma and eps unused) or
+    "nse-batch" (sigma required)."""
+    if kind not in _KINDS:
+        raise ValueError(f"loss kind must be one of {sorted(_KINDS)}, got {kind!r}")
+    if kind == "nse-batch" and sigma is None:
+        raise ValueError("loss kind nse-batch needs the per-gauge sigma")
+    return _GaugeNse.apply(runoff, plan, time_major, tau, warmup, obs, sigma, eps, kind)

@@ -1,8 +1,9 @@
 """One training micro-batch of the reference's driver on the library's
 kernels: KAN head -> denormalize -> gauged routing window -> tau trim, daily
 pooling and batch NSE -> routing adjoint -> KAN backward -> global-norm
-clipping -> AdaDelta (src/training/driver.rs:93-215 for the micro-batch,
-:295-330 for the single-batch optimizer step).
+clipping -> AdaDelta or Adam (src/training/driver.rs:93-215 for the
+micro-batch, :295-330 for the single-batch optimizer step), with the nse-batch
+or L1 objective.
 
 The discharge is read out at the gauges inside the routing window (the
 forward writes Q only at the gauge reaches, the adjoint takes dL/dQ only
@@ -10,7 +11,8 @@ there), so a batch never holds the [T, N] discharge the reference builds
 before its scatter_add_by_group; the loss and every gradient are those of the
 reference's dense chain.
 
-AdaDelta (src/training/adadelta.rs:16-23,93-125) and clip_grad_norm
+AdaDelta (src/training/adadelta.rs:16-23,93-125), Adam
+(src/training/optimizer.rs:131-149) and clip_grad_norm
 (src/training/optimizer.rs:163-193) update the head's few thousand weights with
 torch tensor arithmetic on the device.
 """
@@ -26,7 +28,7 @@ import torch
 from . import _native as N
 from .config import Params
 from .kan import KanHead
-from .loss import GaugePlan, gauge_nse_loss
+from .loss import GaugePlan, gauge_batch_loss
 from .routing import denormalize, hotstart, route_window_gauged
 from .sparse import Network, SparseAdjacency
 
@@ -69,6 +71,47 @@ class AdaDelta:
             self.state[id(p)] = (sq, acc)
 
 
+class Adam:
+    """Adam with the reference's settings (beta1 0.9, beta2 0.999, eps 1e-8;
+    src/training/optimizer.rs:131-149, the default optimizer), the published
+    update (Kingma and Ba 2015) that burn's Adam and torch.optim.Adam implement,
+    per parameter at step t:
+
+        m = beta1 m + (1 - beta1) g
+        v = beta2 v + (1 - beta2) g^2
+        param -= lr * (m / (1 - beta1^t)) / (sqrt(v / (1 - beta2^t)) + eps)
+
+    with m and v starting at zero."""
+
+    def __init__(self, params: Iterable[torch.nn.Parameter], beta1: float = 0.9,
+                 beta2: float = 0.999, epsilon: float = 1e-8):
+        self.params: List[torch.nn.Parameter] = list(params)
+        self.beta1, self.beta2, self.epsilon = float(beta1), float(beta2), float(epsilon)
+        self.state: Dict[int, tuple] = {}
+
+    @torch.no_grad()
+    def step(self, lr: float = 1e-3) -> None:
+        for p in self.params:
+            g = p.grad
+            if g is None:
+                continue
+            m, v, t = self.state.get(id(p), (None, None, 0))
+            if m is None:
+                m, v = torch.zeros_like(p), torch.zeros_like(p)
+            t += 1
+            m = m * self.beta1 + g * (1.0 - self.beta1)
+            v = v * self.beta2 + (g * g) * (1.0 - self.beta2)
+            m_hat = m / (1.0 - self.beta1 ** t)
+            v_hat = v / (1.0 - self.beta2 ** t)
+            p.sub_(lr * m_hat / (torch.sqrt(v_hat) + self.epsilon))
+            self.state[id(p)] = (m, v, t)
+
+    def moments(self, p: torch.nn.Parameter):
+        """(m, v) of a parameter after the last step."""
+        m, v, _ = self.state[id(p)]
+        return m, v
+
+
 @torch.no_grad()
 def clip_grad_norm(params: Iterable[torch.nn.Parameter], max_norm: float) -> float:
     """Scale every gradient by max_norm / norm when the global L2 norm over all
@@ -100,14 +143,14 @@ class GaugedBatch:
                   prediction (outflow_idx of a gauge is its own reach);
     observations  [G, T_days] daily observations, NaN where missing;
     gauge_obs_std [G] per-gauge standard deviation of the observations (the
-                  nse-batch sigma)."""
+                  nse-batch sigma; None for the L1 objective)."""
 
     adjacency: SparseAdjacency
     attributes: torch.Tensor
     q_prime: torch.Tensor
     gauges: np.ndarray
     observations: torch.Tensor
-    gauge_obs_std: torch.Tensor
+    gauge_obs_std: Optional[torch.Tensor]
 
 
 @dataclass
@@ -127,14 +170,20 @@ class Trainer:
     def __init__(self, head: KanHead, params: Optional[Params] = None, tau: int = 9,
                  warmup: int = 5, eps: float = 0.1, grad_clip: float = 1.0,
                  window: Optional[int] = None, form: int = N.FORM_PERSISTENT,
-                 network: Optional[Network] = None):
+                 network: Optional[Network] = None, loss: str = "nse-batch",
+                 optimizer: str = "adadelta"):
         if "n" not in head.learnable or "q_spatial" not in head.learnable:
             raise ValueError("the head must produce n and q_spatial")
+        if loss not in ("l1", "nse-batch"):
+            raise ValueError(f"loss must be 'l1' or 'nse-batch', got {loss!r}")
+        if optimizer not in ("adam", "adadelta"):
+            raise ValueError(f"optimizer must be 'adam' or 'adadelta', got {optimizer!r}")
         self.head = head
         self.params = params if params is not None else Params()
         self.tau, self.warmup, self.eps, self.grad_clip = int(tau), int(warmup), float(eps), grad_clip
         self.window, self.form = window, form
-        self.opt = AdaDelta(head.parameters())
+        self.loss_kind = loss
+        self.opt = (AdaDelta if optimizer == "adadelta" else Adam)(head.parameters())
         self._net: Optional[Network] = network
         self._adj = network.adj if network is not None else None
         self._plan: Optional[GaugePlan] = None
@@ -190,13 +239,17 @@ class Trainer:
         filter and batch NSE (driver.rs:126-201) of predict(batch)."""
         Qg = self.predict(batch)
         plan = self._gauge_plan(Qg.shape[1], Qg.shape[0])
-        loss, n_valid, _ = gauge_nse_loss(Qg, plan, self.tau, self.warmup, batch.observations,
-                                          batch.gauge_obs_std, self.eps, time_major=True)
+        loss, n_valid, _ = gauge_batch_loss(Qg, plan, self.tau, self.warmup, batch.observations,
+                                            batch.gauge_obs_std, self.eps, kind=self.loss_kind,
+                                            time_major=True)
         return loss, n_valid
 
-    def step(self, batch: GaugedBatch, lr: float = 1.0) -> StepOutcome:
-        """One optimizer step (driver.rs:315-329). A batch whose gauges are all
-        dropped by the NaN filter takes no step (its loss is 0)."""
+    def step(self, batch: GaugedBatch, lr: Optional[float] = None) -> StepOutcome:
+        """One optimizer step (driver.rs:315-329) at learning rate lr (default:
+        1 for AdaDelta, 1e-3 for Adam). A batch whose gauges are all dropped by
+        the NaN filter takes no step (its loss is 0)."""
+        if lr is None:
+            lr = 1.0 if isinstance(self.opt, AdaDelta) else 1e-3
         for prm in self.head.parameters():
             prm.grad = None
         loss, n_valid = self.loss(batch)

@@ -418,6 +418,17 @@ ddrs_status ddrs_gauge_nse_loss(ddrs_gauge_plan_t plan, const float* d_runoff, i
                                 const float* d_obs, const float* d_sigma, float eps,
                                 float* d_loss, int32_t* d_n_valid, float* d_grunoff,
                                 float* d_hourly, void* stream);
+/* The same objective for the loss kinds of batch_loss (src/training/loss.rs:179-208,
+ * LossKind, src/config.rs:269-289): DDRS_LOSS_L1, the mean of |p - o| (loss.rs:186; its
+ * gradient sign(p - o) / n_valid, 0 where p == o; d_sigma and eps unused, d_sigma may be
+ * NULL), or DDRS_LOSS_NSE_BATCH (as ddrs_gauge_nse_loss). */
+enum { DDRS_LOSS_L1 = 0, DDRS_LOSS_NSE_BATCH = 1 };
+ddrs_status ddrs_gauge_batch_loss(ddrs_gauge_plan_t plan, int32_t kind, const float* d_runoff,
+                                  int32_t T_hours, int64_t stride_n, int64_t stride_t,
+                                  int32_t tau, int32_t warmup, const float* d_obs,
+                                  const float* d_sigma, float eps, float* d_loss,
+                                  int32_t* d_n_valid, float* d_grunoff, float* d_hourly,
+                                  void* stream);
 
 /* ---------------------------------------------------------------------------
  * Network stores on disk (host): zarr v3 arrays and group attributes, as the

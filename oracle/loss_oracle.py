@@ -12,8 +12,10 @@ Restates, in numpy:
   s = L / M with fractional overlaps W[i, j] = (min(j+1, (i+1)s) - max(j, i s)) / s,
   all in float32 as the reference computes them;
 - the driver's warmup slice and NaN-gauge filter (src/training/driver.rs:150-195)
-  and nse_batch_loss (src/training/loss.rs:226-237):
-  mean over kept gauges and post-warmup days of (p - o)^2 / (sigma + eps)^2.
+  and the batch losses of batch_loss (src/training/loss.rs:179-208):
+  nse-batch (loss.rs:226-237), the mean over kept gauges and post-warmup days
+  of (p - o)^2 / (sigma + eps)^2, and L1 (loss.rs:186), the mean of |p - o|
+  (its gradient sign(p - o) / n, 0 where p == o).
 
 Pinned by the reference's own tests (tests/test_loss_oracle.py): the
 scatter fixture of src/training/forward.rs:1030-1058, the hand-computed
@@ -105,12 +107,20 @@ def nse_batch_loss(p, o, sigma, eps):
     return float((((p - o) ** 2) / (d * d)).mean())
 
 
+def l1_batch_loss(p, o):
+    """Mean of |p - o| (loss.rs:186), in float64."""
+    return float(np.abs(np.asarray(p, np.float64) - np.asarray(o, np.float64)).mean())
+
+
 def gauge_nse_loss(runoff, flat_indices, group_ids, num_gauges, tau, warmup, obs, sigma, eps,
-                   dtype=np.float64):
+                   dtype=np.float64, kind="nse-batch"):
     """The training objective from the routed discharge: (loss, n_valid,
     dL/d runoff). runoff [N, T_hours]; obs [G, T_days] with NaN for missing
-    days; sigma [G]. With no gauge kept the loss is 0 and n_valid 0. The
-    gauge sums run in `dtype`, the rest in float64."""
+    days; sigma [G] (nse-batch only). With no gauge kept the loss is 0 and
+    n_valid 0. The gauge sums run in `dtype`, the rest in float64. kind:
+    "nse-batch" or "l1"."""
+    if kind not in ("nse-batch", "l1"):
+        raise ValueError(kind)
     hourly = scatter_add_by_group(runoff, flat_indices, group_ids, num_gauges, dtype)
     daily = tau_trim_and_downsample(hourly, tau)
     keep = kept_gauges(obs, warmup)
@@ -122,9 +132,14 @@ def gauge_nse_loss(runoff, flat_indices, group_ids, num_gauges, tau, warmup, obs
     else:
         p = daily[keep, warmup:]
         o = np.asarray(obs, np.float64)[keep, warmup:]
-        d = (np.asarray(sigma, np.float64)[keep] + eps)[:, None]
-        loss = float((((p - o) ** 2) / (d * d)).sum() / n_valid)
-        g_daily[np.ix_(keep, np.arange(warmup, daily.shape[1]))] = 2.0 * (p - o) / (d * d) / n_valid
+        sel = np.ix_(keep, np.arange(warmup, daily.shape[1]))
+        if kind == "l1":
+            loss = float(np.abs(p - o).sum() / n_valid)
+            g_daily[sel] = np.sign(p - o) / n_valid
+        else:
+            d = (np.asarray(sigma, np.float64)[keep] + eps)[:, None]
+            loss = float((((p - o) ** 2) / (d * d)).sum() / n_valid)
+            g_daily[sel] = 2.0 * (p - o) / (d * d) / n_valid
     g_hourly = tau_trim_and_downsample_bwd(g_daily, hourly.shape[1], tau)
     g_runoff = np.zeros(np.asarray(runoff).shape)
     for k in range(len(flat_indices)):

@@ -253,3 +253,29 @@ def test_workload_size_properties():
         say(f"{n}: split err {err:.2e} max|g| {scale:.2e}")
         assert err <= 2e-5 * scale, n
     np.testing.assert_array_equal(gx_all, np.concatenate([gx_a, gx_b]))
+
+
+def test_training_step_fixture_parameters():
+    """The reference's training-step fixture (DDR, gauge 10336740): the head at
+    the seed-42 weights on the batch's 19 normalised attribute rows,
+    denormalised on the device (n, q_spatial linear; p_spatial in log space),
+    gives the fixture's routed parameters within 1e-6 relative."""
+    import ddrs_amd as D
+    say("training-step fixture parameters")
+    with np.load(os.path.join(os.path.dirname(__file__), "golden",
+                              "training_step_77006074.npz"), allow_pickle=False) as d:
+        tf = {k: d[k] for k in d.files}
+    head = K.KanHead.from_npz(FIXTURE, NAMES, 2, DEV)
+    x = torch.as_tensor(tf["mc_forward_norm_attrs"], device=DEV)
+    with torch.no_grad():
+        out = head(x)
+    P = D.Params()
+    rng, log = P.parameter_ranges, set(P.log_space_parameters)
+    got = {"n": D.denormalize(out["n"], rng.n, "n" in log),
+           "q_spatial": D.denormalize(out["q_spatial"], rng.q_spatial, "q_spatial" in log),
+           "p_spatial": D.denormalize(out["p_spatial"], rng.p_spatial, "p_spatial" in log)}
+    for name, v in got.items():
+        want = tf[f"mc_forward_{name}_param"]
+        rel = float(np.abs(v.cpu().numpy() - want).max() / np.abs(want).max())
+        print(f"  {name}: max rel err {rel:.2e}")
+        assert rel <= 1e-6, name

@@ -202,3 +202,30 @@ def test_workload_readout_c2_shape():
     g = x.grad[:, torch.as_tensor(reaches, device=DEV)].T.cpu().numpy()
     assert np.abs(g - wg).max() <= 1e-5 * np.abs(wg).max()
     assert float(x.grad.abs().sum()) == pytest.approx(float(np.abs(g).sum()), rel=1e-6)
+
+
+@pytest.mark.parametrize("time_major", [True, False])
+def test_fused_l1_matches_oracle(time_major):
+    """The L1 kind of the fused objective (loss.rs:186, the reference's default
+    loss): loss within 1e-5 relative of the float64 oracle, n_valid exact, the
+    gradient sign(p - o) / n_valid through the pooling and readout within 1e-5
+    of its max-abs; sigma may be absent."""
+    say("fused L1", time_major)
+    Nr, T, G = 2000, 24 * 30 + 5, 17
+    flat, group, runoff, obs, sigma = _fused_case(9, Nr, T, G, 2, [(4, 6)])
+    tau, warmup = 3, 2
+    plan = L.GaugePlan(Nr, flat, group, G, T)
+    x = t32(runoff.T.copy() if time_major else runoff).requires_grad_(True)
+    loss, n_valid, _ = L.gauge_batch_loss(x, plan, tau, warmup, t32(obs), None, kind="l1",
+                                          time_major=time_major)
+    loss.backward()
+    wl, wn, wg = lo.gauge_nse_loss(runoff, flat, group, G, tau, warmup, obs, None, 0.1,
+                                   kind="l1")
+    say(f"loss {loss.item():.8g} oracle {wl:.8g}")
+    assert int(n_valid) == wn == (G - 1) * ((T - 24) // 24 - warmup)
+    assert abs(loss.item() - wl) <= 1e-5 * abs(wl)
+    g = x.grad.cpu().numpy()
+    g = g.T if time_major else g
+    assert np.abs(g - wg).max() <= 1e-5 * np.abs(wg).max()
+    with pytest.raises(ValueError):
+        L.gauge_batch_loss(x, plan, tau, warmup, t32(obs), None, kind="nse-batch")

@@ -130,3 +130,40 @@ def test_step_is_clip_plus_adadelta_and_descends():
         losses.append(tb.step(batch).loss)
     say("losses", losses)
     assert losses[-1] < losses[0]
+
+
+def test_adam_l1_step():
+    """The reference's default objective and optimizer (L1, Adam at lr 1e-3):
+    Trainer.step equals clipping + torch.optim.Adam on the CPU applied to the
+    same device gradients, within 1e-6 relative; the L1 loss of the step
+    equals the oracle chain's."""
+    say("training step, L1 + Adam")
+    b = make_batch(2)
+    batch = TR.GaugedBatch(b.adj_obj, t(b.x), t(b.qprime), b.gauges, t(b.obs), None)
+    ha, hb = make_head(42), make_head(42)
+    ta = TR.Trainer(ha, tau=b.tau, warmup=b.warmup, loss="l1", optimizer="adam")
+    tb = TR.Trainer(hb, tau=b.tau, warmup=b.warmup, loss="l1", optimizer="adam",
+                    network=ta.network(batch))
+    loss, nv = ta.loss(batch)
+    loss.backward()
+    cpu = [torch.nn.Parameter(p.detach().cpu().clone()) for p in ha.parameters()]
+    for c, p in zip(cpu, ha.parameters()):
+        c.grad = p.grad.detach().cpu().clone()
+    norm = TR.clip_grad_norm(cpu, 1.0)
+    torch.optim.Adam(cpu, lr=1e-3, betas=(0.9, 0.999), eps=1e-8).step()
+    out = tb.step(batch)
+    assert out.grad_norm == pytest.approx(norm, rel=1e-6)
+    for c, p in zip(cpu, hb.parameters()):
+        np.testing.assert_allclose(p.detach().cpu().numpy(), c.detach().numpy(), rtol=1e-6,
+                                   atol=1e-9)
+    with torch.no_grad():
+        o = ha(batch.attributes)
+        rng = ta.params.parameter_ranges
+        nq = [(o[k] * (r[1] - r[0]) + r[0]).cpu().numpy()
+              for k, r in (("n", rng.n), ("q_spatial", rng.q_spatial))]
+    Qg, _ = b.oracle_Qg(TC.oracle_params(ha), ha.k, nq=nq)
+    from oracle import loss_oracle as lo
+    G = b.gauges.size
+    wl, wnv, _ = lo.gauge_nse_loss(Qg.T, np.arange(G), np.arange(G), G, b.tau, b.warmup, b.obs,
+                                   None, 0.1, kind="l1")
+    assert int(nv) == wnv and abs(out.loss - wl) <= 1e-4 * abs(wl)
