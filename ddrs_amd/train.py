@@ -261,6 +261,44 @@ class Trainer:
         self.opt.step(lr)
         return StepOutcome(float(loss.detach()), nv, norm)
 
+    def step_accumulated(self, batches, lr: Optional[float] = None) -> StepOutcome:
+        """One optimizer step over a group of micro-batches (the accumulation
+        path, driver.rs:360-430): each micro-batch's mean loss is scaled back
+        to a sum by its n_valid before its backward, the summed gradients are
+        scaled by 1 / sum(n_valid), so that they are the pooled mean's, then
+        clipped and stepped. Micro-batches with every gauge dropped add
+        nothing; a group of only such takes no step. The outcome's loss is the
+        n_valid-weighted mean."""
+        if lr is None:
+            lr = 1.0 if isinstance(self.opt, AdaDelta) else 1e-3
+        return _accumulate_step(self, batches, lr)
+
+
+def _accumulate_step(tr: "Trainer", batches, lr) -> StepOutcome:
+    """Trainer.step_accumulated (a function so that the class stays short)."""
+    for prm in tr.head.parameters():
+        prm.grad = None
+    total_n, weighted = 0, 0.0
+    for b in batches:
+        loss, n_valid = tr.loss(b)
+        n = int(n_valid)
+        if n == 0:
+            continue
+        # the mean back to a sum; the group is renormalised by 1 / total_n
+        (loss * float(n)).backward()
+        total_n += n
+        weighted += float(loss.detach()) * n
+    if total_n == 0:
+        return StepOutcome(0.0, 0, 0.0)
+    scale = float(np.float32(1.0) / np.float32(total_n))
+    with torch.no_grad():
+        for prm in tr.head.parameters():
+            if prm.grad is not None:
+                prm.grad.mul_(scale)
+    norm = clip_grad_norm(tr.head.parameters(), tr.grad_clip)
+    tr.opt.step(lr)
+    return StepOutcome(weighted / total_n, total_n, norm)
+
 
 def upstream_sizes(n: int, rows, cols) -> np.ndarray:
     """Reaches upstream of each reach, itself included, on a tree in

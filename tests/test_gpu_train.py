@@ -167,3 +167,37 @@ def test_adam_l1_step():
     wl, wnv, _ = lo.gauge_nse_loss(Qg.T, np.arange(G), np.arange(G), G, b.tau, b.warmup, b.obs,
                                    None, 0.1, kind="l1")
     assert int(nv) == wnv and abs(out.loss - wl) <= 1e-4 * abs(wl)
+
+
+def test_gradient_accumulation_is_the_pooled_mean():
+    """step_accumulated over two micro-batches of different networks: the
+    stepped weights equal clipping + AdaDelta applied to
+    (n1 g1 + n2 g2) / (n1 + n2) of the separate micro-batch gradients, within
+    1e-6; the reported loss is the n_valid-weighted mean."""
+    say("gradient accumulation over two micro-batches")
+    b1, b2 = make_batch(3), make_batch(4)
+    d1, d2 = device_batch(b1), device_batch(b2)
+    ha, hb = make_head(42), make_head(42)
+    ta = TR.Trainer(ha, tau=b1.tau, warmup=b1.warmup)
+    grads, ns, losses = [], [], []
+    for d in (d1, d2):
+        for p in ha.parameters():
+            p.grad = None
+        loss, nv = ta.loss(d)
+        loss.backward()
+        grads.append([p.grad.detach().cpu().clone() for p in ha.parameters()])
+        ns.append(int(nv))
+        losses.append(loss.item())
+    cpu = [torch.nn.Parameter(p.detach().cpu().clone()) for p in ha.parameters()]
+    for i, c in enumerate(cpu):
+        c.grad = (grads[0][i] * ns[0] + grads[1][i] * ns[1]) / (ns[0] + ns[1])
+    TR.clip_grad_norm(cpu, 1.0)
+    torch.optim.Adadelta(cpu).step()
+    tb = TR.Trainer(hb, tau=b1.tau, warmup=b1.warmup)
+    out = tb.step_accumulated([d1, d2])
+    assert out.n_valid == sum(ns)
+    assert out.loss == pytest.approx((losses[0] * ns[0] + losses[1] * ns[1]) / sum(ns),
+                                     rel=1e-6)
+    for c, p in zip(cpu, hb.parameters()):
+        np.testing.assert_allclose(p.detach().cpu().numpy(), c.detach().numpy(), rtol=1e-6,
+                                   atol=1e-9)
