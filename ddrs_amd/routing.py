@@ -226,13 +226,20 @@ class _WindowGauged(torch.autograd.Function):
 
 def _gauge_workspace(net: Network, window: int, max_steps: int, gauges) -> "Workspace":
     """A workspace of `net` without the dense dL/dQ buffer, its gauges set to
-    `gauges` (kept while unchanged)."""
+    `gauges` (kept while unchanged). One that holds at least `window` and
+    `max_steps` is reused (a longer checkpoint window gives the same results);
+    when none does, the smaller ones are freed before the new one is made, so
+    that windows of growing length do not pile up device memory. A backward
+    whose forward ran on a freed workspace recomputes its forward."""
     from .sparse import Workspace
-    key = ("gauged", int(window), int(max_steps))
-    ws = net._ws.get(key)
+    window, max_steps = int(window), int(max_steps)
+    gauged = [k for k in net._ws if k[0] == "gauged"]
+    ws = next((net._ws[k] for k in gauged if k[1] >= window and k[2] >= max_steps), None)
     if ws is None:
-        ws = Workspace(net, int(window), int(max_steps), dense_grad=False)
-        net._ws[key] = ws
+        for k in gauged:
+            net._ws.pop(k).close()
+        ws = Workspace(net, window, max_steps, dense_grad=False)
+        net._ws[("gauged", window, max_steps)] = ws
     g = np.ascontiguousarray(gauges, np.int32)
     if getattr(ws, "gauges", None) is None or not np.array_equal(ws.gauges, g):
         ws.set_gauges(g)

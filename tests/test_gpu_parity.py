@@ -506,6 +506,33 @@ def test_route_window_gauged_autograd():
             np.testing.assert_array_equal(got[k], ref[k], err_msg=f"{kw}: {k}")
 
 
+def test_gauged_workspace_reuse():
+    """Gauged windows of growing then shrinking length on one network keep a
+    single gauged workspace (the smaller one is freed when a longer window
+    needs more, the longer one serves shorter windows) and give the bits of a
+    fresh network's windows."""
+    say("gauged workspace reuse")
+    c = Case(3000, 61, seed=71, storms=True)
+    gauges = np.array([c.R // 2, c.R - 1], np.int32)
+    params = to_params(c.P)
+
+    def run(net, T):
+        n = t(c.n).requires_grad_(True)
+        Qg = D.route_window_gauged(net, params, n, t(c.qsp), t(c.p), t(c.qprime[:T]),
+                                   t(c.q0()), t(c.length), t(c.slope_raw), gauges)
+        Qg.backward(torch.ones_like(Qg))
+        return Qg.detach().cpu().numpy(), n.grad.cpu().numpy()
+
+    for T in (31, 61, 21):
+        got = run(c.net, T)
+        want = run(Network(SparseAdjacency.from_coo(c.R, c.rows, c.cols, c.vals, c.length,
+                                                     c.slope_raw)), T)
+        for a, b in zip(got, want):
+            np.testing.assert_array_equal(a, b)
+        assert sum(1 for k in c.net._ws if k[0] == "gauged") == 1
+    assert max(k[2] for k in c.net._ws if k[0] == "gauged") == 60
+
+
 @pytest.mark.skip(reason="linearising in the forward window is disabled in the library: a "
                          "350000-reach, 720-step window with the concurrent launch did not "
                          "finish on the GPU and the cause is not known")
