@@ -36,7 +36,15 @@ class DdrsParams(ctypes.Structure):
 FORM_PLAIN = 0
 FORM_PERSISTENT = 1
 WS_NO_DENSE_GRAD = 1
-LOSS_L1, LOSS_NSE_BATCH = 0, 1
+LOSS_L1, LOSS_NSE_BATCH, LOSS_NNSE_KGE, LOSS_KGE = 0, 1, 2, 3
+
+
+class LossConfig(ctypes.Structure):
+    """ddrs_loss_config (the reference's LossConfig, src/config.rs:220-262)."""
+    _fields_ = [("kind", ctypes.c_int32), ("eps", ctypes.c_float),
+                ("nnse_weight", ctypes.c_float), ("kge_weight", ctypes.c_float),
+                ("r_weight", ctypes.c_float), ("alpha_weight", ctypes.c_float),
+                ("beta_weight", ctypes.c_float), ("kge_clamp", ctypes.c_float)]
 
 # name -> (restype, argtypes); restype None means ddrs_status (int32)
 _SIGS = {
@@ -101,8 +109,8 @@ _SIGS = {
     "ddrs_nse_batch_loss_bwd": (c_int, [P, P, P, c_int, c_int, c_float, P, P, P]),
     "ddrs_gauge_nse_loss": (c_int, [P, P, c_int, ctypes.c_int64, ctypes.c_int64, c_int, c_int,
                                     P, P, c_float, P, P, P, P, P]),
-    "ddrs_gauge_batch_loss": (c_int, [P, c_int, P, c_int, ctypes.c_int64, ctypes.c_int64, c_int,
-                                      c_int, P, P, c_float, P, P, P, P, P]),
+    "ddrs_gauge_batch_loss": (c_int, [P, ctypes.POINTER(LossConfig), P, c_int, ctypes.c_int64,
+                                      ctypes.c_int64, c_int, c_int, P, P, P, P, P, P, P]),
 }
 
 EXPORTED = tuple(_SIGS)

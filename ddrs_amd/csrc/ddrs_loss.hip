@@ -4,8 +4,9 @@
 //  - tau_trim_and_downsample (src/training/loss.rs:33-95): hours
 //    [tau, T - (24 - tau)) pooled into days with the area weights of
 //    area_pool_weights, computed in float32 as the reference does;
-//  - nse_batch_loss (src/training/loss.rs:226-237) with the driver's warmup
-//    slice and NaN-gauge filter (src/training/driver.rs:150-195);
+//  - the batch losses (src/training/loss.rs:179-373: L1, nse-batch, nnse-kge,
+//    kge) with the driver's warmup slice and NaN-gauge filter
+//    (src/training/driver.rs:150-195);
 //  - the three fused into the objective of one micro-batch, with its gradient
 //    straight into dL/d(routed discharge).
 // Everything is deterministic: sums run in a fixed order (the reference's k
@@ -35,6 +36,8 @@ struct ddrs_gauge_plan {
     double* part;    // [G]
     int* valid;      // [G]
     int* n_valid;    // [1]
+    double* gstats;  // [G][8] per-gauge moments of the KGE objectives
+    float* gcoef;    // [G][6] their gradient coefficients and means
 };
 
 namespace ddrs {
@@ -290,6 +293,139 @@ __global__ void k_nse_bwd(const float* p, const float* o, const float* sigma, in
     gp[e] = v;
 }
 
+// The per-gauge objectives nnse-kge and kge (loss.rs:257-373) over days
+// [t0, T). Moments of gauge g, in double: st[g] = {mean p, mean o, population
+// variances of p and o, covariance, sum of squared residuals, sum of squared
+// centred o}; valid[g] = no NaN in its observations there.
+__global__ __launch_bounds__(256) void k_kge_stats(const float* p, const float* o, int T, int t0,
+                                                   double* st, int* valid) {
+    __shared__ double sh[256];
+    __shared__ int bad;
+    const int g = blockIdx.x;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    const float* pg = p + static_cast<size_t>(g) * T;
+    const float* og = o + static_cast<size_t>(g) * T;
+    double sp = 0.0, so = 0.0;
+    for (int t = t0 + threadIdx.x; t < T; t += blockDim.x) {
+        const float ov = og[t];
+        if (isnan(ov)) bad = 1;
+        sp += pg[t];
+        so += ov;
+    }
+    const double n = static_cast<double>(T - t0);
+    const double mp = block_sum(sp, sh) / n;
+    const double mo = block_sum(so, sh) / n;
+    double vp = 0.0, vo = 0.0, cv = 0.0, se = 0.0;
+    for (int t = t0 + threadIdx.x; t < T; t += blockDim.x) {
+        const double pc = pg[t] - mp, oc = og[t] - mo;
+        const double r = static_cast<double>(pg[t]) - static_cast<double>(og[t]);
+        vp += pc * pc;
+        vo += oc * oc;
+        cv += pc * oc;
+        se += r * r;
+    }
+    vp = block_sum(vp, sh);
+    vo = block_sum(vo, sh);
+    cv = block_sum(cv, sh);
+    se = block_sum(se, sh);
+    if (threadIdx.x == 0) {
+        double* s = st + static_cast<size_t>(g) * 8;
+        s[0] = mp;
+        s[1] = mo;
+        s[2] = vp / n;
+        s[3] = vo / n;
+        s[4] = cv / n;
+        s[5] = se;
+        s[6] = vo;
+        valid[g] = bad ? 0 : 1;
+    }
+}
+
+// Loss (mean over kept gauges) and, per kept gauge, the coefficients of
+// dL/dp_t = A (p_t - mean p) + B (o_t - mean o) + C (p_t - o_t) + D, from
+// d r = (o_t - mean o) / (n sp so) - r (p_t - mean p) / (n sp^2),
+// d alpha = (p_t - mean p) / (n sp so), d beta = 1 / (n (mean o + eps)) and
+// d (1 - NNSE) = 2 NNSE^2 (p_t - o_t) / sso (sp, so: the std's with eps).
+__global__ __launch_bounds__(256) void k_kge_final(const double* st, const int* valid, int G,
+                                                   int days, ddrs_loss_config cfg, float* loss,
+                                                   int* n_valid, int* n_valid_out,
+                                                   float* coef) {
+    __shared__ double sh[256];
+    double cnt = 0.0;
+    for (int g = threadIdx.x; g < G; g += blockDim.x) cnt += valid[g];
+    const double kept = block_sum(cnt, sh);
+    const double n = static_cast<double>(days), eps = cfg.eps;
+    double acc = 0.0;
+    for (int g = threadIdx.x; g < G; g += blockDim.x) {
+        float* c = coef + static_cast<size_t>(g) * 6;
+        const double* s = st + static_cast<size_t>(g) * 8;
+        if (!valid[g]) {
+            for (int i = 0; i < 6; ++i) c[i] = 0.0f;
+            continue;
+        }
+        const double mp = s[0], mo = s[1];
+        const double sp = sqrt(s[2] + eps), so = sqrt(s[3] + eps);
+        const double r = s[4] / (sp * so), alpha = sp / so, beta = mp / (mo + eps);
+        const double sso = s[6] + eps;
+        const double nnse = 1.0 / (1.0 + s[5] / sso);
+        const double dr = r - 1.0, da = alpha - 1.0, db = beta - 1.0;
+        const double jr_pc = -r / (n * sp * sp), jr_oc = 1.0 / (n * sp * so);
+        const double ja_pc = 1.0 / (n * sp * so), jb = 1.0 / (n * (mo + eps));
+        const double g_resid = 2.0 * nnse * nnse / sso;
+        double lg, A, B, C, D;
+        if (cfg.kind == DDRS_LOSS_NNSE_KGE) {
+            const double dist = sqrt(dr * dr + da * da + db * db);
+            lg = cfg.nnse_weight * (1.0 - nnse) + cfg.kge_weight * dist;
+            const double w = cfg.kge_weight / dist;
+            A = w * (dr * jr_pc + da * ja_pc);
+            B = w * dr * jr_oc;
+            D = w * db * jb;
+            C = cfg.nnse_weight * g_resid;
+        } else {
+            const double term = cfg.r_weight * dr * dr + cfg.alpha_weight * da * da +
+                                cfg.beta_weight * db * db;
+            const double m = (term <= cfg.kge_clamp) ? 1.0 : 0.0;
+            lg = fmin(term, static_cast<double>(cfg.kge_clamp));
+            A = 2.0 * m * (cfg.r_weight * dr * jr_pc + cfg.alpha_weight * da * ja_pc);
+            B = 2.0 * m * cfg.r_weight * dr * jr_oc;
+            D = 2.0 * m * cfg.beta_weight * db * jb;
+            C = 0.0;
+            if (cfg.nnse_weight != 0.0f) {
+                lg += cfg.nnse_weight * (1.0 - nnse);
+                C = cfg.nnse_weight * g_resid;
+            }
+        }
+        acc += lg;
+        c[0] = static_cast<float>(A / kept);
+        c[1] = static_cast<float>(B / kept);
+        c[2] = static_cast<float>(C / kept);
+        c[3] = static_cast<float>(D / kept);
+        c[4] = static_cast<float>(mp);
+        c[5] = static_cast<float>(mo);
+    }
+    const double total = block_sum(acc, sh);
+    if (threadIdx.x == 0) {
+        if (loss) *loss = kept > 0 ? static_cast<float>(total / kept) : 0.0f;
+        if (n_valid) *n_valid = static_cast<int>(kept);
+        if (n_valid_out) *n_valid_out = static_cast<int>(kept);
+    }
+}
+
+__global__ void k_kge_bwd(const float* p, const float* o, int G, int T, int t0, const int* valid,
+                          const float* coef, float* gp) {
+    const int g = blockIdx.y;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    const size_t e = static_cast<size_t>(g) * T + t;
+    float v = 0.0f;
+    if (t >= t0 && valid[g]) {
+        const float* c = coef + static_cast<size_t>(g) * 6;
+        v = c[0] * (p[e] - c[4]) + c[1] * (o[e] - c[5]) + c[2] * (p[e] - o[e]) + c[3];
+    }
+    gp[e] = v;
+}
+
 dim3 grid2(int n, int rows) { return dim3((n + 255) / 256, rows); }
 
 ddrs_status hip_check(const char* what) {
@@ -370,6 +506,8 @@ ddrs_status ddrs_gauge_plan_create(int32_t n_reaches, int32_t num_gauges, int32_
     dalloc(reinterpret_cast<void**>(&p->part), G * 8);
     dalloc(reinterpret_cast<void**>(&p->valid), G * 4);
     dalloc(reinterpret_cast<void**>(&p->n_valid), 4);
+    dalloc(reinterpret_cast<void**>(&p->gstats), G * 8 * 8);
+    dalloc(reinterpret_cast<void**>(&p->gcoef), G * 6 * 4);
     auto up = [&](void* d, const void* h, size_t bytes) {
         if (ok && bytes && hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) != hipSuccess) ok = false;
     };
@@ -393,6 +531,7 @@ void ddrs_gauge_plan_destroy(ddrs_gauge_plan_t p) {
                     static_cast<void*>(p->ugroup), static_cast<void*>(p->hourly),
                     static_cast<void*>(p->daily), static_cast<void*>(p->gdaily),
                     static_cast<void*>(p->part), static_cast<void*>(p->valid),
+                    static_cast<void*>(p->gstats), static_cast<void*>(p->gcoef),
                     static_cast<void*>(p->n_valid)})
         if (q) (void)hipFree(q);
     delete p;
@@ -498,19 +637,25 @@ ddrs_status ddrs_gauge_nse_loss(ddrs_gauge_plan_t p, const float* d_runoff, int3
                                 float* d_loss, int32_t* d_n_valid, float* d_grunoff,
                                 float* d_hourly, void* stream) {
     if (!d_sigma) return lfail(DDRS_ERR_INVALID, "gauge_nse_loss: NULL sigma");
-    return ddrs_gauge_batch_loss(p, DDRS_LOSS_NSE_BATCH, d_runoff, T_hours, stride_n, stride_t,
-                                 tau, warmup, d_obs, d_sigma, eps, d_loss, d_n_valid, d_grunoff,
-                                 d_hourly, stream);
+    ddrs_loss_config cfg{};
+    cfg.kind = DDRS_LOSS_NSE_BATCH;
+    cfg.eps = eps;
+    return ddrs_gauge_batch_loss(p, &cfg, d_runoff, T_hours, stride_n, stride_t, tau, warmup,
+                                 d_obs, d_sigma, d_loss, d_n_valid, d_grunoff, d_hourly, stream);
 }
 
-ddrs_status ddrs_gauge_batch_loss(ddrs_gauge_plan_t p, int32_t kind, const float* d_runoff,
-                                  int32_t T_hours, int64_t stride_n, int64_t stride_t,
-                                  int32_t tau, int32_t warmup, const float* d_obs,
-                                  const float* d_sigma, float eps, float* d_loss,
+ddrs_status ddrs_gauge_batch_loss(ddrs_gauge_plan_t p, const ddrs_loss_config* cfg,
+                                  const float* d_runoff, int32_t T_hours, int64_t stride_n,
+                                  int64_t stride_t, int32_t tau, int32_t warmup,
+                                  const float* d_obs, const float* d_sigma, float* d_loss,
                                   int32_t* d_n_valid, float* d_grunoff, float* d_hourly,
                                   void* stream) {
     int start, L, M;
-    if (kind != DDRS_LOSS_L1 && kind != DDRS_LOSS_NSE_BATCH)
+    if (!cfg) return lfail(DDRS_ERR_INVALID, "gauge_batch_loss: NULL loss config");
+    const int kind = cfg->kind;
+    const float eps = cfg->eps;
+    const bool per_gauge = kind == DDRS_LOSS_NNSE_KGE || kind == DDRS_LOSS_KGE;
+    if (kind != DDRS_LOSS_L1 && kind != DDRS_LOSS_NSE_BATCH && !per_gauge)
         return lfail(DDRS_ERR_INVALID, "gauge_batch_loss: unknown loss kind");
     if (!p || !d_runoff || !d_obs || !d_loss || (kind == DDRS_LOSS_NSE_BATCH && !d_sigma))
         return lfail(DDRS_ERR_INVALID, "gauge_batch_loss: NULL argument");
@@ -538,14 +683,25 @@ ddrs_status ddrs_gauge_batch_loss(ddrs_gauge_plan_t p, int32_t kind, const float
         hipLaunchKernelGGL(k_area_pool, grid2(M, G), dim3(256), 0, s, hourly, T_hours, start, L,
                            M, p->daily);
     }
-    hipLaunchKernelGGL(k_nse_partial, dim3(G), dim3(256), 0, s, p->daily, d_obs, d_sigma, M,
-                       warmup, eps, 1, static_cast<int>(kind), p->part, p->valid);
-    hipLaunchKernelGGL(k_nse_final, dim3(1), dim3(256), 0, s, p->part, p->valid, G, M - warmup,
-                       d_loss, p->n_valid, d_n_valid);
+    if (per_gauge) {
+        hipLaunchKernelGGL(k_kge_stats, dim3(G), dim3(256), 0, s, p->daily, d_obs, M, warmup,
+                           p->gstats, p->valid);
+        hipLaunchKernelGGL(k_kge_final, dim3(1), dim3(256), 0, s, p->gstats, p->valid, G,
+                           M - warmup, *cfg, d_loss, p->n_valid, d_n_valid, p->gcoef);
+    } else {
+        hipLaunchKernelGGL(k_nse_partial, dim3(G), dim3(256), 0, s, p->daily, d_obs, d_sigma, M,
+                           warmup, eps, 1, kind, p->part, p->valid);
+        hipLaunchKernelGGL(k_nse_final, dim3(1), dim3(256), 0, s, p->part, p->valid, G,
+                           M - warmup, d_loss, p->n_valid, d_n_valid);
+    }
     if (d_grunoff && p->U > 0) {
-        hipLaunchKernelGGL(k_nse_bwd, grid2(M, G), dim3(256), 0, s, p->daily, d_obs, d_sigma, G,
-                           M, warmup, eps, static_cast<int>(kind), nullptr, p->valid, p->n_valid,
-                           p->gdaily);
+        if (per_gauge)
+            hipLaunchKernelGGL(k_kge_bwd, grid2(M, G), dim3(256), 0, s, p->daily, d_obs, G, M,
+                               warmup, p->valid, p->gcoef, p->gdaily);
+        else
+            hipLaunchKernelGGL(k_nse_bwd, grid2(M, G), dim3(256), 0, s, p->daily, d_obs, d_sigma,
+                               G, M, warmup, eps, kind, nullptr, p->valid, p->n_valid,
+                               p->gdaily);
         if (rows) {
             hipLaunchKernelGGL(k_area_pool_bwd_tg, dim3(T_hours, (G + 255) / 256), dim3(256), 0, s,
                                p->gdaily, G, start, L, M, p->hourly);

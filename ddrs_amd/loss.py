@@ -157,12 +157,24 @@ def nse_batch_loss(p: torch.Tensor, o: torch.Tensor, sigma: torch.Tensor, eps: f
     return _Nse.apply(p, o, sigma, eps, plan)
 
 
-_KINDS = {"l1": N.LOSS_L1, "nse-batch": N.LOSS_NSE_BATCH}
+_KINDS = {"l1": N.LOSS_L1, "nse-batch": N.LOSS_NSE_BATCH, "nnse-kge": N.LOSS_NNSE_KGE,
+          "kge": N.LOSS_KGE}
+# LossConfig's defaults (src/config.rs:250-262)
+LOSS_WEIGHTS = dict(nnse_weight=1.0, kge_weight=1.0, r_weight=1.0, alpha_weight=1.0,
+                    beta_weight=1.0, kge_clamp=10.0)
+
+
+def _loss_config(kind: str, eps: float, weights: dict) -> "N.LossConfig":
+    w = {**LOSS_WEIGHTS, **(weights or {})}
+    unknown = set(w) - set(LOSS_WEIGHTS)
+    if unknown:
+        raise ValueError(f"unknown loss weights {sorted(unknown)}")
+    return N.LossConfig(_KINDS[kind], float(eps), *[float(w[k]) for k in LOSS_WEIGHTS])
 
 
 class _GaugeNse(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, runoff, plan, time_major, tau, warmup, obs, sigma, eps, kind):
+    def forward(ctx, runoff, plan, time_major, tau, warmup, obs, sigma, eps, kind, weights):
         x = _cuda_f32(runoff, "runoff")
         o = _cuda_f32(obs, "obs")
         s = _cuda_f32(sigma, "sigma") if sigma is not None else None
@@ -171,10 +183,11 @@ class _GaugeNse(torch.autograd.Function):
         n_valid = torch.empty((), device=x.device, dtype=torch.int32)
         hourly = torch.empty(plan.num_gauges, T, device=x.device)
         g = torch.zeros_like(x) if ctx.needs_input_grad[0] else None
-        N.check(N.lib.ddrs_gauge_batch_loss(plan.handle, _KINDS[kind], _ptr(x), T, sn, st,
-                                            int(tau), int(warmup), _ptr(o), _ptr(s), float(eps),
-                                            _ptr(loss), _ptr(n_valid), _ptr(g), _ptr(hourly),
-                                            _stream()), "ddrs_gauge_batch_loss")
+        cfg = _loss_config(kind, eps, weights)
+        N.check(N.lib.ddrs_gauge_batch_loss(plan.handle, ctypes.byref(cfg), _ptr(x), T, sn, st,
+                                            int(tau), int(warmup), _ptr(o), _ptr(s), _ptr(loss),
+                                            _ptr(n_valid), _ptr(g), _ptr(hourly), _stream()),
+                "ddrs_gauge_batch_loss")
         ctx.save_for_backward(g if g is not None else torch.empty(0, device=x.device))
         ctx.mark_non_differentiable(n_valid, hourly)
         return loss, n_valid, hourly
@@ -182,7 +195,7 @@ class _GaugeNse(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gl, _gn, _gh):
         (g,) = ctx.saved_tensors
-        return g * gl.to(torch.float32), None, None, None, None, None, None, None, None
+        return g * gl.to(torch.float32), None, None, None, None, None, None, None, None, None
 
 
 def gauge_nse_loss(runoff: torch.Tensor, plan: GaugePlan, tau: int, warmup: int,
@@ -194,17 +207,23 @@ def gauge_nse_loss(runoff: torch.Tensor, plan: GaugePlan, tau: int, warmup: int,
     n_valid is loss_denominator (kept gauges x post-warmup days; 0 and loss 0
     when every gauge has a NaN), hourly the [G, T] readout. obs is
     [G, T_days] with NaN for missing days, sigma [G]."""
-    return _GaugeNse.apply(runoff, plan, time_major, tau, warmup, obs, sigma, eps, "nse-batch")
+    return _GaugeNse.apply(runoff, plan, time_major, tau, warmup, obs, sigma, eps, "nse-batch",
+                           None)
 
 
 def gauge_batch_loss(runoff: torch.Tensor, plan: GaugePlan, tau: int, warmup: int,
                      obs: torch.Tensor, sigma: Optional[torch.Tensor] = None, eps: float = 0.1,
-                     kind: str = "l1", time_major: bool = True):
-    """gauge_nse_loss for a loss.kind of batch_loss (src/training/loss.rs:179-208):
-    "l1" (the reference's default, mean |p - o|; sigma and eps unused) or
-    "nse-batch" (sigma required)."""
+                     kind: str = "l1", time_major: bool = True, **weights):
+    """gauge_nse_loss for any loss.kind of batch_loss (src/training/loss.rs:179-373):
+    "l1" (the reference's default, mean |p - o|), "nse-batch" (sigma
+    required), "nnse-kge" or "kge" (per-gauge objectives averaged over
+    gauges; weights nnse_weight, kge_weight, r_weight, alpha_weight,
+    beta_weight, kge_clamp as LossConfig, default 1, 1, 1, 1, 1, 10). n_valid
+    is loss_denominator: kept gauges x days, or kept gauges for the per-gauge
+    kinds."""
     if kind not in _KINDS:
         raise ValueError(f"loss kind must be one of {sorted(_KINDS)}, got {kind!r}")
     if kind == "nse-batch" and sigma is None:
         raise ValueError("loss kind nse-batch needs the per-gauge sigma")
-    return _GaugeNse.apply(runoff, plan, time_major, tau, warmup, obs, sigma, eps, kind)
+    return _GaugeNse.apply(runoff, plan, time_major, tau, warmup, obs, sigma, eps, kind,
+                           weights)

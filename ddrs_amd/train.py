@@ -171,11 +171,11 @@ class Trainer:
                  warmup: int = 5, eps: float = 0.1, grad_clip: float = 1.0,
                  window: Optional[int] = None, form: int = N.FORM_PERSISTENT,
                  network: Optional[Network] = None, loss: str = "nse-batch",
-                 optimizer: str = "adadelta"):
+                 optimizer: str = "adadelta", loss_weights: Optional[dict] = None):
         if "n" not in head.learnable or "q_spatial" not in head.learnable:
             raise ValueError("the head must produce n and q_spatial")
-        if loss not in ("l1", "nse-batch"):
-            raise ValueError(f"loss must be 'l1' or 'nse-batch', got {loss!r}")
+        if loss not in ("l1", "nse-batch", "nnse-kge", "kge"):
+            raise ValueError(f"loss must be 'l1', 'nse-batch', 'nnse-kge' or 'kge', got {loss!r}")
         if optimizer not in ("adam", "adadelta"):
             raise ValueError(f"optimizer must be 'adam' or 'adadelta', got {optimizer!r}")
         self.head = head
@@ -183,6 +183,7 @@ class Trainer:
         self.tau, self.warmup, self.eps, self.grad_clip = int(tau), int(warmup), float(eps), grad_clip
         self.window, self.form = window, form
         self.loss_kind = loss
+        self.loss_weights = dict(loss_weights or {})
         self.opt = (AdaDelta if optimizer == "adadelta" else Adam)(head.parameters())
         self._net: Optional[Network] = network
         self._adj = network.adj if network is not None else None
@@ -241,7 +242,7 @@ class Trainer:
         plan = self._gauge_plan(Qg.shape[1], Qg.shape[0])
         loss, n_valid, _ = gauge_batch_loss(Qg, plan, self.tau, self.warmup, batch.observations,
                                             batch.gauge_obs_std, self.eps, kind=self.loss_kind,
-                                            time_major=True)
+                                            time_major=True, **self.loss_weights)
         return loss, n_valid
 
     def step(self, batch: GaugedBatch, lr: Optional[float] = None) -> StepOutcome:

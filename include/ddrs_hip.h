@@ -418,15 +418,35 @@ ddrs_status ddrs_gauge_nse_loss(ddrs_gauge_plan_t plan, const float* d_runoff, i
                                 const float* d_obs, const float* d_sigma, float eps,
                                 float* d_loss, int32_t* d_n_valid, float* d_grunoff,
                                 float* d_hourly, void* stream);
-/* The same objective for the loss kinds of batch_loss (src/training/loss.rs:179-208,
- * LossKind, src/config.rs:269-289): DDRS_LOSS_L1, the mean of |p - o| (loss.rs:186; its
- * gradient sign(p - o) / n_valid, 0 where p == o; d_sigma and eps unused, d_sigma may be
- * NULL), or DDRS_LOSS_NSE_BATCH (as ddrs_gauge_nse_loss). */
-enum { DDRS_LOSS_L1 = 0, DDRS_LOSS_NSE_BATCH = 1 };
-ddrs_status ddrs_gauge_batch_loss(ddrs_gauge_plan_t plan, int32_t kind, const float* d_runoff,
-                                  int32_t T_hours, int64_t stride_n, int64_t stride_t,
-                                  int32_t tau, int32_t warmup, const float* d_obs,
-                                  const float* d_sigma, float eps, float* d_loss,
+/* The same objective for every loss kind of batch_loss (src/training/loss.rs:179-208;
+ * LossKind and LossConfig, src/config.rs:220-289):
+ *   DDRS_LOSS_L1         the mean of |p - o| (loss.rs:186; gradient sign(p - o) / n_valid,
+ *                        0 where p == o; d_sigma unused and may be NULL);
+ *   DDRS_LOSS_NSE_BATCH  as ddrs_gauge_nse_loss;
+ *   DDRS_LOSS_NNSE_KGE   per kept gauge nnse_weight (1 - NNSE) + kge_weight (1 - KGE),
+ *                        averaged over gauges (loss.rs:257-318);
+ *   DDRS_LOSS_KGE        per kept gauge min(r_w (r-1)^2 + alpha_w (alpha-1)^2 +
+ *                        beta_w (beta-1)^2, kge_clamp) + nnse_weight (1 - NNSE), averaged
+ *                        over gauges (loss.rs:320-373; the clamp passes gradient where the
+ *                        sum is at most kge_clamp).
+ * *d_n_valid is loss_denominator (loss.rs:164-169): kept gauges x days for the first two,
+ * kept gauges for the per-gauge kinds. Where the reference's gradient is NaN (1 - KGE = 0
+ * in NNSE_KGE), so is this one. */
+enum { DDRS_LOSS_L1 = 0, DDRS_LOSS_NSE_BATCH = 1, DDRS_LOSS_NNSE_KGE = 2, DDRS_LOSS_KGE = 3 };
+typedef struct ddrs_loss_config {
+    int32_t kind;       /* DDRS_LOSS_* */
+    float eps;          /* LossConfig.eps (0.1) */
+    float nnse_weight;  /* 1 */
+    float kge_weight;   /* 1 */
+    float r_weight;     /* 1 */
+    float alpha_weight; /* 1 */
+    float beta_weight;  /* 1 */
+    float kge_clamp;    /* 10 */
+} ddrs_loss_config;
+ddrs_status ddrs_gauge_batch_loss(ddrs_gauge_plan_t plan, const ddrs_loss_config* cfg,
+                                  const float* d_runoff, int32_t T_hours, int64_t stride_n,
+                                  int64_t stride_t, int32_t tau, int32_t warmup,
+                                  const float* d_obs, const float* d_sigma, float* d_loss,
                                   int32_t* d_n_valid, float* d_grunoff, float* d_hourly,
                                   void* stream);
 

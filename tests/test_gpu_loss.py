@@ -229,3 +229,54 @@ def test_fused_l1_matches_oracle(time_major):
     assert np.abs(g - wg).max() <= 1e-5 * np.abs(wg).max()
     with pytest.raises(ValueError):
         L.gauge_batch_loss(x, plan, tau, warmup, t32(obs), None, kind="nse-batch")
+
+
+@pytest.mark.parametrize("kind,weights", [
+    ("nnse-kge", {}),
+    ("nnse-kge", dict(nnse_weight=0.5, kge_weight=2.0)),
+    ("kge", {}),
+    ("kge", dict(nnse_weight=0.0, alpha_weight=2.0, kge_clamp=0.3)),
+])
+@pytest.mark.parametrize("time_major", [True, False])
+def test_fused_per_gauge_kinds_match_oracle(kind, weights, time_major):
+    """The per-gauge objectives nnse-kge and kge (loss.rs:257-373) through the
+    fused call: loss within 1e-5 relative of the float64 oracle, n_valid the
+    kept gauges, the gradient within 1e-5 of its max-abs."""
+    say("fused", kind, weights, time_major)
+    Nr, T, G = 1500, 24 * 20 + 7, 9
+    flat, group, runoff, obs, sigma = _fused_case(12, Nr, T, G, 2, [(5, 8)])
+    tau, warmup = 9, 3
+    plan = L.GaugePlan(Nr, flat, group, G, T)
+    x = t32(runoff.T.copy() if time_major else runoff).requires_grad_(True)
+    loss, n_valid, _ = L.gauge_batch_loss(x, plan, tau, warmup, t32(obs), None, 0.1, kind=kind,
+                                          time_major=time_major, **weights)
+    loss.backward()
+    ow = {("clamp" if k == "kge_clamp" else k): v for k, v in weights.items()}
+    wl, wn, wg = lo.gauge_nse_loss(runoff, flat, group, G, tau, warmup, obs, None, 0.1,
+                                   kind=kind, **ow)
+    say(f"loss {loss.item():.8g} oracle {wl:.8g}")
+    assert int(n_valid) == wn == G - 1
+    assert abs(loss.item() - wl) <= 1e-5 * abs(wl)
+    g = x.grad.cpu().numpy()
+    g = g.T if time_major else g
+    assert np.abs(g - wg).max() <= 1e-5 * np.abs(wg).max()
+
+
+def test_per_gauge_kinds_reference_hand_value():
+    """nnse_kge_loss's hand value (loss.rs:391-403: 0.7 at eps 0) through the
+    fused call: hourly values constant over each pooled day give the daily
+    series [1.5, 2.5, 1.5, 2.5] against observations [1, 3, 1, 3]."""
+    say("per-gauge kinds hand value")
+    days, tau = 4, 0
+    hourly = np.zeros((1, 24 * (days + 1)), np.float32)
+    for j, v in enumerate([1.5, 2.5, 1.5, 2.5]):
+        hourly[0, 24 * j: 24 * (j + 1)] = v
+    plan = L.GaugePlan(1, [0], [0], 1, hourly.shape[1])
+    obs = t32([[1.0, 3.0, 1.0, 3.0]])
+    x = t32(hourly)
+    loss, nv, _ = L.gauge_batch_loss(x, plan, tau, 0, obs, None, 0.0, kind="nnse-kge",
+                                     time_major=False)
+    assert abs(loss.item() - 0.7) < 1e-6 and int(nv) == 1
+    loss, _, _ = L.gauge_batch_loss(x, plan, tau, 0, obs, None, 0.0, kind="kge",
+                                    time_major=False, nnse_weight=0.0, kge_clamp=1e9)
+    assert abs(loss.item() - 0.25) < 1e-6

@@ -120,3 +120,62 @@ def test_gauge_loss_matches_central_differences_and_filter():
     obs_nan = obs.copy()
     obs_nan[:, -1] = np.nan
     assert lo.gauge_nse_loss(runoff, flat, group, G, 7, 1, obs_nan, sigma, 0.1)[:2] == (0.0, 0)
+
+
+def test_kge_objectives_reference_values():
+    """nnse_kge_loss and kge_component_loss at the reference's hand values and
+    properties (src/training/loss.rs:391-490)."""
+    p = np.array([[1.5, 2.5, 1.5, 2.5]])
+    o = np.array([[1.0, 3.0, 1.0, 3.0]])
+    # 1 - KGE = 0.5 (alpha 0.5), 1 - NNSE = 0.2
+    assert abs(lo.kge_batch_loss(p, o, "nnse-kge", 0.0)[0] - 0.7) < 1e-12
+    # perfect prediction: 0; attenuated: larger
+    assert lo.kge_batch_loss(o, o, "nnse-kge", 0.0, nnse_weight=0.0)[0] < 1e-12
+    assert lo.kge_batch_loss(p, o, "nnse-kge", 0.0, nnse_weight=0.0)[0] > 0.1
+    # the components: (0.5 - 1)^2 = 0.25, doubled by alpha_weight 2
+    l1 = lo.kge_batch_loss(p, o, "kge", 0.0, nnse_weight=0.0, clamp=1e9)[0]
+    l2 = lo.kge_batch_loss(p, o, "kge", 0.0, nnse_weight=0.0, alpha_weight=2.0, clamp=1e9)[0]
+    assert abs(l1 - 0.25) < 1e-12 and abs(l2 - 2 * l1) < 1e-12
+    # a collapsed-variance gauge blows up without the clamp, not with it
+    p2 = np.array([[1.5, 2.5, 1.5, 2.5], [10.0, 90.0, 10.0, 90.0]])
+    o2 = np.array([[1.0, 3.0, 1.0, 3.0], [1.0, 1.0, 1.0, 1.0001]])
+    assert lo.kge_batch_loss(p2, o2, "kge", 1e-6, nnse_weight=0.0, alpha_weight=2.0,
+                             clamp=1e9)[0] > 100.0
+    assert lo.kge_batch_loss(p2, o2, "kge", 1e-6, nnse_weight=0.0, alpha_weight=2.0,
+                             clamp=5.0)[0] <= 5.0 + 1e-9
+    # gradients push the attenuated peaks (1, 3) up and the troughs (0, 2) down
+    for kind in ("nnse-kge", "kge"):
+        _, g = lo.kge_batch_loss(p, o, kind, 0.1, nnse_weight=0.0)
+        assert g[0, 1] < 0 and g[0, 3] < 0 and g[0, 0] > 0 and g[0, 2] > 0
+
+
+@pytest.mark.parametrize("kind,weights", [
+    ("nnse-kge", dict(nnse_weight=0.7, kge_weight=1.3)),
+    ("kge", dict(nnse_weight=0.5, r_weight=1.2, alpha_weight=2.0, beta_weight=0.8, clamp=10.0)),
+    ("kge", dict(nnse_weight=0.0, clamp=0.05)),
+])
+def test_kge_gradients_against_finite_differences(kind, weights):
+    rng = np.random.default_rng(3)
+    P = rng.lognormal(0, 0.5, (3, 9))
+    O = rng.lognormal(0, 0.5, (3, 9))
+    _, g = lo.kge_batch_loss(P, O, kind, 0.1, **weights)
+    h = 1e-6
+    for i in range(3):
+        for j in range(9):
+            Pp, Pm = P.copy(), P.copy()
+            Pp[i, j] += h
+            Pm[i, j] -= h
+            fd = (lo.kge_batch_loss(Pp, O, kind, 0.1, **weights)[0]
+                  - lo.kge_batch_loss(Pm, O, kind, 0.1, **weights)[0]) / (2 * h)
+            assert abs(fd - g[i, j]) <= 1e-7 * max(1.0, np.abs(g).max()), (i, j, fd, g[i, j])
+
+
+def test_gauge_objective_per_gauge_kinds_count_gauges():
+    rng = np.random.default_rng(4)
+    runoff = rng.lognormal(0, 0.5, (6, 24 * 8)).astype(np.float32)
+    obs = rng.lognormal(0, 0.5, (3, 7)).astype(np.float32)
+    obs[1, 4] = np.nan
+    flat, group = [0, 1, 2, 3, 4, 5], [0, 0, 1, 1, 2, 2]
+    for kind in ("nnse-kge", "kge"):
+        loss, nv, g = lo.gauge_nse_loss(runoff, flat, group, 3, 5, 2, obs, None, 0.1, kind=kind)
+        assert nv == 2 and np.isfinite(loss) and not g[2:4].any() and g[[0, 1, 4, 5]].any()
