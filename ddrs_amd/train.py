@@ -200,7 +200,8 @@ class Trainer:
 
     def _static_inputs(self, batch: GaugedBatch, dev):
         """length, slope floored at its minimum (mmc.rs:259-264), p_spatial at
-        its default, x_storage 0.3 (training/forward.rs:322-334)."""
+        its default (used when the head does not produce it), x_storage 0.3
+        (training/forward.rs:322-334)."""
         if self._statics is None:
             adj, P = batch.adjacency, self.params
             f = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32), device=dev)
@@ -230,6 +231,8 @@ class Trainer:
         rng, log = P.parameter_ranges, set(P.log_space_parameters)
         n = denormalize(out["n"], rng.n, "n" in log)
         qsp = denormalize(out["q_spatial"], rng.q_spatial, "q_spatial" in log)
+        if "p_spatial" in out:   # learned when the head produces it (forward.rs:319-373)
+            p = denormalize(out["p_spatial"], rng.p_spatial, "p_spatial" in log)
         q0 = hotstart(net, P, batch.q_prime[0].detach())
         return route_window_gauged(net, P, n, qsp, p, batch.q_prime.detach(), q0.detach(),
                                    length, slope, batch.gauges, x_storage=x, form=self.form,
@@ -237,7 +240,8 @@ class Trainer:
 
     def loss(self, batch: GaugedBatch):
         """(loss, n_valid): tau trim and daily pooling, warmup slice, NaN-gauge
-        filter and batch NSE (driver.rs:126-201) of predict(batch)."""
+        filter and the configured objective (driver.rs:126-201) of
+        predict(batch); n_valid is loss_denominator."""
         Qg = self.predict(batch)
         plan = self._gauge_plan(Qg.shape[1], Qg.shape[0])
         loss, n_valid, _ = gauge_batch_loss(Qg, plan, self.tau, self.warmup, batch.observations,

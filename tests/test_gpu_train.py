@@ -201,3 +201,49 @@ def test_gradient_accumulation_is_the_pooled_mean():
     for c, p in zip(cpu, hb.parameters()):
         np.testing.assert_allclose(p.detach().cpu().numpy(), c.detach().numpy(), rtol=1e-6,
                                    atol=1e-9)
+
+
+def test_learned_p_spatial_gets_its_gradient():
+    """A head that also produces p_spatial routes with it (denormalised in log
+    space, forward.rs:319-373): the chain's weight gradients match the oracle
+    chain routed with the device head's three parameters, within 1e-3 max|g|."""
+    say("learned p_spatial")
+    b = make_batch(5)
+    names = ["n", "q_spatial", "p_spatial"]
+    head = K.KanHeadConfig([f"a{i}" for i in range(TC.F)], names, 42).init(DEV)
+    tr = TR.Trainer(head, tau=b.tau, warmup=b.warmup)
+    batch = device_batch(b)
+    loss, nv = tr.loss(batch)
+    loss.backward()
+    import math
+    from oracle import kan_oracle as ko
+    from oracle import loss_oracle as lo
+    from oracle import oracle as O
+    with torch.no_grad():
+        o = head(batch.attributes)
+        rng = tr.params.parameter_ranges
+        n = (o["n"] * (rng.n[1] - rng.n[0]) + rng.n[0]).cpu().numpy()
+        q = (o["q_spatial"] * (rng.q_spatial[1] - rng.q_spatial[0]) + rng.q_spatial[0]).cpu().numpy()
+        lo_, hi_ = math.log(rng.p_spatial[0] + 1e-6), math.log(rng.p_spatial[1])
+        ps = torch.exp(o["p_spatial"] * (hi_ - lo_) + lo_).cpu().numpy()
+    p = TC.oracle_params(head)
+    out, cache = ko.kan_head_fwd(b.x, p, head.k)
+    q0 = O.hotstart(b.P, b.pat, b.qprime[0])
+    Q = O.route_fwd(b.P, b.pat, n, q, ps, b.length, b.slope, b.qprime, q0=q0)
+    G = b.gauges.size
+    wl, wnv, g_run = lo.gauge_nse_loss(Q[:, b.gauges].T, np.arange(G), np.arange(G), G, b.tau,
+                                       b.warmup, b.obs, b.sigma, 0.1)
+    gQ = np.zeros((b.T, b.R), np.float32)
+    gQ[:, b.gauges] = g_run.T
+    ga = O.route_adj(b.P, b.pat, n, q, ps, b.length, b.slope, b.qprime, gQ, q0=q0)
+    gout = np.stack([ga["n"].astype(np.float64) * (rng.n[1] - rng.n[0]),
+                     ga["q_spatial"].astype(np.float64),
+                     ga["p_spatial"].astype(np.float64) * ps.astype(np.float64) * (hi_ - lo_)],
+                    axis=1)
+    g = ko.kan_head_bwd(gout, p, cache)
+    assert int(nv) == wnv and abs(loss.item() - wl) <= 1e-4 * abs(wl)
+    for name in ("input_weight", "output_weight", "output_bias"):
+        got = getattr(head, name).grad.double().cpu().numpy()
+        err = float(np.abs(got - g[name]).max() / np.abs(g[name]).max())
+        print(f"  grad {name}: {err:.3e}")
+        assert err <= 1e-3, name
