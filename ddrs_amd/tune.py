@@ -47,23 +47,19 @@ class TunedSchedule:
 
 def choose_window(net: Network, T: int, budget_bytes: int, dense_grad: bool = False) -> int:
     """The longest checkpoint window (at most T - 1) whose workspace fits
-    budget_bytes. Raises MemoryError when even one step does not fit."""
+    budget_bytes. The size is not monotone in the window (the window's state
+    grows with it, the checkpoints shrink), so windows are tried from the
+    longest down (each try is a host-side size query). Raises MemoryError when
+    none fits."""
     steps = max(int(T) - 1, 1)
-    fits = lambda w: net.workspace_bytes(w, steps, dense_grad) <= budget_bytes
-    if fits(steps):
-        return steps
-    if not fits(1):
-        raise MemoryError(f"a workspace for {steps} steps needs "
-                          f"{net.workspace_bytes(1, steps, dense_grad)} bytes even with "
-                          f"window 1; the budget is {budget_bytes}")
-    lo, hi = 1, steps  # fits(lo), not fits(hi)
-    while hi - lo > 1:
-        mid = (lo + hi) // 2
-        if fits(mid):
-            lo = mid
-        else:
-            hi = mid
-    return lo
+    smallest = None
+    for w in range(steps, 0, -1):
+        b = net.workspace_bytes(w, steps, dense_grad)
+        if b <= budget_bytes:
+            return w
+        smallest = b if smallest is None else min(smallest, b)
+    raise MemoryError(f"a workspace for {steps} steps needs at least {smallest} bytes at any "
+                      f"checkpoint window; the budget is {budget_bytes}")
 
 
 def _default_budget() -> int:
@@ -72,10 +68,11 @@ def _default_budget() -> int:
 
 
 def _probe(net: Network, params: Params, adj: SparseAdjacency, lengths: Sequence[int],
-           q_prime: Optional[torch.Tensor], n: Optional[torch.Tensor],
+           window: int, q_prime: Optional[torch.Tensor], n: Optional[torch.Tensor],
            q_spatial: Optional[torch.Tensor], gauges, repeats: int, dev) -> list:
     """(forward ms, forward + adjoint ms) of persistent gauged windows of each
-    length (timesteps), minimum over `repeats` runs after one warm-up run."""
+    length (timesteps) with checkpoint window at most `window`, minimum over
+    `repeats` runs after one warm-up run."""
     R = adj.n
     steps = max(lengths)
     f = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32), device=dev)
@@ -90,7 +87,7 @@ def _probe(net: Network, params: Params, adj: SparseAdjacency, lengths: Sequence
         qp = torch.ones((steps + 1, R), device=dev)
     q0 = hotstart(net, params, qp[0])
     cp = params.to_c()
-    ws = Workspace(net, steps, steps, dense_grad=False)
+    ws = Workspace(net, min(steps, int(window)), steps, dense_grad=False)
     try:
         ws.set_gauges(np.asarray(gauges if gauges is not None else [R - 1], np.int32))
         s = _stream()
@@ -152,19 +149,22 @@ def tune_schedule(adj: SparseAdjacency, T: int, params: Optional[Params] = None,
         for sl in sub_levels:
             net = Network(adj, int(sl), int(blk))
             window = choose_window(net, T, budget, dense_grad)
-            r = _probe(net, params, adj, lengths, q_prime, n, q_spatial, gauges, repeats, dev)
+            r = _probe(net, params, adj, lengths, window, q_prime, n, q_spatial, gauges,
+                       repeats, dev)
             if direct:
+                # timed as it will run, recomputed segments included
                 f_step = r[0][0] / steps
                 tot = r[0][1]
                 step_ms = tot / steps
+                pred = tot
             else:
                 (f1, t1), (f2, t2) = r[0], r[-1]
                 L1, L2 = lengths[0], lengths[-1]
                 step_ms = (t2 - t1) / (L2 - L1)
                 f_step = (f2 - f1) / (L2 - L1)
                 tot = t2 + step_ms * (steps - L2)
-            last = steps - window * ((steps - 1) // window)   # length of the last segment
-            pred = tot + f_step * (steps - last)                # recomputed forward segments
+                last = steps - window * ((steps - 1) // window)   # length of the last segment
+                pred = tot + f_step * (steps - last)                # recomputed forward segments
             probes.append(dict(sub_levels=int(sl), block=int(blk), window=window,
                                predicted_ms=pred, fwd_ms_per_step=f_step,
                                total_ms_per_step=step_ms, probe_lengths=lengths))

@@ -197,5 +197,8 @@ def test_workspace_size_query_and_window_choice():
     assert TU.choose_window(net, T, full) == T - 1
     w = TU.choose_window(net, T, full - 1)
     assert w < T - 1 and b(w, T - 1, False) <= full - 1 < b(w + 1, T - 1, False)
+    # the size is not monotone in the window: checkpoints grow as it shrinks
+    assert b(1, T - 1, False) > b(2, T - 1, False)
+    assert TU.choose_window(net, T, b(2, T - 1, False)) >= 2
     with pytest.raises(MemoryError):
-        TU.choose_window(net, T, b(1, T - 1, False) - 1)
+        TU.choose_window(net, T, min(b(w, T - 1, False) for w in range(1, T)) - 1)
