@@ -88,6 +88,7 @@ struct ddrs_workspace {
     uint64_t *hist = nullptr, *amsg = nullptr, *slot = nullptr;
     float2* ihist = nullptr;  // [window][P] the forward's upstream sums per reach-step
     unsigned long long* claim_ctr = nullptr;  // chunk claims of persistent launches
+    LaunchState* ls = nullptr;                // claim base and epoch of the current launch
     unsigned long long claims = 0;            // claims made so far
     float* qps = nullptr;                    // [max_steps][P] q' in schedule order
     float* gqs = nullptr;                    // [max_steps][P] dL/dQ in schedule order
@@ -622,6 +623,7 @@ static void ws_buffers(ddrs_workspace* ws, int32_t flags, F&& f) {
     f(&ws->gbuf, WR * 4);
     f(&ws->slot, R * 8);
     f(&ws->claim_ctr, 8);
+    f(&ws->ls, sizeof(LaunchState));
     f(&ws->lin_ctr, 8);
     f(&ws->err, sizeof(int));
 }
@@ -678,6 +680,7 @@ ddrs_status ddrs_workspace_create_ex(ddrs_plan_t plan, int32_t window, int32_t m
     if (e == hipSuccess) e = hipMemset(ws->amsg, 0, WR * 16);
     if (e == hipSuccess) e = hipMemset(ws->slot, 0, R * 8);
     if (e == hipSuccess) e = hipMemset(ws->claim_ctr, 0, 8);
+    if (e == hipSuccess) e = hipMemset(ws->ls, 0, sizeof(LaunchState));
     if (e == hipSuccess) e = hipMemset(ws->err, 0, sizeof(int));
     if (e != hipSuccess) {
         ddrs_workspace_destroy(ws);
@@ -796,16 +799,20 @@ static bool permute_qprime(ddrs_workspace* ws, const float* q, int rows, hipStre
     return true;
 }
 
-// Claims of the next persistent launch of `grid` workgroups.
-static Claim next_claim(ddrs_workspace* ws, int grid) {
-    Claim c{ws->claim_ctr, ws->claims};
+// Begins the next persistent launch of `grid` workgroups on stream s: a
+// one-thread kernel sets its claim base and epoch on the device (so that a
+// captured launch replays with fresh ones). ws->claims and ws->epoch shadow
+// them on the host for eager calls.
+static Claim begin_launch(ddrs_workspace* ws, int grid, hipStream_t s) {
+    launch_begin(ws->ls, ws->claim_ctr, s);
     ws->claims += static_cast<unsigned long long>(grid);
-    return c;
+    ++ws->epoch;
+    return Claim{ws->claim_ctr, ws->ls};
 }
 
 static void launch_fwd_ws(ddrs_workspace* ws, FwdArgs a, const ddrs_params* p, hipStream_t s) {
     const int v = variant(p);
-    a.claim = next_claim(ws, ws->grid_fwd[v]);
+    a.claim = begin_launch(ws, ws->grid_fwd[v], s);
     a.trace.rec = ws->trace[0];
     ws->trace_n[0] = ws->grid_fwd[v];
     launch_fwd(a, v, ws->rpl_fwd[v], ws->grid_fwd[v], s);
@@ -884,8 +891,6 @@ static void run_adj_segment(ddrs_workspace* ws, const ddrs_params* p, const AdjS
     auto phases = [&](int s_base, int n) {
         L.s_base = W.s_base = F.s_base = s_base;
         L.nsteps = W.nsteps = F.nsteps = n;
-        W.epoch = ++ws->epoch;
-        W.claim = next_claim(ws, ws->grid_sweep);
         W.trace.rec = ws->trace[1];
         ws->trace_n[1] = ws->grid_sweep;
         if (!have_rec) {
@@ -894,6 +899,7 @@ static void run_adj_segment(ddrs_workspace* ws, const ddrs_params* p, const AdjS
             record(ws, TK_LIN, s);
         }
         record(ws, TK_SWEEP, s);
+        W.claim = begin_launch(ws, ws->grid_sweep, s);
         launch_sweep(W, ws->rpl_sweep, ws->grid_sweep, s);
         record(ws, TK_SWEEP, s);
         record(ws, TK_FIN, s);
@@ -923,7 +929,6 @@ ddrs_status ddrs_mc_step_fwd(ddrs_workspace_t ws, const ddrs_params* params, con
     a.t0 = 0;
     a.s_base = 0;
     a.nsteps = 1;
-    a.epoch = ++ws->epoch;
     a.X_out = d_x_sol;
     a.Q_out = nullptr;
     launch_fwd_ws(ws, a, params, s);
@@ -1011,7 +1016,6 @@ ddrs_status run_fwd_segment(ddrs_workspace* ws, const ddrs_params* params, int T
     if (form == DDRS_FORM_PERSISTENT) {
         a.s_base = 0;
         a.nsteps = len;
-        a.epoch = ++ws->epoch;
         const bool conc = lin && ws->lin_in_fwd && len > 0;
         const int n_items = lin_items(len, ws->plan->P);
         if (conc) {
@@ -1035,7 +1039,7 @@ ddrs_status run_fwd_segment(ddrs_workspace* ws, const ddrs_params* params, int T
             L.n_items = n_items;
             L.trace = ws->trace[2];
             L.tag = 1;
-            L.fwd_epoch = a.epoch;
+            L.fwd_epoch = ws->epoch;
             L.fwd_claim_ctr = ws->claim_ctr;
             L.fwd_claims = ws->claims;  // after this forward's claims
             launch_lin(L, v, std::min(L.n_items, ws->lin_grid), ws->lin_stream);
@@ -1053,7 +1057,6 @@ ddrs_status run_fwd_segment(ddrs_workspace* ws, const ddrs_params* params, int T
         for (int k = 0; k < len; ++k) {
             a.s_base = k;
             a.nsteps = 1;
-            a.epoch = ++ws->epoch;
             if (timed) record(ws, TK_FWD, s);
             launch_fwd_ws(ws, a, params, s);
             if (timed) record(ws, TK_FWD, s);
@@ -1255,17 +1258,17 @@ ddrs_status ddrs_tri_solve_fwd(ddrs_workspace_t ws, const float* d_a_values, con
                                float* d_x, void* stream) {
     if (!ws) return fail(DDRS_ERR_INVALID, "workspace is NULL");
     if (!d_a_values || !d_b || !d_x) return fail(DDRS_ERR_INVALID, "NULL argument");
+    hipStream_t s = S(stream);
     TriArgs a{};
     a.sc = ws->plan->sc;
     a.ch = ws->plan->ch[0].dev;
-    a.claim = next_claim(ws, ws->plan->ch[0].C);
-    a.epoch = ++ws->epoch;
+    a.claim = begin_launch(ws, ws->plan->ch[0].C, s);
     a.a_values = d_a_values;
     a.b = d_b;
     a.x = d_x;
     a.slot = ws->slot;
     a.err = ws->err;
-    launch_tri_fwd(a, ws->grid_tri_fwd, S(stream));
+    launch_tri_fwd(a, ws->grid_tri_fwd, s);
     HIPCHK(hipGetLastError());
     return DDRS_OK;
 }
@@ -1280,8 +1283,7 @@ ddrs_status ddrs_tri_solve_adj(ddrs_workspace_t ws, const float* d_a_values, con
     TriArgs a{};
     a.sc = ws->plan->sc;
     a.ch = ws->plan->ch[0].dev;
-    a.claim = next_claim(ws, ws->plan->ch[0].C);
-    a.epoch = ++ws->epoch;
+    a.claim = begin_launch(ws, ws->plan->ch[0].C, s);
     a.a_values = d_a_values;
     a.b = d_grad_x;
     a.x = d_grad_b;
@@ -1303,11 +1305,11 @@ ddrs_status ddrs_hotstart(ddrs_workspace_t ws, const ddrs_params* params,
     ddrs_status st = check_params(params);
     if (st != DDRS_OK) return st;
     if (!d_q_prime0 || !d_q0) return fail(DDRS_ERR_INVALID, "NULL argument");
+    hipStream_t s = S(stream);
     TriArgs a{};
     a.sc = ws->plan->sc;
     a.ch = ws->plan->ch[0].dev;
-    a.claim = next_claim(ws, ws->plan->ch[0].C);
-    a.epoch = ++ws->epoch;
+    a.claim = begin_launch(ws, ws->plan->ch[0].C, s);
     a.a_values = nullptr;  // A = I - N
     a.b = d_q_prime0;
     a.x = d_q0;
@@ -1315,7 +1317,7 @@ ddrs_status ddrs_hotstart(ddrs_workspace_t ws, const ddrs_params* params,
     a.clamp = 1;
     a.lb = params->discharge_lb;
     a.err = ws->err;
-    launch_tri_fwd(a, ws->grid_tri_fwd, S(stream));
+    launch_tri_fwd(a, ws->grid_tri_fwd, s);
     HIPCHK(hipGetLastError());
     return DDRS_OK;
 }
@@ -1326,11 +1328,11 @@ ddrs_status ddrs_hotstart_adj(ddrs_workspace_t ws, const ddrs_params* params, co
     ddrs_status st = check_params(params);
     if (st != DDRS_OK) return st;
     if (!d_q0 || !d_grad_q0 || !d_grad_q_prime0) return fail(DDRS_ERR_INVALID, "NULL argument");
+    hipStream_t s = S(stream);
     TriArgs a{};
     a.sc = ws->plan->sc;
     a.ch = ws->plan->ch[0].dev;
-    a.claim = next_claim(ws, ws->plan->ch[0].C);
-    a.epoch = ++ws->epoch;
+    a.claim = begin_launch(ws, ws->plan->ch[0].C, s);
     a.a_values = nullptr;
     a.b = d_grad_q0;
     a.mask_x = d_q0;
@@ -1338,7 +1340,7 @@ ddrs_status ddrs_hotstart_adj(ddrs_workspace_t ws, const ddrs_params* params, co
     a.x = d_grad_q_prime0;
     a.slot = ws->slot;
     a.err = ws->err;
-    launch_tri_adj(a, ws->grid_tri_adj, S(stream));
+    launch_tri_adj(a, ws->grid_tri_adj, s);
     HIPCHK(hipGetLastError());
     return DDRS_OK;
 }

@@ -61,11 +61,21 @@ struct Chunks {
 // of its links while it waits.
 constexpr int kRepMax = 64;
 
+// Per-launch values of a persistent launch, written on the device by a
+// one-thread kernel enqueued right before it, so that a launch captured in a
+// graph replays with fresh values: the claim counter at the start of the
+// launch, and the epoch that tags the words the launch publishes.
+struct LaunchState {
+    unsigned long long claim_base;
+    uint32_t epoch;
+    uint32_t pad;
+};
+
 // Chunk claims of a persistent launch: the workgroup that increments *ctr to
-// base + k takes the k-th chunk in dependency order.
+// ls->claim_base + k takes the k-th chunk in dependency order.
 struct Claim {
     unsigned long long* ctr;
-    unsigned long long base;
+    const LaunchState* ls;
 };
 
 // Optional per-chunk trace of a persistent launch (null: off): for chunk c,
@@ -88,7 +98,6 @@ struct FwdArgs {
     int t0;          // out[t0] is the window's initial state
     int s_base;      // window-local index of this launch's first step
     int nsteps;      // steps in this launch
-    uint32_t epoch;
     const float* qps;      // [T-1][P] q' in schedule order
     float* Q_out;          // [T][R] reference order or null
     float* X_out;          // [R] pre-clamp solve (single step) or null
@@ -160,7 +169,6 @@ struct SweepArgs {
     const float* gqs;
     const int* gslot;
     int gK;
-    uint32_t epoch;
     const float4* rec;
     uint64_t* amsg;        // [window][P][2] tagged (P, G)
     float* st_lam;         // [P] gradient into x of the step (masked dL/d out[t+1])
@@ -185,7 +193,6 @@ struct TriArgs {
     Sched sc;
     Chunks ch;             // 64-wide
     Claim claim;
-    uint32_t epoch;
     const float* a_values;  // [nnz] or null (A = I - N)
     const float* b;         // [R] reference order
     const float* mask_x;    // [R] or null: gradient masked where mask_x <= lb
@@ -196,6 +203,8 @@ struct TriArgs {
     int* err;
 };
 
+// The LaunchState of the next persistent launch: claim base from *ctr, epoch + 1.
+void launch_begin(LaunchState* ls, const unsigned long long* ctr, hipStream_t s);
 // Persistent launches run one workgroup per chunk (grid = Chunks::C).
 void launch_fwd(const FwdArgs& a, int variant, int rpl, int grid, hipStream_t s);
 // grid: one workgroup per item (a.items null), else `grid` workgroups

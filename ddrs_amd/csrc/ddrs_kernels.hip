@@ -90,17 +90,32 @@ constexpr uint64_t kWatchdogTicks = 200000000ull;  // 2 s at 100 MHz
 // One chunk per workgroup, for all steps of the launch.
 
 // Chunk of this workgroup: the claim counter only grows (claims of earlier
-// launches are subtracted through `base`), so it is never reset. Dir = +1
-// hands out chunks 0, 1, ..., Dir = -1 hands out C-1, C-2, ...
+// launches are subtracted through the launch's claim base), so it is never
+// reset. Dir = +1 hands out chunks 0, 1, ..., Dir = -1 hands out C-1, C-2, ...
+// A claim outside [0, C) (a launch whose state was not begun) returns -1 with
+// the error flag raised; the caller's workgroup then returns at once.
 template <int Dir>
-__device__ __forceinline__ int claim_chunk(const Claim& cl, int C, int* lds) {
+__device__ __forceinline__ int claim_chunk(const Claim& cl, int C, int* lds, int* err) {
     if (threadIdx.x == 0) {
-        const unsigned long long k = atomicAdd(cl.ctr, 1ull) - cl.base;
-        *lds = static_cast<int>(k);
+        const unsigned long long k = atomicAdd(cl.ctr, 1ull) - cl.ls->claim_base;
+        const bool ok = k < static_cast<unsigned long long>(C);
+        if (!ok) atomicExch(err, 1);
+        *lds = ok ? static_cast<int>(k) : -1;
     }
     __syncthreads();
     const int k = uni(*lds);
+    if (k < 0) return -1;
     return (Dir > 0) ? k : (C - 1 - k);
+}
+
+// The tag of the words this launch publishes (see LaunchState).
+__device__ __forceinline__ uint32_t launch_epoch(const Claim& cl) { return cl.ls->epoch; }
+
+__global__ void k_begin_launch(LaunchState* ls, const unsigned long long* ctr) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ls->claim_base = *ctr;
+        ls->epoch = ls->epoch + 1u;
+    }
 }
 
 // For the launch's steps in time order (Dir = +1) or reverse (Dir = -1):
@@ -260,7 +275,9 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
     const int lane = threadIdx.x;
     const Sched& sc = a.sc;
     const int R = sc.R, P = sc.P;
-    const int c = claim_chunk<+1>(a.claim, a.ch.C, &claimed);
+    const int c = claim_chunk<+1>(a.claim, a.ch.C, &claimed, a.err);
+    if (c < 0) return;
+    const uint32_t epoch = launch_epoch(a.claim);
     bool unit = true;  // every diagonal of this step's I - c1 N is exactly 1
     const int rep0 = a.ch.rep_up_ptr[c];
     const int nrep_all = a.ch.rep_up_ptr[c + 1] - rep0;
@@ -353,7 +370,7 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
         for (int k = 0; k < kIssueSlots; ++k) {
             const int e = lane + 64 * k;
             if (e < nlinks && skind[e] < 0) {
-                if (!word_ready(pw[k], a.epoch)) ok = false;
+                if (!word_ready(pw[k], epoch)) ok = false;
                 else sval[e] = word_value(pw[k]);
             }
         }
@@ -371,7 +388,7 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
             for (int k = 0; k < kMore; ++k) {
                 const int e = lane + 64 * (kIssueSlots + k);
                 if (e < nlinks && skind[e] < 0) {
-                    if (!word_ready(wx[k], a.epoch)) ok = false;
+                    if (!word_ready(wx[k], epoch)) ok = false;
                     else sval[e] = word_value(wx[k]);
                 }
             }
@@ -385,7 +402,7 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
         if (nrep <= 0) return true;  // no representatives: read the links again
         const uint64_t* row = a.hist + static_cast<size_t>(a.s_base + s) * P;
         bool ok = true;
-        if (lane < nrep) ok = word_ready(poll(row + spos[srep[lane]]), a.epoch);
+        if (lane < nrep) ok = word_ready(poll(row + spos[srep[lane]]), epoch);
         return __all(ok);
     };
 
@@ -457,7 +474,7 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
         // its solve is done.
 #pragma unroll
         for (int r = 0; r < RPL; ++r)
-            if (my[r] >= 0) publish(row + p0 + r * 64 + lane, x[r], a.epoch);
+            if (my[r] >= 0) publish(row + p0 + r * 64 + lane, x[r], epoch);
         if (a.ihist) {
 #pragma unroll
             for (int r = 0; r < RPL; ++r)
@@ -717,7 +734,9 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
     const int lane = threadIdx.x;
     const Sched& sc = a.sc;
     const int P = sc.P;
-    const int c = claim_chunk<-1>(a.claim, a.ch.C, &claimed);
+    const int c = claim_chunk<-1>(a.claim, a.ch.C, &claimed, a.err);
+    if (c < 0) return;
+    const uint32_t epoch = launch_epoch(a.claim);
     bool unit = true;  // every diagonal of this step's I - c1 N is exactly 1
     const int rep0 = a.ch.rep_dn_ptr[c];
     const int nrep_all = a.ch.rep_dn_ptr[c + 1] - rep0;
@@ -798,7 +817,7 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
         const uint64_t* mrow = a.amsg + static_cast<size_t>(a.s_base + s) * P * 2;
         bool ok = true;
         if (lane < nlinks && skind[lane] < 0) {
-            if (!word_ready(pwP, a.epoch) || !word_ready(pwG, a.epoch)) {
+            if (!word_ready(pwP, epoch) || !word_ready(pwG, epoch)) {
                 ok = false;
             } else {
                 stP[lane] = word_value(pwP);
@@ -824,7 +843,7 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
             for (int k = 0; k < kMore; ++k) {
                 const int e = lane + 64 * (k + 1);
                 if (e < nlinks && skind[e] < 0) {
-                    if (!word_ready(wP[k], a.epoch) || !word_ready(wG[k], a.epoch)) {
+                    if (!word_ready(wP[k], epoch) || !word_ready(wG[k], epoch)) {
                         ok = false;
                     } else {
                         stP[e] = word_value(wP[k]);
@@ -843,7 +862,7 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
         if (nrep <= 0) return true;  // no representatives: read the links again
         const uint64_t* row = a.amsg + static_cast<size_t>(a.s_base + s) * P * 2;
         bool ok = true;
-        if (lane < nrep) ok = word_ready(poll(row + 2 * spos[srep[lane]] + 1), a.epoch);
+        if (lane < nrep) ok = word_ready(poll(row + 2 * spos[srep[lane]] + 1), epoch);
         return __all(ok);
     };
 
@@ -926,8 +945,8 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
         for (int r = 0; r < RPL; ++r) {
             if (my[r] >= 0) {
                 const int j = r * 64 + lane;
-                publish(mrow + 2 * (p0 + j), ldsP[j], a.epoch);
-                publish(mrow + 2 * (p0 + j) + 1, ldsG[j], a.epoch);
+                publish(mrow + 2 * (p0 + j), ldsP[j], epoch);
+                publish(mrow + 2 * (p0 + j) + 1, ldsG[j], epoch);
             }
         }
 #pragma unroll
@@ -979,7 +998,9 @@ __global__ __launch_bounds__(64) void k_tri_fwd(TriArgs a) {
     __shared__ int claimed;
     const int lane = threadIdx.x;
     const Sched& sc = a.sc;
-    const int c = claim_chunk<+1>(a.claim, a.ch.C, &claimed);
+    const int c = claim_chunk<+1>(a.claim, a.ch.C, &claimed, a.err);
+    if (c < 0) return;
+    const uint32_t epoch = launch_epoch(a.claim);
     const int base = sc.up_ptr[c * 64];
     const int end = sc.up_ptr[c * 64 + 64];
     auto prepare = [&](int, bool) {};
@@ -989,7 +1010,7 @@ __global__ __launch_bounds__(64) void k_tri_fwd(TriArgs a) {
             const int u = sc.up_pos[e];
             if ((u >> 6) != c) {
                 const uint64_t wv = poll(a.slot + u);
-                if (!word_ready(wv, a.epoch)) ok = false;
+                if (!word_ready(wv, epoch)) ok = false;
                 else stage[e - base] = word_value(wv);
             }
         }
@@ -1015,7 +1036,7 @@ __global__ __launch_bounds__(64) void k_tri_fwd(TriArgs a) {
                 const float diag = a.a_values ? a.a_values[sc.diag_k[p]] : 1.0f;
                 x = (a.b[sc.perm[p]] - sum) / diag;
                 lds_x[lane] = x;
-                publish(a.slot + p, x, a.epoch);
+                publish(a.slot + p, x, epoch);
             }
             __syncthreads();
         }
@@ -1032,7 +1053,9 @@ __global__ __launch_bounds__(64) void k_tri_adj(TriArgs a) {
     __shared__ int claimed;
     const int lane = threadIdx.x;
     const Sched& sc = a.sc;
-    const int c = claim_chunk<-1>(a.claim, a.ch.C, &claimed);
+    const int c = claim_chunk<-1>(a.claim, a.ch.C, &claimed, a.err);
+    if (c < 0) return;
+    const uint32_t epoch = launch_epoch(a.claim);
     const int base = sc.dn_ptr[c * 64];
     const int end = sc.dn_ptr[c * 64 + 64];
     auto prepare = [&](int, bool) {};
@@ -1042,7 +1065,7 @@ __global__ __launch_bounds__(64) void k_tri_adj(TriArgs a) {
             const int d = sc.dn_pos[e];
             if ((d >> 6) != c) {
                 const uint64_t wv = poll(a.slot + d);
-                if (!word_ready(wv, a.epoch)) ok = false;
+                if (!word_ready(wv, epoch)) ok = false;
                 else stage[e - base] = word_value(wv);
             }
         }
@@ -1070,7 +1093,7 @@ __global__ __launch_bounds__(64) void k_tri_adj(TriArgs a) {
                 const float gin = a.mask_x ? ((a.mask_x[o] > a.lb) ? a.b[o] : 0.0f) : a.b[o];
                 g = (gin - sum) / diag;
                 lds_g[lane] = g;
-                publish(a.slot + p, g, a.epoch);
+                publish(a.slot + p, g, epoch);
             }
             __syncthreads();
         }
@@ -1207,6 +1230,9 @@ static void launch_fwd_r(const FwdArgs& a, int variant, int grid, hipStream_t s)
     }
 }
 
+void launch_begin(LaunchState* ls, const unsigned long long* ctr, hipStream_t s) {
+    hipLaunchKernelGGL(k_begin_launch, dim3(1), dim3(64), 0, s, ls, ctr);
+}
 void launch_fwd(const FwdArgs& a, int variant, int rpl, int grid, hipStream_t s) {
     if (rpl == 2) launch_fwd_r<2>(a, variant, grid, s);
     else launch_fwd_r<1>(a, variant, grid, s);
