@@ -104,6 +104,7 @@ struct ddrs_workspace {
     bool statics_bound = false;
     // last forward window
     int last_T = -1, hist_seg = -1;
+    bool fwd_captured = false;  // the last window forward was enqueued under graph capture
     // Linearising the adjoint while the forward window runs (opt-in): a
     // second stream of default priority, events ordering it against the
     // caller's stream, the work-item counter, and the segment whose records
@@ -1071,6 +1072,12 @@ ddrs_status run_fwd_segment(ddrs_workspace* ws, const ddrs_params* params, int T
 
 extern "C" {
 
+// Whether work enqueued on s is being captured into a graph.
+static bool capturing(hipStream_t s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &st) == hipSuccess && st == hipStreamCaptureStatusActive;
+}
+
 static ddrs_status window_fwd(ddrs_workspace_t ws, const ddrs_params* params, int32_t T,
                               const float* d_q_prime, const float* d_q0, float* d_Q_out,
                               float* d_Qg_out, ddrs_form form, void* stream) {
@@ -1102,6 +1109,9 @@ static ddrs_status window_fwd(ddrs_workspace_t ws, const ddrs_params* params, in
     }
     ws->last_T = T;
     if (nseg == 0) ws->hist_seg = -1;
+    // a captured forward runs only when its graph is replayed: only an adjoint
+    // captured after it may follow it
+    ws->fwd_captured = capturing(S(stream));
     return DDRS_OK;
 }
 
@@ -1129,6 +1139,10 @@ static ddrs_status window_adj(ddrs_workspace_t ws, const ddrs_params* params, in
     ddrs_status st = check_params(params);
     if (st != DDRS_OK) return st;
     if (ws->last_T != T) return fail(DDRS_ERR_INVALID, "no matching ddrs_mc_window_fwd before the adjoint");
+    const bool cap = capturing(S(stream));
+    if (ws->fwd_captured && !cap)
+        return fail(DDRS_ERR_INVALID, "the last window forward was captured in a graph: replay the "
+                                      "graph, or run the forward eagerly, before an eager adjoint");
     if (!d_q0 || (T > 1 && !d_q_prime)) return fail(DDRS_ERR_INVALID, "NULL input");
     if (form != DDRS_FORM_PLAIN && form != DDRS_FORM_PERSISTENT)
         return fail(DDRS_ERR_INVALID, "unknown form");
@@ -1186,6 +1200,12 @@ static ddrs_status window_adj(ddrs_workspace_t ws, const ddrs_params* params, in
         g.ck_i = ck_i(ws, j);
         run_adj_segment(ws, params, g, form, s);
         HIPCHK(hipGetLastError());
+    }
+    if (cap) {
+        // the host's view of the history is that of capture time, not of the
+        // graph's replays: an eager adjoint needs an eager forward first
+        ws->last_T = -1;
+        ws->hist_seg = -1;
     }
     if (d_grad_q0) launch_grad_q0(sc, ws->st_lam, d_q0, params->discharge_lb, d_grad_q0, s);
     if (d_grad_n) launch_unpermute(sc, ws->acc_n, d_grad_n, s);

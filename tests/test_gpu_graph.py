@@ -101,3 +101,18 @@ def test_captured_window_pair_replays_bitwise():
                 assert torch.equal(got, w), f"replay {rep} trial {i}: {name}"
             # eager work between replays
             _ = (qpi * 2.0).sum().item()
+    # the host cannot know which forward the graph left in the history: an
+    # eager adjoint now needs an eager forward first
+    with pytest.raises(NAT.DdrsError):
+        NAT.check(NAT.lib.ddrs_mc_window_adj_gauged(ws.handle, cp, T, ptr(qp), ptr(q0),
+                                                    ptr(gQg), ptr(gn), ptr(gq), None, None, None,
+                                                    NAT.FORM_PERSISTENT,
+                                                    torch.cuda.current_stream().cuda_stream))
+    qpi, q0i, gi = trials[1]
+    qp.copy_(qpi)
+    q0.copy_(q0i)
+    gQg.copy_(gi)
+    pair(torch.cuda.current_stream().cuda_stream)
+    ws.check()
+    for got, w in zip((Qg, gn, gq), want[1]):
+        assert torch.equal(got, w)
