@@ -112,6 +112,9 @@ def main():
     ap.add_argument("--sublevels", type=int, default=0,
                     help="largest chain of links in one schedule block (0: library default)")
     ap.add_argument("--block", type=int, default=0, help="schedule block, 64 or 128 (0: default)")
+    ap.add_argument("--graph", action="store_true",
+                    help="also time the step captured in a HIP graph and replayed (and the plain "
+                         "form as one graph unless --no-plain)")
     ap.add_argument("--tune", action="store_true",
                     help="choose the schedule's sub-level cap and the checkpoint window with the "
                          "autotuner (always on for c5 unless --sublevels is given)")
@@ -220,13 +223,21 @@ def main():
     obs = (0.9 * Q).clone()
     scale = 2.0 / (T * K)
 
-    def step(form):
+    def step(form, stream=None):
+        st = sptr if stream is None else stream
         NAT.check(NAT.lib.ddrs_mc_window_fwd_gauged(ws.handle, cp, T, ptr(qprime), ptr(q0),
-                                                    ptr(Q), form, sptr))
+                                                    ptr(Q), form, st))
         torch.mul(torch.sub(Q, obs), scale, out=gQ)
         NAT.check(NAT.lib.ddrs_mc_window_adj_gauged(ws.handle, cp, T, ptr(qprime), ptr(q0),
                                                     ptr(gQ), ptr(gn), ptr(gqsp), None, None, None,
-                                                    form, sptr))
+                                                    form, st))
+
+    def graph_of(form):
+        """The step captured in a HIP graph (torch.cuda.graph's capture stream)."""
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            step(form, torch.cuda.current_stream().cuda_stream)
+        return g
 
     for _ in range(args.warmup):
         step(NAT.FORM_PERSISTENT)
@@ -269,6 +280,41 @@ def main():
         f"fwd kernel {fwd_ms / args.steps:.2f} ms, adjoint {adj_ms / args.steps:.2f} ms "
         f"(lin {phase_ms[0]:.2f}, sweep {phase_ms[1]:.2f}, fin {phase_ms[2]:.2f}); {layout}")
     gn_pers, gqsp_pers, Q_pers = gn.clone(), gqsp.clone(), Q.clone()
+
+    graph = None
+    if args.graph:
+        # the same step captured once and replayed: its replays must give the
+        # eager step's bits
+        gp = graph_of(NAT.FORM_PERSISTENT)
+        gp.replay()
+        torch.cuda.synchronize()
+        ws.check()
+        torch.cuda.synchronize()
+        tg0 = time.perf_counter()
+        for _ in range(args.steps):
+            gp.replay()
+        torch.cuda.synchronize()
+        tg = (time.perf_counter() - tg0) / args.steps
+        ws.check()
+        same = bool(torch.equal(gn, gn_pers) and torch.equal(gqsp, gqsp_pers)
+                    and torch.equal(Q, Q_pers))
+        graph = dict(value=R * (T - 1) / tg, ms_per_step=1000 * tg,
+                     bit_identical_to_eager=same)
+        del gp
+        log(f"[bench] graph replay: {1000 * tg:.2f} ms/step, identical={same}")
+        if not args.no_plain:
+            gpl = graph_of(NAT.FORM_PLAIN)
+            torch.cuda.synchronize()
+            tq0 = time.perf_counter()
+            gpl.replay()
+            torch.cuda.synchronize()
+            tq = time.perf_counter() - tq0
+            ws.check()
+            same_p = bool(torch.equal(gn, gn_pers) and torch.equal(gqsp, gqsp_pers))
+            graph["plain"] = dict(value=R * (T - 1) / tq, ms_per_step=1000 * tq,
+                                  bit_identical_to_eager=same_p)
+            del gpl
+            log(f"[bench] plain form as one graph: {1000 * tq:.1f} ms/step, identical={same_p}")
 
     plain = None
     if not args.no_plain:
@@ -336,6 +382,7 @@ def main():
         "hbm": hbm,
         "plain": plain,
         "batched_over_plain": (value / plain["value"]) if plain else None,
+        "graph": graph,
         "grad_check": grad_check(torch, D, NW, Network, SparseAdjacency),
         "cpu_baseline": cpu,
     }
