@@ -128,7 +128,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(basins shard across GPUs with no exchange, see DESIGN.md)"}))
+                                   "(basins would shard across GPUs with no exchange during routing)"}))
         sys.exit(2)
 
     import torch
