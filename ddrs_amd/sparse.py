@@ -74,6 +74,7 @@ class Network:
         schedule may hold; block: positions per block, 64 or 128 (0: library
         defaults). Results do not depend on either."""
         self.adj = adj
+        self.max_sub_levels, self.block = int(max_sub_levels), int(block)
         h = ctypes.c_void_p()
         N.check(N.lib.ddrs_plan_create_ex(adj.n, adj.nnz, _ptr(adj.rows), _ptr(adj.cols),
                                           _ptr(adj.values), int(max_sub_levels), int(block),
