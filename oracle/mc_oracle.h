@@ -110,7 +110,13 @@ int orc_pattern_from_coo(int n, int nnz_off, const int* rows, const int* cols,
                             const REAL* slope_raw, const REAL* xst,                   \
                             const REAL* qprime_raw, const REAL* q0_in,                \
                             const REAL* gQ, REAL* gn, REAL* gqsp, REAL* gp,           \
-                            REAL* gqprime, REAL* gq0);
+                            REAL* gqprime, REAL* gq0, REAL* Q_out, REAL* X_out);      \
+    int orc_route_path_##SUF(const orc_params* P, int n, int T, const int* crow,     \
+                             const int* col, const REAL* adj, const REAL* diag_mask,  \
+                             const REAL* nman, const REAL* qsp, const REAL* psp,      \
+                             const REAL* length, const REAL* slope_raw,               \
+                             const REAL* xst, const REAL* qprime_raw, const REAL* Qb, \
+                             const REAL* Xb, int m, const int* path, REAL* Qp);
 
 ORC_DECLARE(f, float)
 ORC_DECLARE(d, double)

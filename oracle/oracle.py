@@ -199,7 +199,9 @@ def route_fwd(P: OrcParams, pat: Pattern, n, qsp, p, length, slope_raw, qprime, 
 
 
 def route_adj(P: OrcParams, pat: Pattern, n, qsp, p, length, slope_raw, qprime, gQ, q0=None,
-              xst=None, dtype=np.float32):
+              xst=None, dtype=np.float32, want_q=False, want_x=False):
+    """Gradients of the loss whose dL/dQ is gQ [T][R] (orc_route_adj); with
+    want_q / want_x also the forward's Q and pre-clamp X [T][R] ("Q", "X")."""
     suf, dt = _dt(dtype)
     R = pat.n
     qprime = _c(qprime, dt)
@@ -208,12 +210,56 @@ def route_adj(P: OrcParams, pat: Pattern, n, qsp, p, length, slope_raw, qprime, 
     ins = [_c(v, dt) for v in (n, qsp, p, length, slope_raw, xst)]
     gn, gqsp, gp, gq0 = (np.zeros(R, dt) for _ in range(4))
     gqp = np.zeros((T, R), dt)
+    Qo = np.zeros((T, R), dt) if want_q else None
+    Xo = np.zeros((T, R), dt) if want_x else None
     rc = getattr(lib(), f"orc_route_adj_{suf}")(
         ctypes.byref(P), R, pat.nnz, T, _p(pat.crow), _p(pat.col), _p(pat.row_for_nnz),
         _p(_c(pat.adj_values, dt)), _p(_c(pat.diag_mask, dt)), _p(pat.trans_crow),
         _p(pat.trans_col), _p(pat.trans_to_orig), *[_p(x) for x in ins], _p(qprime),
         _p(_c(q0, dt) if q0 is not None else None), _p(_c(gQ, dt)), _p(gn), _p(gqsp), _p(gp),
-        _p(gqp), _p(gq0))
+        _p(gqp), _p(gq0), _p(Qo), _p(Xo))
     if rc != 0:
         raise MemoryError("oracle adjoint allocation failed")
-    return dict(n=gn, q_spatial=gqsp, p_spatial=gp, q_prime=gqp, q0=gq0)
+    out = dict(n=gn, q_spatial=gqsp, p_spatial=gp, q_prime=gqp, q0=gq0)
+    if want_q:
+        out["Q"] = Qo
+    if want_x:
+        out["X"] = Xo
+    return out
+
+
+def downstream_path(pat: Pattern, r: int) -> np.ndarray:
+    """Reach r and every reach downstream of it, in flow order (each reach's
+    downstream reach is the row of its single off-diagonal entry in the
+    transposed pattern; a reach with none is an outlet)."""
+    path = [int(r)]
+    while True:
+        i = path[-1]
+        down = [int(j) for j in pat.trans_col[pat.trans_crow[i]:pat.trans_crow[i + 1]] if j != i]
+        if not down:
+            return np.asarray(path, np.int32)
+        if len(down) > 1:
+            raise ValueError(f"reach {i} drains to {len(down)} reaches; not a tree")
+        path.append(down[0])
+
+
+def route_path(P: OrcParams, pat: Pattern, n, qsp, p, length, slope_raw, qprime, Qb, Xb, path,
+               xst=None, dtype=np.float64):
+    """Discharge [T][len(path)] of the reaches of a downstream path after the
+    parameters of path[0] changed (orc_route_path): Qb / Xb are the unchanged
+    run's discharge and pre-clamp solve, every other reach keeps its values."""
+    suf, dt = _dt(dtype)
+    R = pat.n
+    qprime = _c(qprime, dt)
+    T = qprime.shape[0]
+    xst = np.full(R, 0.3) if xst is None else xst
+    ins = [_c(v, dt) for v in (n, qsp, p, length, slope_raw, xst)]
+    path = np.ascontiguousarray(path, np.int32)
+    Qp = np.zeros((T, path.shape[0]), dt)
+    rc = getattr(lib(), f"orc_route_path_{suf}")(
+        ctypes.byref(P), R, T, _p(pat.crow), _p(pat.col), _p(_c(pat.adj_values, dt)),
+        _p(_c(pat.diag_mask, dt)), *[_p(x) for x in ins], _p(qprime), _p(_c(Qb, dt)),
+        _p(_c(Xb, dt)), path.shape[0], _p(path), _p(Qp))
+    if rc != 0:
+        raise ValueError("route_path: the path is not a downstream chain")
+    return Qp
