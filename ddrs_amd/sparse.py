@@ -142,9 +142,13 @@ class Workspace:
         self.fwd_token = None
 
     def close(self):
-        if self.handle:
+        # at interpreter exit the library binding may already be gone
+        if getattr(self, "handle", None) and getattr(N, "lib", None) is not None:
             N.lib.ddrs_workspace_destroy(self.handle)
             self.handle = None
+
+    def __del__(self):
+        self.close()
 
     def nbytes(self) -> int:
         b = ctypes.c_size_t()

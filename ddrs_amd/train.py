@@ -220,9 +220,12 @@ class Trainer:
             self._plan_key = (K, T)
         return self._plan
 
-    def predict(self, batch: GaugedBatch) -> torch.Tensor:
+    def predict(self, batch: GaugedBatch, trace: Optional[dict] = None) -> torch.Tensor:
         """Hourly discharge at the gauges, [T, G], differentiable in the head's
-        weights (training/forward.rs:301-387 with the gauge readout)."""
+        weights (training/forward.rs:301-387 with the gauge readout). trace: a
+        dict that receives the routed physical parameters "n", "q_spatial",
+        the initial state "q0" and the gauge discharge "Qg", each keeping its
+        gradient after a backward."""
         net = self.network(batch)
         dev = batch.q_prime.device
         length, slope, p, x = self._static_inputs(batch, dev)
@@ -234,15 +237,21 @@ class Trainer:
         if "p_spatial" in out:   # learned when the head produces it (forward.rs:319-373)
             p = denormalize(out["p_spatial"], rng.p_spatial, "p_spatial" in log)
         q0 = hotstart(net, P, batch.q_prime[0].detach())
-        return route_window_gauged(net, P, n, qsp, p, batch.q_prime.detach(), q0.detach(),
-                                   length, slope, batch.gauges, x_storage=x, form=self.form,
-                                   window=self.window)
+        Qg = route_window_gauged(net, P, n, qsp, p, batch.q_prime.detach(), q0.detach(),
+                                 length, slope, batch.gauges, x_storage=x, form=self.form,
+                                 window=self.window)
+        if trace is not None:
+            for v in (n, qsp, Qg):
+                if v.requires_grad:
+                    v.retain_grad()
+            trace.update(n=n, q_spatial=qsp, q0=q0, Qg=Qg)
+        return Qg
 
-    def loss(self, batch: GaugedBatch):
+    def loss(self, batch: GaugedBatch, trace: Optional[dict] = None):
         """(loss, n_valid): tau trim and daily pooling, warmup slice, NaN-gauge
         filter and the configured objective (driver.rs:126-201) of
-        predict(batch); n_valid is loss_denominator."""
-        Qg = self.predict(batch)
+        predict(batch); n_valid is loss_denominator. trace as in predict."""
+        Qg = self.predict(batch, trace)
         plan = self._gauge_plan(Qg.shape[1], Qg.shape[0])
         loss, n_valid, _ = gauge_batch_loss(Qg, plan, self.tau, self.warmup, batch.observations,
                                             batch.gauge_obs_std, self.eps, kind=self.loss_kind,

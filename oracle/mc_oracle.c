@@ -49,14 +49,13 @@
 
 /* ---------------- CsrPattern::from_sparse ---------------- */
 
-static const int* g_sort_rows;
-static const int* g_sort_cols;
-
-static int cmp_rc(const void* a, const void* b) {
-    int ka = *(const int*)a, kb = *(const int*)b;
-    if (g_sort_rows[ka] != g_sort_rows[kb]) return g_sort_rows[ka] < g_sort_rows[kb] ? -1 : 1;
-    if (g_sort_cols[ka] != g_sort_cols[kb]) return g_sort_cols[ka] < g_sort_cols[kb] ? -1 : 1;
-    return ka < kb ? -1 : (ka > kb);
+/* Stable counting sort of idx[0..m) by key[idx[.]] in [0, n) into out. */
+static void counting_sort(int m, int n, const int* key, const int* idx, int* out) {
+    int* start = (int*)calloc((size_t)n + 1, sizeof(int));
+    for (int j = 0; j < m; ++j) start[key[idx[j]] + 1] += 1;
+    for (int i = 0; i < n; ++i) start[i + 1] += start[i];
+    for (int j = 0; j < m; ++j) out[start[key[idx[j]]]++] = idx[j];
+    free(start);
 }
 
 /* CsrPattern::from_sparse (sparse/mod.rs:173-227): sort COO by (row, col),
@@ -68,11 +67,16 @@ int orc_pattern_from_coo(int n, int nnz_off, const int* rows, const int* cols,
                          int* trans_col, int* trans_to_orig) {
     for (int k = 0; k < nnz_off; ++k)
         if (!(rows[k] > cols[k]) || rows[k] >= n || cols[k] < 0) return -1;
+    /* entries ordered by (row, col, input index): a stable sort by column,
+     * then a stable sort by row (no shared state, so threads may build
+     * patterns concurrently) */
     int* order = (int*)malloc(sizeof(int) * (size_t)(nnz_off > 0 ? nnz_off : 1));
-    for (int k = 0; k < nnz_off; ++k) order[k] = k;
-    g_sort_rows = rows;
-    g_sort_cols = cols;
-    qsort(order, (size_t)nnz_off, sizeof(int), cmp_rc);
+    int* tmp = (int*)malloc(sizeof(int) * (size_t)(nnz_off > 0 ? nnz_off : 1));
+    for (int k = 0; k < nnz_off; ++k) tmp[k] = k;
+    counting_sort(nnz_off, n, cols, tmp, order);
+    memcpy(tmp, order, sizeof(int) * (size_t)nnz_off);
+    counting_sort(nnz_off, n, rows, tmp, order);
+    free(tmp);
     int nnz = nnz_off + n;
     int cursor = 0, w = 0;
     crow[0] = 0;

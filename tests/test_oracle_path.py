@@ -129,3 +129,22 @@ def test_path_fd_matches_full_fd_and_adjoint():
             assert abs(fd_path - fd_full) <= roundoff, (fd_path, fd_full, roundoff)
             worst = max(worst, abs(g[key][r] - fd_path) / abs(fd_path))
     assert worst < 1e-4, worst
+
+
+def test_pattern_build_is_thread_safe():
+    """The workload-size checks build many basins' patterns in parallel
+    threads: concurrent builds must equal serial ones (the build keeps no
+    shared state)."""
+    from concurrent.futures import ThreadPoolExecutor
+    cases = []
+    for seed in range(16):
+        R = 40000 + 1009 * seed
+        rows, cols = NW.gen_tree(R, 8 + seed, 900 + seed)
+        cases.append((R, rows, cols))
+    serial = [O.pattern_from_coo(R, r, c) for R, r, c in cases]
+    with ThreadPoolExecutor(8) as pool:
+        for _ in range(3):
+            par = list(pool.map(lambda a: O.pattern_from_coo(*a), cases))
+            for a, b in zip(serial, par):
+                for f in ("crow", "col", "row_for_nnz", "adj_values", "trans_col", "trans_to_orig"):
+                    assert np.array_equal(getattr(a, f), getattr(b, f)), f
