@@ -47,6 +47,28 @@
 #undef RExp
 #undef RSq
 
+/* ---------------- float with the device's powers (devmath.cpp) ----------------
+ * The same float arithmetic with the device's correctly rounded fp32 power
+ * and logarithm in place of glibc's powf / logf (everything else, sqrtf and
+ * expf included, unchanged): the _c variant. */
+float orc_devpowf(float x, float y);
+float orc_devlogf(float x);
+#define REAL float
+#define SUF c
+#define RPow orc_devpowf
+#define RSqrt sqrtf
+#define RLog orc_devlogf
+#define RExp expf
+#define RSq(x) ((x) * (x))
+#include "mc_oracle_body.inc"
+#undef REAL
+#undef SUF
+#undef RPow
+#undef RSqrt
+#undef RLog
+#undef RExp
+#undef RSq
+
 /* ---------------- CsrPattern::from_sparse ---------------- */
 
 /* Stable counting sort of idx[0..m) by key[idx[.]] in [0, n) into out. */

@@ -6,10 +6,13 @@
  * bench.py's cpu_baseline leg as the checker. The product (ddrs_amd/, the HIP
  * library) never links, loads or calls anything here.
  *
- * Every routine exists twice: an `_f` variant computing in float (the
- * reference's arithmetic type, op-for-op in the reference's order) and a `_d`
- * variant computing the same algorithm in double (the FP64 oracle used for
- * finite-difference gradient checks and for stating FP64 error).
+ * Every routine exists three times: an `_f` variant computing in float (the
+ * reference's arithmetic type, op-for-op in the reference's order, with
+ * glibc's powf / logf as the reference's libm calls), a `_d` variant computing
+ * the same algorithm in double (the FP64 oracle used for finite-difference
+ * gradient checks and for stating FP64 error), and a `_c` variant: `_f` with
+ * the device's correctly rounded fp32 power and logarithm (devmath.cpp), which
+ * separates the device kernels' arithmetic from libm's rounding.
  *
  * Reference files (taddyb/ddrs):
  *   src/routing/mmc_op.rs   forward_chain_inner 1061-1336, timestep_backward_core 219-848
@@ -120,6 +123,8 @@ int orc_pattern_from_coo(int n, int nnz_off, const int* rows, const int* cols,
 
 ORC_DECLARE(f, float)
 ORC_DECLARE(d, double)
+/* float with the device's powers and logarithms (devmath.cpp) */
+ORC_DECLARE(c, float)
 
 #ifdef __cplusplus
 }

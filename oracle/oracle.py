@@ -107,7 +107,14 @@ def pattern_from_coo(n: int, rows, cols, vals=None) -> Pattern:
     return Pattern(n=n, **out)
 
 
+# dtype of the float variant with the device's correctly rounded powers and
+# logarithms (orc_*_c) in place of glibc's powf / logf
+F32_DEVICE_MATH = "float32-device-math"
+
+
 def _dt(dtype):
+    if isinstance(dtype, str) and dtype == F32_DEVICE_MATH:
+        return "c", np.float32
     return ("f", np.float32) if np.dtype(dtype) == np.float32 else ("d", np.float64)
 
 

@@ -4,8 +4,10 @@ Used by tests/ and by bench.py's check leg, only as the checker: every function
 takes device results already copied to numpy and runs the oracle (oracle.py)
 on the same inputs. Nothing here touches the GPU.
 
-route_pair     the f32 and f64 oracle forward + adjoint of one network in two
-               threads (ctypes releases the GIL), with the same dL/dQ
+route_pair     the oracle's forward + adjoint of one network in float (the
+               reference's arithmetic), double, and float with the device's
+               powers, in three threads (ctypes releases the GIL), with the
+               same dL/dQ
 discharge_errors / grad_errors
                the parity figures: max relative discharge error against the
                f32 oracle (everywhere, and where the f32 oracle is itself
@@ -36,61 +38,77 @@ def threads() -> int:
 
 
 def route_pair(P, pat, n, qsp, p, length, slope, qprime, gQ, q0, want_x64=False, pool=None):
-    """(r32, r64): oracle.route_adj in float and in double with the forward's
-    Q (and, with want_x64, the double pre-clamp X for fd_check)."""
+    """{"f32", "f64", "dev"}: oracle.route_adj in float (glibc's powers, the
+    reference's arithmetic), in double, and in float with the device's
+    correctly rounded powers, in three threads, each with the forward's Q
+    (and, with want_x64, the double pre-clamp X for fd_check)."""
     q64 = np.ascontiguousarray(qprime, np.float64)
     own = pool is None
-    pool = pool or ThreadPoolExecutor(2)
+    pool = pool or ThreadPoolExecutor(3)
     try:
         f32 = pool.submit(O.route_adj, P, pat, n, qsp, p, length, slope, qprime, gQ, q0=q0,
                           want_q=True)
+        dev = pool.submit(O.route_adj, P, pat, n, qsp, p, length, slope, qprime, gQ, q0=q0,
+                          want_q=True, dtype=O.F32_DEVICE_MATH)
         f64 = pool.submit(O.route_adj, P, pat, n, qsp, p, length, slope, q64,
                           np.asarray(gQ, np.float64), q0=np.asarray(q0, np.float64),
                           dtype=np.float64, want_q=True, want_x=want_x64)
-        return f32.result(), f64.result()
+        return {"f32": f32.result(), "f64": f64.result(), "dev": dev.result()}
     finally:
         if own:
             pool.shutdown()
 
 
-def discharge_errors(got, ref32, ref64) -> dict:
-    """Relative discharge errors. "f32-determined" values are those where the
-    f32 oracle itself lies within 1e-4 of FP64; elsewhere the recurrence has
-    amplified f32 rounding past 1e-4 (with Cunge X most reaches have c1 < 0), so
-    any two f32 evaluations that differ in one rounding may differ there by
-    more than 1e-4 while both are as far from FP64."""
-    got, ref32, ref64 = (np.asarray(v, np.float64) for v in (got, ref32, ref64))
+def discharge_errors(got, r: dict, cols=None) -> dict:
+    """Relative discharge errors of the device against the three oracles of
+    route_pair (optionally at columns `cols` of their [T][R] outputs), and the
+    spread of the f32 oracle against the device-math one: the part of the
+    reference's f32 result that libm's rounding of a small fraction of powers
+    decides. Where the recurrence amplifies single roundings (with Cunge X most
+    reaches have c1 < 0), two f32 evaluations that round one power differently
+    part there by more than 1e-4 while both stay as far from FP64."""
+    take = (lambda a: a) if cols is None else (lambda a: a[:, cols])
+    got = np.asarray(got, np.float64)
+    ref32, ref64, dev = (np.asarray(take(r[k]["Q"]), np.float64) for k in ("f32", "f64", "dev"))
     r32 = np.abs(got - ref32) / np.abs(ref32)
-    o64 = np.abs(ref32 - ref64) / np.abs(ref64)
-    det = o64 <= 1e-4
+    rdev = np.abs(got - dev) / np.abs(dev)
+    spread = np.abs(ref32 - dev) / np.abs(dev)
     over = r32 > 1e-4
     return {"max_rel_err_vs_f32_oracle": float(r32.max()),
             "frac_within_1e-4": float(1.0 - over.mean()),
             "count_over_1e-4": int(over.sum()),
-            "max_rel_err_vs_f32_oracle_f32_determined": float(r32[det].max()) if det.any() else 0.0,
-            "count_over_1e-4_f32_determined": int((over & det).sum()),
-            "f32_undetermined_values": int((~det).sum()),
+            "bit_identical_to_device_math_oracle": float((got == dev).mean()),
+            "max_rel_err_vs_device_math_oracle": float(rdev.max()),
+            "f32_oracle_vs_device_math_oracle_max_rel": float(spread.max()),
+            "f32_oracle_vs_device_math_oracle_count_over_1e-4": int((spread > 1e-4).sum()),
             "max_rel_err_vs_fp64": float((np.abs(got - ref64) / np.abs(ref64)).max()),
-            "f32_oracle_max_rel_err_vs_fp64": float(o64.max()),
+            "f32_oracle_max_rel_err_vs_fp64": float((np.abs(ref32 - ref64) / np.abs(ref64)).max()),
             "values": int(got.size)}
 
 
-def grad_errors(got, ref32, ref64, well=1e-2) -> dict:
-    got, ref32, ref64 = (np.asarray(v, np.float64) for v in (got, ref32, ref64))
+def grad_errors(got, r: dict, key: str, idx=None, well=1e-2) -> dict:
+    """Gradient figures of the device's `got` against route_pair's r[.][key]
+    (optionally at entries idx)."""
+    take = (lambda a: a) if idx is None else (lambda a: a[idx])
+    got = np.asarray(got, np.float64)
+    ref32, ref64, dev = (np.asarray(take(r[k][key]), np.float64) for k in ("f32", "f64", "dev"))
     s32 = max(float(np.abs(ref32).max()), 1e-300)
     s64 = max(float(np.abs(ref64).max()), 1e-300)
     wc = np.abs(ref64) >= well * s64
-    rel32 = np.abs(got - ref32)[wc] / np.abs(ref32)[wc]
-    rel64 = np.abs(got - ref64)[wc] / np.abs(ref64)[wc]
-    orc64 = np.abs(ref32 - ref64)[wc] / np.abs(ref64)[wc]
+    rel = lambda a, b: np.abs(a - b)[wc] / np.abs(b)[wc]
+    rel32, rel64, orc64 = rel(got, ref32), rel(got, ref64), rel(ref32, ref64)
+    m = lambda a: float(a.max()) if a.size else 0.0
     return {"norm_err_vs_f32_oracle": float(np.abs(got - ref32).max()) / s32,
+            "norm_err_vs_device_math_oracle": float(np.abs(got - dev).max()) / s32,
+            "f32_oracle_vs_device_math_oracle_norm": float(np.abs(ref32 - dev).max()) / s32,
             "norm_err_vs_fp64": float(np.abs(got - ref64).max()) / s64,
             "f32_oracle_norm_err_vs_fp64": float(np.abs(ref32 - ref64).max()) / s64,
             "well_conditioned_entries": int(wc.sum()), "entries": int(got.size),
-            "elem_max_rel_err_vs_f32_oracle": float(rel32.max()) if rel32.size else 0.0,
+            "elem_max_rel_err_vs_f32_oracle": m(rel32),
             "elem_frac_within_1e-3_vs_f32_oracle": float((rel32 <= 1e-3).mean()) if rel32.size else 1.0,
-            "elem_max_rel_err_vs_fp64": float(rel64.max()) if rel64.size else 0.0,
-            "f32_oracle_elem_max_rel_err_vs_fp64": float(orc64.max()) if orc64.size else 0.0}
+            "elem_max_rel_err_vs_device_math_oracle": m(rel(got, dev)),
+            "elem_max_rel_err_vs_fp64": m(rel64),
+            "f32_oracle_elem_max_rel_err_vs_fp64": m(orc64)}
 
 
 def fd_check(P, pat, params, length, slope, qprime, Q64, X64, w, gauges, reaches,
@@ -155,8 +173,7 @@ def fd_check(P, pat, params, length, slope, qprime, Q64, X64, w, gauges, reaches
             pool.shutdown()
 
 
-def fd_errors(fd: dict, reaches, got: dict, ref32: dict, ref64: dict, floor=1e-6,
-              agree=1e-3) -> dict:
+def fd_errors(fd: dict, reaches, got: dict, r: dict, floor=1e-6, agree=1e-3) -> dict:
     """Errors against the finite differences, per gradient. A reach counts as
     resolved when its two chosen steps agree within `agree` and its difference
     is at least floor x the largest over the reaches (below that the gradient
@@ -170,7 +187,7 @@ def fd_errors(fd: dict, reaches, got: dict, ref32: dict, ref64: dict, floor=1e-6
         res = (np.abs(f) >= floor * top) & (d["consistency"] <= agree)
         den = np.where(res, np.abs(f), top)
         e = lambda g: np.abs(np.asarray(g, np.float64)[reaches] - f) / den
-        eg, e32, e64 = e(got[key]), e(ref32[key]), e(ref64[key])
+        eg, e32, e64 = e(got[key]), e(r["f32"][key]), e(r["f64"][key])
         pick = lambda a: float(a[res].max()) if res.any() else 0.0
         out[key] = {"reaches": int(len(f)), "resolved": int(res.sum()),
                     "vanishing": int((np.abs(f) < floor * top).sum()),

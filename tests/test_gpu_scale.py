@@ -1,6 +1,6 @@
 """Parity at the measured workloads' full sizes: the HIP path through the C ABI
-against the f32 and f64 oracle on the benchmark's own networks, forcing and
-gauges (ddrs_amd.workloads). Needs an MI355X.
+against the oracle on the benchmark's own networks, forcing and gauges
+(ddrs_amd.workloads). Needs an MI355X.
 
 c2  the whole 350,000-reach x 720-step window: discharge at every reach-step
     (a dense forward) and at the 64 gauges, the gradients w.r.t. n and
@@ -20,15 +20,26 @@ c5  the training micro-batch: the routing of a sample of its trees (discharge
     dL/dQ at the gauges), the loss stage on the device's gauge discharge, and
     the KAN backward of the device's per-reach gradients over all reaches.
 
-Bars. Discharge within 1e-4 relative of the f32 oracle at every compared
-reach-step where the f32 oracle is itself within 1e-4 of FP64, and at 99.999%
-of all of them, with an FP64 error at most 3x the f32 oracle's own. At full
-size the recurrence amplifies f32 rounding past 1e-4 at some reach-steps (on
-c2 the f32 oracle is up to 12% from FP64 on a few): there the device's powers
-(correctly rounded) and glibc's powf (not, on ~0.06% of inputs) may round
-once differently and the two f32 results part by more than 1e-4 while both
-stay as far from FP64. Gradients: vectors within 1e-3 max|g| of the f32
-oracle, with an FP64 error at most 3x the oracle's own. Beside the norm bar, the element-wise
+Three oracles take part (oracle/scale_check.py): the f32 restatement of the
+reference (glibc's powf and logf, as the reference's libm calls), the same
+float arithmetic with the device's correctly rounded powers and logarithms
+("device-math"), and FP64. glibc's powf rounds about 0.06% of the path's powers
+away from the correctly rounded result, and at full size the recurrence
+amplifies single roundings past 1e-4 at a few reach-steps (with Cunge X most
+reaches have c1 < 0; on c2 the f32 oracle is up to 12% from FP64 at some).
+There two f32 evaluations that round one power differently part by more than
+1e-4 while both stay as far from FP64. The device-math oracle separates that
+from the kernels' own arithmetic.
+
+Bars. Discharge: within 1e-5 relative of the device-math oracle at every
+compared reach-step (bit-identical is expected: the fraction is reported);
+within 1e-4 of the f32 oracle at 99.999% of them, and nowhere farther from it
+than twice the spread between the f32 and device-math oracles; an FP64 error
+at most 3x the f32 oracle's own. Gradients: vectors within 1e-4 max|g| of the
+device-math oracle (the adjoint's gradient-only divisions use the hardware
+reciprocal) and within 1e-3 max|g| of the f32 oracle, or twice the spread
+between the two oracles where libm's rounding moves that one further, with an
+FP64 error at most 3x the f32 oracle's own. Beside the norm bar, the element-wise
 relative error on the well-conditioned entries (|g| >= 1e-2 max|g|) is
 reported. The finite differences (oracle/scale_check.py: the step chosen per
 reach from five, by the agreement of adjacent steps): on every reach they
@@ -93,16 +104,18 @@ def report(name, figures):
 
 
 def assert_discharge(label, e):
-    """Within 1e-4 of the f32 oracle wherever the f32 oracle is within 1e-4 of
-    FP64, at least 99.999% of all values within 1e-4 of it, and an FP64 error
-    at most 3x the f32 oracle's own."""
-    assert e["max_rel_err_vs_f32_oracle_f32_determined"] <= 1e-4, (label, e)
+    """The bars of the module docstring on discharge_errors' figures."""
+    assert e["max_rel_err_vs_device_math_oracle"] <= 1e-5, (label, e)
     assert e["frac_within_1e-4"] >= 0.99999, (label, e)
+    assert e["max_rel_err_vs_f32_oracle"] <= (
+        2.0 * e["f32_oracle_vs_device_math_oracle_max_rel"] + 1e-5), (label, e)
     assert e["max_rel_err_vs_fp64"] <= 3.0 * e["f32_oracle_max_rel_err_vs_fp64"] + 1e-6, (label, e)
 
 
 def assert_grad(label, e):
-    assert e["norm_err_vs_f32_oracle"] <= 1e-3, (label, e)
+    assert e["norm_err_vs_f32_oracle"] <= max(
+        1e-3, 2.0 * e["f32_oracle_vs_device_math_oracle_norm"] + 1e-4), (label, e)
+    assert e["norm_err_vs_device_math_oracle"] <= 1e-4, (label, e)
     assert e["norm_err_vs_fp64"] <= 3.0 * e["f32_oracle_norm_err_vs_fp64"] + 1e-6, (label, e)
 
 
@@ -150,22 +163,21 @@ def test_c2_full_size_parity():
     gQd = np.zeros((T, R), np.float32)
     gQd[:, wl.gauges] = gQ
     with ThreadPoolExecutor(SC.threads()) as pool:
-        r32, r64 = SC.route_pair(P, pat, at["n"], at["q_spatial"], p, at["length"], at["slope"],
-                                 qp, gQd, q0, want_x64=True, pool=pool)
+        r = SC.route_pair(P, pat, at["n"], at["q_spatial"], p, at["length"], at["slope"], qp,
+                          gQd, q0, want_x64=True, pool=pool)
         say(f"oracle f32 + f64 done ({time.time() - t0:.0f}s)")
         figs = {"forcing_host_vs_device": gen,
-                "discharge_dense": SC.discharge_errors(Qdense, r32["Q"], r64["Q"]),
-                "discharge_gauges": SC.discharge_errors(Qg, r32["Q"][:, wl.gauges],
-                                                        r64["Q"][:, wl.gauges])}
+                "discharge_dense": SC.discharge_errors(Qdense, r),
+                "discharge_gauges": SC.discharge_errors(Qg, r, wl.gauges)}
         del Qdense
         for k in SC.GRAD_KEYS:
-            figs[f"grad_{k}"] = SC.grad_errors(got[k], r32[k], r64[k])
+            figs[f"grad_{k}"] = SC.grad_errors(got[k], r, k)
         rng = np.random.default_rng(4102)
         reaches = np.sort(rng.choice(R, 64, replace=False))
         params = {"n": at["n"], "q_spatial": at["q_spatial"], "p_spatial": p}
-        fd = SC.fd_check(P, pat, params, at["length"], at["slope"], qp, r64["Q"], r64["X"],
+        fd = SC.fd_check(P, pat, params, at["length"], at["slope"], qp, r["f64"]["Q"], r["f64"]["X"],
                          gQ, wl.gauges, reaches, pool=pool)
-    figs["fd_64_reaches"] = SC.fd_errors(fd, reaches, got, r32, r64)
+    figs["fd_64_reaches"] = SC.fd_errors(fd, reaches, got, r)
     figs["seconds"] = time.time() - t0
     report("c2", figs)
     assert_discharge("c2 dense", figs["discharge_dense"])
@@ -192,13 +204,11 @@ def _basin_parity(wl, b, lo, s, T_b, Qg, gQ, gn, gqsp, q0, qp):
     p = np.full(size, 21.0, np.float32)
     q0b = q0[lo:lo + size]
     np.testing.assert_array_equal(q0b, O.hotstart(P, pat, qp[0]), err_msg=f"basin {b} q0")
-    r32, r64 = SC.route_pair(P, pat, at["n"], at["q_spatial"], p, at["length"], at["slope"],
-                             qp, gQ, q0b)
-    figs = {"reaches": size, "steps": T_b,
-            "discharge": SC.discharge_errors(Qg, r32["Q"], r64["Q"])}
+    r = SC.route_pair(P, pat, at["n"], at["q_spatial"], p, at["length"], at["slope"], qp, gQ, q0b)
+    figs = {"reaches": size, "steps": T_b, "discharge": SC.discharge_errors(Qg, r)}
     got = {"n": gn[lo:lo + size], "q_spatial": gqsp[lo:lo + size]}
     for k in SC.GRAD_KEYS:
-        figs[f"grad_{k}"] = SC.grad_errors(got[k], r32[k], r64[k])
+        figs[f"grad_{k}"] = SC.grad_errors(got[k], r, k)
     return figs
 
 
@@ -313,11 +323,11 @@ def _tree_parity(wl, reach_ids, T, Qg, gQ, gn, gqsp, nq, q0, qp):
     p = np.full(R, 21.0, np.float32)
     q0t = q0[reach_ids]
     np.testing.assert_array_equal(q0t, O.hotstart(P, pat, qp[0]))
-    r32, r64 = SC.route_pair(P, pat, n, qsp, p, at["length"], at["slope"], qp, gQ, q0t)
-    figs = {"reaches": int(R), "steps": T, "discharge": SC.discharge_errors(Qg, r32["Q"], r64["Q"])}
+    r = SC.route_pair(P, pat, n, qsp, p, at["length"], at["slope"], qp, gQ, q0t)
+    figs = {"reaches": int(R), "steps": T, "discharge": SC.discharge_errors(Qg, r)}
     got = {"n": gn[reach_ids], "q_spatial": gqsp[reach_ids]}
     for k in SC.GRAD_KEYS:
-        figs[f"grad_{k}"] = SC.grad_errors(got[k], r32[k], r64[k])
+        figs[f"grad_{k}"] = SC.grad_errors(got[k], r, k)
     return figs
 
 
