@@ -41,9 +41,11 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from ddrs_amd import workloads as W  # noqa: E402
+
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-# KAN head of the batched and training workloads, evaluated per reach
+# KAN head of the batched and training workloads, evaluated over 350,000 reaches
 KAN_SHAPE = dict(F=10, H=21, P=2, L=2, grid=5, k=3, N=350_000)
 # gauge readout + loss of the training workload
 LOSS_SHAPE = dict(R=350_000, T=8760, G=2000, tau=9, warmup=30)
@@ -72,29 +74,6 @@ ALGO_BYTES = {"fwd": 40, "adjoint": 72}
 KERNEL_BYTES = {"fwd": 28, "lin": 52, "sweep": 52, "fin": 20}
 KERNEL_NAMES = {"fwd": "k_fwd_window (persistent forward)", "lin": "k_adj_lin (linearise)",
                 "sweep": "k_adj_sweep (persistent reverse sweep)", "fin": "k_adj_fin"}
-CONFIGS = {
-    "c1": dict(R=5000, T=240, window=256, storms=False),
-    "c2": dict(R=350000, T=720, window=256, storms=True),
-    # 512 basins of log-uniform size in [10, 50000] side by side, 2160 hourly
-    # steps, the adjoint over checkpoint segments of 256 steps, the loss at
-    # every basin outlet
-    "c3": dict(basins=512, size_lo=10, size_hi=50000, T=2160, window=256, ckpt=256,
-               storms=True),
-    # 200,000 basins, Zipf(1.8) sizes capped at 300,000 scaled to 2.94 M reaches,
-    # 8760 hourly steps, checkpoints every 168 steps, the loss at the outlets of
-    # the 2000 largest basins
-    "c4": dict(basins=200_000, zipf=1.8, size_hi=300_000, total=2_940_000, T=8760,
-               window=256, ckpt=168, storms=True, gauges=2000),
-    # training micro-batch: 2000 nested gauges (basins log-uniform 10-50000) on
-    # a 350,000-reach tree, the union of their basins compressed to the batch,
-    # 365 days hourly; KAN head -> routing -> daily NSE -> adjoint -> KAN
-    # backward -> AdaDelta; the autotuner picks the schedule and the checkpoint
-    # window (one segment when it fits)
-    "c5": dict(R=350_000, window=256, gauges=2000, size_lo=10, size_hi=50_000, T=8760,
-               ckpt=None, storms=True, tau=9, warmup=5, eps=0.1, noise=0.1),
-}
-
-
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -104,9 +83,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-parity", action="store_true",
+                    help="skip the oracle check of the timed step's results (c1, c2: the whole "
+                         "network, about 90 s of CPU work at c2, and FP64 differences on 64 "
+                         "reaches)")
     ap.add_argument("--cpu-steps", type=int, default=240,
                     help="timesteps of the single-thread oracle sample (about 12 s on C2)")
     ap.add_argument("--sublevels", type=int, default=0,
@@ -132,10 +115,7 @@ def main():
         sys.exit(2)
 
     import torch
-    import ddrs_amd as D
     from ddrs_amd import _native as NAT
-    from ddrs_amd import network as NW
-    from ddrs_amd.sparse import Network, SparseAdjacency
 
     if args.kan_only:
         kan = kan_bench(torch, args.steps, args.warmup, not args.no_cpu)
@@ -157,80 +137,26 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    cfg = CONFIGS[args.config]
-    T = cfg["T"]
-    ci = {"c1": 1, "c2": 2, "c3": 3, "c4": 4}[args.config]
+    wl = W.build(args.config)
+    R, T = wl.R, wl.T
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    outlets = None
-    if "basins" in cfg:
-        rs = np.random.default_rng(7000 + ci)
-        if "zipf" in cfg:
-            z = np.minimum(rs.zipf(cfg["zipf"], cfg["basins"]), cfg["size_hi"]).astype(np.float64)
-            sizes = np.maximum(1, np.floor(z * cfg["total"] / z.sum())).astype(np.int64)
-            sizes[np.argmax(sizes)] += cfg["total"] - int(sizes.sum())
-        else:
-            sizes = np.exp(rs.uniform(np.log(cfg["size_lo"]), np.log(cfg["size_hi"]),
-                                      cfg["basins"])).astype(np.int64)
-        rows, cols, outlets = NW.gen_forest(sizes, cfg["window"], 1000 + ci)
-        if cfg.get("gauges"):
-            outlets = outlets[np.argsort(sizes)[::-1][: cfg["gauges"]]]
-        R = int(sizes.sum())
-    else:
-        R = cfg["R"]
-        rows, cols = NW.gen_tree(R, cfg["window"], 1000 + ci)
     log(f"[bench] building network R={R} T={T}")
-    at = NW.gen_attributes(R, 2000 + ci)
-    adj = SparseAdjacency.from_coo(R, rows, cols, None, at["length"], at["slope"])
     t0 = time.time()
-    f = lambda x: torch.as_tensor(np.ascontiguousarray(x, np.float32), device=dev)
-    n, qsp = f(at["n"]), f(at["q_spatial"])
-    p = torch.full((R,), 21.0, dtype=torch.float32, device=dev)
-    length, slope = f(at["length"]), f(at["slope"])
-    qprime = NW.gen_forcing_device(R, T, 3000 + ci, cfg["storms"], device=dev)
-    rng = np.random.default_rng(4000 + ci)
-    if outlets is not None:
-        gauges_np = np.sort(outlets)                      # every basin outlet
-    else:
-        gauges_np = np.sort(np.concatenate([[R - 1], rng.choice(R - 1, 63, replace=False)]))
-    net, window, schedule = schedule_for(args, adj, T, qprime, gauges_np, cfg.get("ckpt"),
+    qprime = wl.forcing_device(dev)
+    net, window, schedule = schedule_for(args, wl.adjacency(), T, qprime, wl.gauges, wl.ckpt,
                                          tune=args.tune)
     log(f"[bench] plan {time.time() - t0:.2f}s: {net.n_chunks} chunks, {net.n_levels} levels; "
         f"schedule {schedule}")
-    params = D.Params()
-    cp = params.to_c()
-    # the step reads out gauges only: no dense dL/dQ buffer
-    ws = D.Workspace(net, window or (T - 1), T - 1, dense_grad=False)
+    gw = W.GaugedWindow(net, wl, qprime, window=window)
+    ws = gw.ws
     log(f"[bench] workspace {ws.nbytes() / 2**30:.2f} GiB")
-    stream = torch.cuda.current_stream()
-    sptr = stream.cuda_stream
-    ptr = lambda x: None if x is None else x.data_ptr()
-    NAT.check(NAT.lib.ddrs_mc_bind_statics(ws.handle, cp, ptr(n), ptr(qsp), ptr(p), ptr(length),
-                                           ptr(slope), None, sptr))
-    q0 = torch.empty(R, dtype=torch.float32, device=dev)
-    hot = net.workspace()
-    NAT.check(NAT.lib.ddrs_hotstart(hot.handle, cp, ptr(qprime[0]), ptr(q0), sptr))
-    hot.check()
-    gn, gqsp = torch.empty(R, device=dev), torch.empty(R, device=dev)
-    ws.set_gauges(gauges_np)
-    K = gauges_np.size
-    Q = torch.empty((T, K), dtype=torch.float32, device=dev)   # discharge at the gauges
-    gQ = torch.empty((T, K), dtype=torch.float32, device=dev)
     # fixed observation target: the routed discharge at the gauges scaled by 0.9
-    NAT.check(NAT.lib.ddrs_mc_window_fwd_gauged(ws.handle, cp, T, ptr(qprime), ptr(q0), ptr(Q),
-                                                NAT.FORM_PERSISTENT, sptr))
-    ws.check()
-    obs = (0.9 * Q).clone()
-    scale = 2.0 / (T * K)
+    obs = (0.9 * gw.forward()).clone()
+    gw.check()
 
     def step(form, stream=None):
-        st = sptr if stream is None else stream
-        NAT.check(NAT.lib.ddrs_mc_window_fwd_gauged(ws.handle, cp, T, ptr(qprime), ptr(q0),
-                                                    ptr(Q), form, st))
-        torch.mul(torch.sub(Q, obs), scale, out=gQ)
-        NAT.check(NAT.lib.ddrs_mc_window_adj_gauged(ws.handle, cp, T, ptr(qprime), ptr(q0),
-                                                    ptr(gQ), ptr(gn), ptr(gqsp), None, None, None,
-                                                    form, st))
+        gw.step(obs, form, stream)
 
     def graph_of(form):
         """The step captured in a HIP graph (torch.cuda.graph's capture stream)."""
@@ -241,45 +167,44 @@ def main():
 
     for _ in range(args.warmup):
         step(NAT.FORM_PERSISTENT)
-    ws.check()
-    NAT.check(NAT.lib.ddrs_set_timing(ws.handle, 1))
-    fwd_ms = adj_ms = 0.0
-    nf = na = 0
-    phase_ms = np.zeros(3)
+    gw.check()
+    # HIP events around every kernel launch on the launch stream, summed over
+    # the timed steps and read once after them (no host synchronisation inside)
+    NAT.check(NAT.lib.ddrs_set_timing(ws.handle, 2))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
         step(NAT.FORM_PERSISTENT)
-        a = [np.zeros(1, np.int32), np.zeros(1, np.float32), np.zeros(1, np.int32),
-             np.zeros(1, np.float32)]
-        NAT.check(NAT.lib.ddrs_get_timing(ws.handle, *[x.ctypes.data for x in a]))
-        nf += int(a[0][0]); fwd_ms += float(a[1][0]); na += int(a[2][0]); adj_ms += float(a[3][0])
-        ph = [np.zeros(1, np.float32) for _ in range(3)]
-        NAT.check(NAT.lib.ddrs_get_adj_phase_timing(ws.handle, *[x.ctypes.data for x in ph]))
-        phase_ms += [float(x[0]) for x in ph]
-        log(f"[bench] step {k + 1}/{args.steps} done")
+        log(f"[bench] step {k + 1}/{args.steps} enqueued")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    tm = [np.zeros(1, np.int32), np.zeros(1, np.float32), np.zeros(1, np.int32),
+          np.zeros(1, np.float32)]
+    NAT.check(NAT.lib.ddrs_get_timing(ws.handle, *[x.ctypes.data for x in tm]))
+    ph = [np.zeros(1, np.float32) for _ in range(3)]
+    NAT.check(NAT.lib.ddrs_get_adj_phase_timing(ws.handle, *[x.ctypes.data for x in ph]))
+    fwd_ms = float(tm[1][0]) / args.steps
+    phase_ms = np.array([float(x[0]) for x in ph]) / args.steps
+    NAT.check(NAT.lib.ddrs_set_timing(ws.handle, 0))
     free_b, total_b = torch.cuda.mem_get_info()
     hbm = {"device_used_GiB": (total_b - free_b) / 2**30, "device_total_GiB": total_b / 2**30,
            "workspace_GiB": ws.nbytes() / 2**30,
            "note": "device memory in use after the timed steps (workspace, q', outputs, "
                    "runtime)"}
     log(f"[bench] HBM in use {hbm['device_used_GiB']:.1f} of {hbm['device_total_GiB']:.1f} GiB")
-    ws.check()
-    NAT.check(NAT.lib.ddrs_set_timing(ws.handle, 0))
+    gw.check()
     rts = R * (T - 1)
     value = rts * args.steps / elapsed
     ms_per_step = 1000.0 * elapsed / args.steps
-    lay = [np.zeros(1, np.int32) for _ in range(4)]
-    NAT.check(NAT.lib.ddrs_workspace_layout(ws.handle, *[x.ctypes.data for x in lay]))
-    layout = dict(fwd_reaches_per_lane=int(lay[0][0]), fwd_workgroups=int(lay[1][0]),
-                  sweep_reaches_per_lane=int(lay[2][0]), sweep_workgroups=int(lay[3][0]))
-    phase_ms /= args.steps
+    layout = ws.layout()
     log(f"[bench] persistent: {ms_per_step:.2f} ms/step, {value / 1e9:.3f} G reach-ts/s; "
-        f"fwd kernel {fwd_ms / args.steps:.2f} ms, adjoint {adj_ms / args.steps:.2f} ms "
+        f"fwd kernel {fwd_ms:.2f} ms, adjoint {float(tm[3][0]) / args.steps:.2f} ms "
         f"(lin {phase_ms[0]:.2f}, sweep {phase_ms[1]:.2f}, fin {phase_ms[2]:.2f}); {layout}")
-    gn_pers, gqsp_pers, Q_pers = gn.clone(), gqsp.clone(), Q.clone()
+    gn_pers, gqsp_pers, Q_pers = gw.gn.clone(), gw.gqsp.clone(), gw.Q.clone()
+
+    def same_as():
+        return bool(torch.equal(gw.gn, gn_pers) and torch.equal(gw.gqsp, gqsp_pers)
+                    and torch.equal(gw.Q, Q_pers))
 
     graph = None
     if args.graph:
@@ -288,20 +213,17 @@ def main():
         gp = graph_of(NAT.FORM_PERSISTENT)
         gp.replay()
         torch.cuda.synchronize()
-        ws.check()
-        torch.cuda.synchronize()
+        gw.check()
         tg0 = time.perf_counter()
         for _ in range(args.steps):
             gp.replay()
         torch.cuda.synchronize()
         tg = (time.perf_counter() - tg0) / args.steps
-        ws.check()
-        same = bool(torch.equal(gn, gn_pers) and torch.equal(gqsp, gqsp_pers)
-                    and torch.equal(Q, Q_pers))
-        graph = dict(value=R * (T - 1) / tg, ms_per_step=1000 * tg,
-                     bit_identical_to_eager=same)
+        gw.check()
+        graph = dict(value=rts / tg, ms_per_step=1000 * tg, bit_identical_to_eager=same_as())
         del gp
-        log(f"[bench] graph replay: {1000 * tg:.2f} ms/step, identical={same}")
+        log(f"[bench] graph replay: {1000 * tg:.2f} ms/step, "
+            f"identical={graph['bit_identical_to_eager']}")
         if not args.no_plain:
             gpl = graph_of(NAT.FORM_PLAIN)
             torch.cuda.synchronize()
@@ -309,12 +231,11 @@ def main():
             gpl.replay()
             torch.cuda.synchronize()
             tq = time.perf_counter() - tq0
-            ws.check()
-            same_p = bool(torch.equal(gn, gn_pers) and torch.equal(gqsp, gqsp_pers))
-            graph["plain"] = dict(value=R * (T - 1) / tq, ms_per_step=1000 * tq,
-                                  bit_identical_to_eager=same_p)
+            gw.check()
+            graph["plain"] = dict(value=rts / tq, ms_per_step=1000 * tq,
+                                  bit_identical_to_eager=same_as())
             del gpl
-            log(f"[bench] plain form as one graph: {1000 * tq:.1f} ms/step, identical={same_p}")
+            log(f"[bench] plain form as one graph: {1000 * tq:.1f} ms/step")
 
     plain = None
     if not args.no_plain:
@@ -324,32 +245,47 @@ def main():
         step(NAT.FORM_PLAIN)
         torch.cuda.synchronize()
         tp = time.perf_counter() - t1
-        ws.check()
+        gw.check()
         a = [np.zeros(1, np.int32), np.zeros(1, np.float32), np.zeros(1, np.int32),
              np.zeros(1, np.float32)]
         NAT.check(NAT.lib.ddrs_get_timing(ws.handle, *[x.ctypes.data for x in a]))
         NAT.check(NAT.lib.ddrs_set_timing(ws.handle, 0))
-        same = bool(torch.equal(gn, gn_pers) and torch.equal(gqsp, gqsp_pers) and torch.equal(Q, Q_pers))
         plain = dict(value=rts / tp, ms_per_step=1000 * tp, fwd_launches=int(a[0][0]),
                      fwd_kernel_ms=float(a[1][0]), adj_launches=int(a[2][0]),
-                     adj_kernel_ms=float(a[3][0]), bit_identical_to_persistent=same)
-        log(f"[bench] plain: {1000 * tp:.1f} ms/step ({value * tp / rts:.1f}x slower), identical={same}")
+                     adj_kernel_ms=float(a[3][0]), bit_identical_to_persistent=same_as())
+        log(f"[bench] plain: {1000 * tp:.1f} ms/step ({value * tp / rts:.1f}x slower), "
+            f"identical={plain['bit_identical_to_persistent']}")
 
     # per-launch times of the persistent form (one launch of each kernel per step)
-    launch_ms = {"fwd": fwd_ms / args.steps, "lin": phase_ms[0], "sweep": phase_ms[1],
-                 "fin": phase_ms[2]}
-    rs = R * (T - 1)  # reach-steps per launch
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_fwd_window_{args.config}.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
+    launch_ms = {"fwd": fwd_ms, "lin": phase_ms[0], "sweep": phase_ms[1], "fin": phase_ms[2]}
+    traffic, traffic_note = stored_traffic(args.config)
+
+    # the parity leg: the timed step's own results against the oracle
+    results = {"Q": gw.Q.cpu().numpy(), "gQ": gw.gQ.cpu().numpy(), "q0": gw.q0.cpu().numpy(),
+               "n": gw.gn.cpu().numpy(), "q_spatial": gw.gqsp.cpu().numpy()}
+    dense = not args.no_parity and wl.sizes is None
+    Qdense = gw.dense_forward().cpu().numpy() if dense else None
+    qp_host = qprime.cpu().numpy() if dense else None
+    gw.check()
+    gw.close()
+    del gw, ws, qprime, obs, gn_pers, gqsp_pers, Q_pers
+    torch.cuda.empty_cache()
+    copy = copy_peak(torch)
+    log(f"[bench] device-to-device copy: {copy['GBs']:.0f} GB/s")
 
     cpu = None
     if not args.no_cpu:
         # bounded sample: about 80 M reach-steps of single-thread oracle work
         cpu_steps = args.cpu_steps if R * args.cpu_steps <= 1e8 else max(1, int(8e7 // R))
-        cpu = cpu_baseline(rows, cols, at, R, cpu_steps, ci, cfg["storms"], net, params)
+        cpu = cpu_baseline(wl, cpu_steps)
+
+    parity = grad_check = None
+    if dense:
+        parity, grad_check = parity_leg(wl, qp_host, Qdense, results)
+        del Qdense, qp_host
+    elif not args.no_parity:
+        parity = {"note": "forests are checked basin by basin at full size in "
+                          "tests/test_gpu_scale.py"}
 
     out = {
         "metric": "reach-timesteps/s fwd+adjoint (batched vs plain, 1 GPU); grad rel err vs FD",
@@ -364,76 +300,135 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded generator; no network access)",
-        "config": {"workload": (f"{args.config}: {cfg['basins']} basins, {R} reaches (sizes "
-                                + (f"Zipf({cfg['zipf']}) capped at {cfg['size_hi']} and scaled to "
-                                   f"{cfg['total']}" if "zipf" in cfg else
-                                   f"log-uniform {cfg['size_lo']}-{cfg['size_hi']}")
-                                + f", generator window {cfg['window']}), {T} hourly steps, "
-                                f"checkpoint window {window or (T - 1)}, forward + adjoint, MSE at "
-                                f"{len(gauges_np)} basin outlets") if outlets is not None else
-                               (f"{args.config}: {R}-reach tree (window {cfg['window']}), "
-                                f"{T} hourly steps, forward + adjoint, MSE at 64 gauges"),
-                   "reaches": R, "timesteps": T, "levels": net.n_levels, "schedule": schedule,
-                   "form": "persistent",
-                   "parallelism": "single GPU"},
-        "roofline": roofline_of(launch_ms, rs, traffic, layout=layout,
-                                note="latency- and VALU-bound persistent pipeline; traffic is "
-                                     "the forward's measured HBM bytes per launch"),
+        "config": {"workload": workload_name(args.config, wl, window), "reaches": R,
+                   "timesteps": T, "levels": net.n_levels, "schedule": schedule,
+                   "form": "persistent", "parallelism": "single GPU"},
+        "roofline": roofline_of(launch_ms, rts, traffic, layout=layout, copy_peak=copy,
+                                traffic_note=traffic_note,
+                                note="latency- and VALU-bound persistent pipeline"),
         "hbm": hbm,
         "plain": plain,
         "batched_over_plain": (value / plain["value"]) if plain else None,
         "graph": graph,
-        "grad_check": grad_check(torch, D, NW, Network, SparseAdjacency),
+        "grad_check": grad_check,
+        "parity": parity,
         "cpu_baseline": cpu,
     }
     if not args.no_kan:
         out["kan_head"] = kan_bench(torch, args.steps, args.warmup, not args.no_cpu)
     if not args.no_loss:
-        del Q, gQ
-        torch.cuda.empty_cache()
         out["gauge_loss"] = loss_bench(torch, args.steps, args.warmup, not args.no_cpu)
     print(json.dumps(out), flush=True)
 
 
-def grad_check(torch, D, NW, Network, SparseAdjacency):
-    """GPU gradients against FP64 central differences of the double oracle on a
-    small network of the same generator (2,000 reaches, 24 steps, 8 reaches)."""
+def workload_name(config, wl, window):
+    cfg = W.CONFIGS[config]
+    T = wl.T
+    if wl.sizes is not None:
+        sizes = (f"Zipf({cfg['zipf']}) capped at {cfg['size_hi']} and scaled to {cfg['total']}"
+                 if "zipf" in cfg else f"log-uniform {cfg['size_lo']}-{cfg['size_hi']}")
+        return (f"{config}: {cfg['basins']} basins, {wl.R} reaches (sizes {sizes}, generator "
+                f"window {cfg['window']}), {T} hourly steps, checkpoint window "
+                f"{window or (T - 1)}, forward + adjoint, MSE at {len(wl.gauges)} basin outlets")
+    return (f"{config}: {wl.R}-reach tree (window {cfg['window']}), {T} hourly steps, "
+            f"forward + adjoint, MSE at {len(wl.gauges)} gauges")
+
+
+def copy_peak(torch, nbytes=4 << 30, reps=10):
+    """Device-to-device copy bandwidth (bytes read + written per second) of a
+    plain torch copy of nbytes, HIP events on the current stream: the
+    attainable HBM rate beside the 8 TB/s published peak."""
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    b.copy_(a)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(reps):
+        b.copy_(a)
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    gbs = 2 * nbytes / (ms * 1e-3) / 1e9
+    return {"GBs": gbs, "bytes_copied": nbytes, "ms": ms, "frac_of_peak": gbs / HBM_PEAK_GBS}
+
+
+def kernel_source_hash() -> str:
+    """sha256 (16 hex digits) of the window kernels' sources."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in ("ddrs_kernels.hip", "mc_physics.h", "mc_math.h", "ddrs_internal.h"):
+        with open(os.path.join(ROOT, "ddrs_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def stored_traffic(config):
+    """The forward window's HBM bytes per launch from the committed rocprofv3
+    counter profile (profiles/traffic_fwd_window_<config>.json), only when it
+    was measured on the current kernel sources: (bytes or None, note)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_fwd_window_{config}.json")
+    if not os.path.exists(path):
+        return None, "no counter profile for this workload"
+    with open(path) as fh:
+        prof = json.load(fh)
+    if prof.get("kernels_sha16") != kernel_source_hash():
+        return None, (f"the counter profile ({prof.get('source')}) was taken on other kernel "
+                      f"sources; re-measure")
+    return prof["bytes_per_launch"], (f"rocprofv3 FETCH_SIZE + WRITE_SIZE per launch from "
+                                      f"{prof.get('source')} (kernel sources "
+                                      f"{prof['kernels_sha16']}), not measured in this run")
+
+
+def parity_leg(wl, qp, Qdense, res):
+    """The timed step's discharge (a dense forward at every reach-step, and the
+    gauge readout) and gradients against the oracle's three variants on the
+    whole network, and FP64 central differences on 64 seeded reaches
+    (oracle/scale_check.py). Returns (parity, grad_check)."""
+    from concurrent.futures import ThreadPoolExecutor
     from oracle import oracle as O
-    R, T = 2000, 24
-    rows, cols = NW.gen_tree(R, 256, 11)
-    at = NW.gen_attributes(R, 12)
-    qp = NW.gen_forcing(R, T, 13, storms=True)
+    from oracle import scale_check as SC
+    t0 = time.time()
+    R, T = wl.R, wl.T
     P = O.make_params()
-    pat = O.pattern_from_coo(R, rows, cols)
-    q0 = O.hotstart(P, pat, qp[0])
+    pat = O.pattern_from_coo(R, wl.rows, wl.cols)
+    at = wl.attrs
     p = np.full(R, 21.0, np.float32)
-    net = Network(SparseAdjacency.from_coo(R, rows, cols, None, at["length"], at["slope"]))
-    f = lambda x: torch.as_tensor(np.ascontiguousarray(x, np.float32), device="cuda:0")
-    w = np.random.default_rng(14).uniform(0.5, 1.5, (T, R))
-    n = f(at["n"]).requires_grad_(True)
-    qsp = f(at["q_spatial"]).requires_grad_(True)
-    Q = D.route_window(net, D.Params(), n, qsp, f(p), f(qp), f(q0), f(at["length"]), f(at["slope"]))
-    Q.backward(f(w))
-    g = {"n": n.grad.cpu().numpy(), "qsp": qsp.grad.cpu().numpy()}
-    base = {"n": at["n"].astype(np.float64), "qsp": at["q_spatial"].astype(np.float64)}
-
-    def loss(v):
-        Qd = O.route_fwd(P, pat, v["n"], v["qsp"], p, at["length"], at["slope"], qp,
-                         q0=q0.astype(np.float64), dtype=np.float64)
-        return float((w * Qd).sum())
-
-    worst = 0.0
-    for i in np.random.default_rng(15).choice(R, 8, replace=False):
-        for key in ("n", "qsp"):
-            vp = {k: v.copy() for k, v in base.items()}
-            vm = {k: v.copy() for k, v in base.items()}
-            h = 1e-6 * max(abs(base[key][i]), 1e-3)
-            vp[key][i] += h
-            vm[key][i] -= h
-            fd = (loss(vp) - loss(vm)) / (2 * h)
-            worst = max(worst, abs(g[key][i] - fd) / max(abs(fd), 1e-12))
-    return {"max_rel_err_vs_fp64_fd": worst, "reaches": 8, "params": ["n", "q_spatial"],
-            "network": "2000-reach tree, 24 steps"}
+    gQd = np.zeros((T, R), np.float32)
+    gQd[:, wl.gauges] = res["gQ"]
+    with ThreadPoolExecutor(SC.threads()) as pool:
+        r = SC.route_pair(P, pat, at["n"], at["q_spatial"], p, at["length"], at["slope"], qp,
+                          gQd, res["q0"], want_x64=True, pool=pool)
+        log(f"[bench] oracle (f32, f64, device-math f32) done in {time.time() - t0:.0f}s")
+        par = {"discharge_all_reach_steps": SC.discharge_errors(Qdense, r),
+               "discharge_gauges": SC.discharge_errors(res["Q"], r, wl.gauges)}
+        for k in SC.GRAD_KEYS:
+            par[f"grad_{k}"] = SC.grad_errors(res[k], r, k)
+        reaches = np.sort(np.random.default_rng(4102).choice(R, 64, replace=False))
+        fd = SC.fd_check(P, pat, {"n": at["n"], "q_spatial": at["q_spatial"], "p_spatial": p},
+                         at["length"], at["slope"], qp, r["f64"]["Q"], r["f64"]["X"], res["gQ"],
+                         wl.gauges, reaches, pool=pool)
+    fe = SC.fd_errors(fd, reaches, res, r)
+    for k in fe:
+        for key in ("resolved_mask", "gpu_rel_err_per_reach", "f32_oracle_rel_err_per_reach",
+                    "fp64_adjoint_rel_err_per_reach"):
+            fe[k].pop(key)
+    par["seconds"] = time.time() - t0
+    par["note"] = ("oracle/scale_check.py: f32 = the reference's arithmetic with glibc powf/"
+                   "logf; device-math = the same with correctly rounded powers; the discharge "
+                   "spread between the two is libm's rounding amplified by the recurrence")
+    grad = {"max_rel_err_vs_fp64_fd": max(fe[k]["max_rel_err_gpu"] for k in fe),
+            "reaches": 64, "params": ["n", "q_spatial"],
+            "network": f"the benchmark network itself ({R} reaches x {T} steps); the loss is "
+                       f"the timed step's gauge MSE linearised at its dL/dQ",
+            "per_param": fe}
+    log(f"[bench] parity: discharge bit-identical to the device-math oracle at "
+        f"{par['discharge_all_reach_steps']['bit_identical_to_device_math_oracle']:.6f} of "
+        f"reach-steps, FD max rel err {grad['max_rel_err_vs_fp64_fd']:.2e} "
+        f"({par['seconds']:.0f}s)")
+    return par, grad
 
 
 def kan_flops(F, H, P, L, k):
@@ -684,54 +679,27 @@ def train_bench(args, torch):
     device). Reach-timesteps/s of the whole step; the routing kernels' share
     from HIP events; the plain per-timestep-launch form of the same step once,
     with its gradients checked bitwise against the persistent form's."""
-    import ddrs_amd as D
     from ddrs_amd import _native as NAT
-    from ddrs_amd import kan as K
-    from ddrs_amd import loss as LS
-    from ddrs_amd import network as NW
     from ddrs_amd import train as TR
-    from ddrs_amd.sparse import Network, SparseAdjacency
 
-    cfg = CONFIGS["c5"]
-    T, R0, ci = cfg["T"], cfg["R"], 5
+    cfg = W.CONFIGS["c5"]
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     t0 = time.time()
-    rows, cols = NW.gen_tree(R0, cfg["window"], 1000 + ci)
-    g_full = TR.nested_gauges(R0, rows, cols, cfg["gauges"], cfg["size_lo"], cfg["size_hi"],
-                              seed=7000 + ci)
-    basin = TR.upstream_sizes(R0, rows, cols)[g_full]
-    ur, uc = TR.upstream_union(R0, rows, cols, g_full)
-    active, cr, cc, cg = NW.compress(ur, uc, g_full)
-    R = int(active.size)
-    at0 = NW.gen_attributes(R0, 2000 + ci)
-    at = {k: np.ascontiguousarray(v[active]) for k, v in at0.items()}
-    adj = SparseAdjacency.from_coo(R, cr, cc, None, at["length"], at["slope"])
-    gen = torch.Generator(device=dev).manual_seed(5000 + ci)
-    names = [f"attr_{i}" for i in range(KAN_SHAPE["F"])]
-    x = torch.randn(R, len(names), generator=gen, device=dev)
-    qprime = NW.gen_forcing_device(R, T, 3000 + ci, cfg["storms"], device=dev)
-    net, window, schedule = schedule_for(args, adj, T, qprime, cg, cfg["ckpt"])
+    wl = W.build_c5()
+    T, R, R0 = wl.T, wl.R, wl.extra["R0"]
+    basin = wl.extra["basin_sizes"]
+    cg = wl.gauges
+    qprime = wl.forcing_device(dev)
+    net, window, schedule = schedule_for(args, wl.adjacency(), T, qprime, cg, cfg["ckpt"])
     log(f"[bench] c5 batch: {R} of {R0} reaches, {cg.size} gauges (basins {basin.min()}-"
         f"{basin.max()}), {net.n_levels} levels; setup {time.time() - t0:.1f}s; "
         f"schedule {schedule}")
-    mk = lambda seed: K.KanHeadConfig(names, ["n", "q_spatial"], seed,
-                                      hidden_size=KAN_SHAPE["H"],
-                                      num_hidden_layers=KAN_SHAPE["L"], grid=KAN_SHAPE["grid"],
-                                      k=KAN_SHAPE["k"]).init(dev)
     tau, wu, eps = cfg["tau"], cfg["warmup"], cfg["eps"]
     # observations: a hidden head's routed discharge at the gauges, daily
     # pooled, times LogNormal(0, 0.1) noise; sigma is each gauge's std
-    truth = TR.Trainer(mk(4242), tau=tau, warmup=wu, eps=eps, window=window, network=net)
-    probe = TR.GaugedBatch(adj, x, qprime, cg, None, None)
-    with torch.no_grad():
-        Qg = truth.predict(probe)
-        daily = LS.tau_trim_and_downsample(Qg.t().contiguous(), tau)
-        obs = daily * torch.exp(cfg["noise"] * torch.randn(daily.shape, generator=gen, device=dev))
-        sigma = obs.std(dim=1)
-    del truth, Qg, daily
-    batch = TR.GaugedBatch(adj, x, qprime, cg, obs.contiguous(), sigma.contiguous())
-    head = mk(42)
+    batch = W.c5_batch(wl, qprime, net, window, dev)
+    head = W.kan_head(42, dev)
     tr = TR.Trainer(head, tau=tau, warmup=wu, eps=eps, window=window, network=net)
     log(f"[bench] c5 observations ready ({time.time() - t0:.1f}s)")
     first = tr.step(batch)
@@ -739,27 +707,25 @@ def train_bench(args, torch):
         tr.step(batch)
     ws = next(v for k, v in net._ws.items() if k[0] == "gauged")
     log(f"[bench] c5 workspace {ws.nbytes() / 2**30:.1f} GiB, first loss {first.loss:.5f}")
-    NAT.check(NAT.lib.ddrs_set_timing(ws.handle, 1))
-    fwd_ms = adj_ms = 0.0
-    phase_ms = np.zeros(3)
+    # launch times summed over the timed steps, read once after them
+    NAT.check(NAT.lib.ddrs_set_timing(ws.handle, 2))
     losses = []
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     for k in range(args.steps):
-        out = tr.step(batch)
+        out = tr.step(batch)   # the step reads its loss back, as the training loop does
         losses.append(out.loss)
-        a = [np.zeros(1, np.int32), np.zeros(1, np.float32), np.zeros(1, np.int32),
-             np.zeros(1, np.float32)]
-        NAT.check(NAT.lib.ddrs_get_timing(ws.handle, *[v.ctypes.data for v in a]))
-        fwd_ms += float(a[1][0])
-        adj_ms += float(a[3][0])
-        ph = [np.zeros(1, np.float32) for _ in range(3)]
-        NAT.check(NAT.lib.ddrs_get_adj_phase_timing(ws.handle, *[v.ctypes.data for v in ph]))
-        phase_ms += [float(v[0]) for v in ph]
         log(f"[bench] c5 step {k + 1}/{args.steps}: loss {out.loss:.5f}, grad norm "
             f"{out.grad_norm:.4g}")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t1
+    a = [np.zeros(1, np.int32), np.zeros(1, np.float32), np.zeros(1, np.int32),
+         np.zeros(1, np.float32)]
+    NAT.check(NAT.lib.ddrs_get_timing(ws.handle, *[v.ctypes.data for v in a]))
+    ph = [np.zeros(1, np.float32) for _ in range(3)]
+    NAT.check(NAT.lib.ddrs_get_adj_phase_timing(ws.handle, *[v.ctypes.data for v in ph]))
+    fwd_ms = float(a[1][0])
+    phase_ms = np.array([float(v[0]) for v in ph])
     NAT.check(NAT.lib.ddrs_set_timing(ws.handle, 0))
     free_b, total_b = torch.cuda.mem_get_info()
     rs = R * (T - 1)
@@ -800,8 +766,7 @@ def train_bench(args, torch):
         tr.form = NAT.FORM_PERSISTENT
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(cr, cc, at, R, max(1, int(8e7 // R)), ci, cfg["storms"], net,
-                           D.Params())
+        cpu = cpu_baseline(wl, max(1, int(8e7 // R)))
         cpu["sample"] += " (routing forward + adjoint; the KAN head and loss add <1% there)"
     return {
         "metric": "reach-timesteps/s fwd+adjoint (batched vs plain, 1 GPU); grad rel err vs FD",
@@ -841,26 +806,29 @@ def train_bench(args, torch):
     }
 
 
-def cpu_baseline(rows, cols, at, R, steps, ci, storms, net, params):
-    """The oracle (f32 restatement of the reference CPU path, one thread) on the
-    same network for `steps` timesteps, forward + adjoint."""
+def cpu_baseline(wl, steps):
+    """The oracle (the f32 restatement of the reference CPU path, one thread) on
+    the workload's own network and forcing (the host generator, bit-identical
+    to the device's) for `steps` timesteps, forward + adjoint."""
     from oracle import oracle as O
-    from ddrs_amd import network as NW
     T = steps + 1
-    qp = NW.gen_forcing(R, T, 3000 + ci, storms)
+    R = wl.R
+    qp = wl.forcing_host(T)
     P = O.make_params()
-    pat = O.pattern_from_coo(R, rows, cols)
+    pat = O.pattern_from_coo(R, wl.rows, wl.cols)
     q0 = O.hotstart(P, pat, qp[0])
     p = np.full(R, 21.0, np.float32)
     gQ = np.zeros((T, R), np.float32)
-    gQ[:, R - 1] = 1.0
+    gQ[:, wl.gauges] = 1.0
+    at = wl.attrs
     t0 = time.perf_counter()
     O.route_adj(P, pat, at["n"], at["q_spatial"], p, at["length"], at["slope"], qp, gQ, q0=q0)
     dt = time.perf_counter() - t0
     log(f"[bench] cpu oracle: {dt:.1f}s for {R}x{steps}")
     return {"value": R * steps / dt, "unit": "reach-timesteps/s", "cores": 1, "kind": "port",
-            "sample": f"same {R}-reach network, {steps} timesteps; oracle route_adj "
-                      "(forward, per-step recompute and adjoint), fp32, one thread",
+            "sample": f"same {R}-reach network and forcing, {steps} timesteps; oracle "
+                      f"route_adj (forward, per-step recompute and adjoint), fp32, one thread "
+                      f"({dt:.1f} s)",
             "cpu_model": _cpu_model(), "host_cores": os.cpu_count()}
 
 

@@ -8,6 +8,11 @@ from __future__ import annotations
 import ctypes
 import os
 
+# torch first: PyTorch-ROCm ships its own HIP runtime, and the library must
+# bind to the one torch's tensors and streams live in. Loaded the other way
+# round, the process holds two runtimes and the library's calls find no device.
+import torch  # noqa: F401
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libddrs_hip.so")
 

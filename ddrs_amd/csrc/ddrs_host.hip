@@ -117,7 +117,7 @@ struct ddrs_workspace {
     int lin_seg = -1;
     ddrs_params lin_params{};
     // timing: pairs of events around each launch of a kind
-    bool timing = false;
+    int timing = 0;  // 0 off, 1 the last window calls, 2 every call since set_timing
     std::vector<hipEvent_t> ev[5];  // forward window, lin, sweep, fin, lin tail
     int n_ev[5] = {0, 0, 0, 0, 0};
     // per-chunk traces of the last forward and sweep launches, per-item trace
@@ -1093,7 +1093,7 @@ static ddrs_status window_fwd(ddrs_workspace_t ws, const ddrs_params* params, in
         return fail(DDRS_ERR_INVALID, "unknown form");
     hipStream_t s = S(stream);
     const int P = ws->plan->P;
-    ws->n_ev[TK_FWD] = ws->n_ev[TK_LIN_TAIL] = 0;
+    if (ws->timing != 2) ws->n_ev[TK_FWD] = ws->n_ev[TK_LIN_TAIL] = 0;
     launch_state(ws->plan->sc, d_q0, 1, params->discharge_lb, ws->st_q, ws->st_i, d_Q_out, s);
     if (d_Qg_out) launch_gauge_row0(ws->greach, ws->gK, d_q0, params->discharge_lb, d_Qg_out, s);
     const int steps = T - 1;
@@ -1149,7 +1149,7 @@ static ddrs_status window_adj(ddrs_workspace_t ws, const ddrs_params* params, in
     hipStream_t s = S(stream);
     const Sched& sc = ws->plan->sc;
     const int R = sc.R, P = sc.P;
-    ws->n_ev[TK_LIN] = ws->n_ev[TK_SWEEP] = ws->n_ev[TK_FIN] = 0;
+    if (ws->timing != 2) ws->n_ev[TK_LIN] = ws->n_ev[TK_SWEEP] = ws->n_ev[TK_FIN] = 0;
     const int steps = T - 1;
     const int nseg = (steps + ws->window - 1) / ws->window;
     if (permute_qprime(ws, d_q_prime, steps, s) ||
@@ -1370,7 +1370,8 @@ ddrs_status ddrs_hotstart_adj(ddrs_workspace_t ws, const ddrs_params* params, co
 
 ddrs_status ddrs_set_timing(ddrs_workspace_t ws, int32_t enable) {
     if (!ws) return fail(DDRS_ERR_INVALID, "workspace is NULL");
-    ws->timing = enable != 0;
+    if (enable < 0 || enable > 2) return fail(DDRS_ERR_INVALID, "timing mode must be 0, 1 or 2");
+    ws->timing = enable;
     for (int& n : ws->n_ev) n = 0;
     return DDRS_OK;
 }

@@ -232,9 +232,11 @@ ddrs_status ddrs_hotstart_adj(ddrs_workspace_t ws, const ddrs_params* params,
  * Timing of the window kernels of the last window calls on their own stream
  * (HIP events around each launch): number of launches and summed milliseconds
  * of the forward-window kernel and of the adjoint's three kernels
- * (linearisation, reverse-time sweep, gradient finish). Enabled by
- * ddrs_set_timing(ws, 1). No reference counterpart (the reference times with
- * host clocks in benches/).
+ * (linearisation, reverse-time sweep, gradient finish). ddrs_set_timing(ws, 1)
+ * times the last forward and the last adjoint window call; mode 2 sums every
+ * call until the next ddrs_set_timing (no host synchronisation until the
+ * times are read); 0 turns timing off. No reference counterpart (the
+ * reference times with host clocks in benches/).
  * ------------------------------------------------------------------------- */
 ddrs_status ddrs_set_timing(ddrs_workspace_t ws, int32_t enable);
 ddrs_status ddrs_get_timing(ddrs_workspace_t ws, int32_t* fwd_launches, float* fwd_ms,

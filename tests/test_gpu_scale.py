@@ -302,8 +302,9 @@ def test_c3_forest_sampled_basins():
 
 @pytest.mark.timeout(1200)
 def test_c4_forest_sampled_basins():
-    """c4: 200,000 basins (2.94 M reaches) x 8,760 steps in checkpoint segments
-    of 168; the largest basin (300,000 reaches) over its first 720 steps, 20
+    """c4: 200,000 basins (2.94 M reaches; Zipf(1.8) sizes capped at 300,000,
+    then scaled to the total, so the largest holds 111,430) x 8,760 steps in
+    checkpoint segments of 168; the largest basin over its first 720 steps, 20
     seeded others and those nearest 1,000 and 10,000 reaches at full length."""
     _forest_parity("c4", 20, (1000, 10000), cut=720)
 
