@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "ddrs_epoch.h"
 #include "ddrs_internal.h"
 
 using namespace ddrs;
@@ -807,7 +808,7 @@ static bool permute_qprime(ddrs_workspace* ws, const float* q, int rows, hipStre
 static Claim begin_launch(ddrs_workspace* ws, int grid, hipStream_t s) {
     launch_begin(ws->ls, ws->claim_ctr, s);
     ws->claims += static_cast<unsigned long long>(grid);
-    ++ws->epoch;
+    ws->epoch = next_epoch(ws->epoch);
     return Claim{ws->claim_ctr, ws->ls};
 }
 

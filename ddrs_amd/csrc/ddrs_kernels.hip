@@ -35,6 +35,7 @@
 #include <stdint.h>
 
 #include "mc_physics.h"
+#include "ddrs_epoch.h"
 #include "ddrs_internal.h"
 
 namespace ddrs {
@@ -114,7 +115,7 @@ __device__ __forceinline__ uint32_t launch_epoch(const Claim& cl) { return cl.ls
 __global__ void k_begin_launch(LaunchState* ls, const unsigned long long* ctr) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         ls->claim_base = *ctr;
-        ls->epoch = ls->epoch + 1u;
+        ls->epoch = next_epoch(ls->epoch);
     }
 }
 

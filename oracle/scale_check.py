@@ -7,7 +7,7 @@ on the same inputs. Nothing here touches the GPU.
 route_pair     the oracle's forward + adjoint of one network in float (the
                reference's arithmetic), double, and float with the device's
                powers, in three threads (ctypes releases the GIL), with the
-               same dL/dQ
+               same dL/dQ (route_fwd_set: the forward alone)
 discharge_errors / grad_errors
                the parity figures: max relative discharge error against the
                f32 oracle (everywhere, and where the f32 oracle is itself
@@ -54,6 +54,22 @@ def route_pair(P, pat, n, qsp, p, length, slope, qprime, gQ, q0, want_x64=False,
                           np.asarray(gQ, np.float64), q0=np.asarray(q0, np.float64),
                           dtype=np.float64, want_q=True, want_x=want_x64)
         return {"f32": f32.result(), "f64": f64.result(), "dev": dev.result()}
+    finally:
+        if own:
+            pool.shutdown()
+
+
+def route_fwd_set(P, pat, n, qsp, p, length, slope, qprime, q0, pool=None):
+    """route_pair's three variants, forward only: {variant: {"Q": [T][R]}}."""
+    own = pool is None
+    pool = pool or ThreadPoolExecutor(3)
+    try:
+        f = lambda dt, qp, q: pool.submit(O.route_fwd, P, pat, n, qsp, p, length, slope, qp,
+                                          q0=q, dtype=dt)
+        futs = {"f32": f(np.float32, qprime, q0), "dev": f(O.F32_DEVICE_MATH, qprime, q0),
+                "f64": f(np.float64, np.ascontiguousarray(qprime, np.float64),
+                         np.asarray(q0, np.float64))}
+        return {k: {"Q": v.result()} for k, v in futs.items()}
     finally:
         if own:
             pool.shutdown()

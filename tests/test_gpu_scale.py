@@ -309,9 +309,10 @@ def test_c4_forest_sampled_basins():
     _forest_parity("c4", 20, (1000, 10000), cut=720)
 
 
-def _tree_parity(wl, reach_ids, T, Qg, gQ, gn, gqsp, nq, q0, qp):
+def _tree_parity(wl, reach_ids, T, Qg, gQ, gn, gqsp, nq, q0, qp, grads=True):
     """One tree of the c5 batch (positions reach_ids, ascending) routed alone by
-    the oracle with the device's n and q_spatial, against the batch run."""
+    the oracle with the device's n and q_spatial, against the batch run (the
+    discharge only, without grads)."""
     R = reach_ids.size
     loc = np.full(wl.R, -1, np.int64)
     loc[reach_ids] = np.arange(R)
@@ -324,6 +325,9 @@ def _tree_parity(wl, reach_ids, T, Qg, gQ, gn, gqsp, nq, q0, qp):
     p = np.full(R, 21.0, np.float32)
     q0t = q0[reach_ids]
     np.testing.assert_array_equal(q0t, O.hotstart(P, pat, qp[0]))
+    if not grads:
+        r = SC.route_fwd_set(P, pat, n, qsp, p, at["length"], at["slope"], qp, q0t)
+        return {"reaches": int(R), "steps": T, "discharge": SC.discharge_errors(Qg, r)}
     r = SC.route_pair(P, pat, n, qsp, p, at["length"], at["slope"], qp, gQ, q0t)
     figs = {"reaches": int(R), "steps": T, "discharge": SC.discharge_errors(Qg, r)}
     got = {"n": gn[reach_ids], "q_spatial": gqsp[reach_ids]}
@@ -335,9 +339,10 @@ def _tree_parity(wl, reach_ids, T, Qg, gQ, gn, gqsp, nq, q0, qp):
 @pytest.mark.timeout(1200)
 def test_c5_training_batch_sampled_trees():
     """c5: the training micro-batch (285,985 reaches in 62 trees, 2,000 gauges,
-    8,760 steps): the device's loss stage, the routing of the largest tree and
-    of a seeded sample of the others at full length, and the KAN backward of
-    the device's per-reach routing gradients over every reach."""
+    8,760 steps): the device's loss stage; at full length the discharge of the
+    largest tree (60,997 reaches) and the discharge and gradients of the trees
+    nearest 5,000 and 12,000 reaches and 8 seeded others; and the KAN backward
+    of the device's per-reach routing gradients over every reach."""
     from oracle import kan_oracle as ko
     from oracle import loss_oracle as lo
     from tests import train_case as TC
@@ -407,7 +412,10 @@ def test_c5_training_batch_sampled_trees():
     say(f"c5: {len(jobs)} trees ({sum(j[1].size for j in jobs)} reaches) to the oracle "
         f"({time.time() - t0:.0f}s)")
     with ThreadPoolExecutor(max(1, SC.threads() // 2)) as pool:
-        futs = {pool.submit(_tree_parity, wl, ids, T, Qt, gQt, gn, gqsp, nq, q0, qpt): i
+        # the largest tree (61k reaches x 8,760 steps) is compared on its
+        # discharge: its adjoint alone would take the oracle several minutes
+        futs = {pool.submit(_tree_parity, wl, ids, T, Qt, gQt, gn, gqsp, nq, q0, qpt,
+                            i != int(order[0])): i
                 for i, ids, Qt, gQt, qpt in jobs}
         # the KAN backward over every reach in chunks (its gradients are sums
         # over reaches), from the device's dL/d(n, q_spatial)
@@ -450,6 +458,7 @@ def test_c5_training_batch_sampled_trees():
     for i, f in per.items():
         assert_discharge(f"c5 tree {i}", f["discharge"])
         for k in SC.GRAD_KEYS:
-            assert_grad(f"c5 tree {i} {k}", f[f"grad_{k}"])
+            if f"grad_{k}" in f:
+                assert_grad(f"c5 tree {i} {k}", f[f"grad_{k}"])
     for k, e in figs["kan_backward_norm_err"].items():
         assert e <= 1e-4, (k, e)
