@@ -81,6 +81,7 @@ _SIGS = {
     "ddrs_tri_solve_fwd": (c_int, [P, P, P, P, P]),
     "ddrs_tri_solve_adj": (c_int, [P, P, P, P, P, P, P]),
     "ddrs_hotstart": (c_int, [P, ctypes.POINTER(DdrsParams), P, P, P]),
+    "ddrs_hotstart_noalloc": (c_int, [P, ctypes.POINTER(DdrsParams), P, P, P, P]),
     "ddrs_hotstart_adj": (c_int, [P, ctypes.POINTER(DdrsParams), P, P, P, P]),
     "ddrs_set_timing": (c_int, [P, c_int]),
     "ddrs_get_timing": (c_int, [P, P, P, P, P]),

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -1510,3 +1511,87 @@ ddrs_status ddrs_workspace_layout(ddrs_workspace_t ws, int32_t* rpl_fwd, int32_t
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// allocation-free cold start (ddrs_hotstart_noalloc)
+//
+// Its scratch is a file-scope device array, one copy per GPU, instead of a
+// workspace: a head holding the chunk-claim counter and the launch state,
+// then one tagged 64-bit word per schedule position. Device memory starts
+// zeroed, and the epoch sequence never returns to 0, so the words need no
+// clearing; every call shares the array (calls that overlap on one GPU race).
+
+namespace {
+
+// the largest plan served: 2^22 positions, 32 MiB of words per GPU
+constexpr size_t kNoallocMaxPositions = size_t(1) << 22;
+constexpr size_t kNoallocHead = 256;  // claim counter at 0, LaunchState at 64
+static __device__ __align__(256) unsigned char scratch[kNoallocHead + kNoallocMaxPositions * 8];
+
+constexpr int kNoallocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+unsigned char* g_noalloc_ptr[kNoallocMaxDevices] = {};
+
+// The device address of `scratch` on the current GPU, looked up once per GPU
+// (with that GPU current) and kept: the array's name in host code is not a
+// device address.
+ddrs_status noalloc_scratch(unsigned char** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoallocMaxDevices)
+        return fail(DDRS_ERR_INVALID, "device index outside [0, 64)");
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(DDRS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                          hipGetErrorString(e));
+        g_noalloc_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_noalloc_ptr[dev];
+    return DDRS_OK;
+}
+
+}  // namespace
+
+extern "C" ddrs_status ddrs_hotstart_noalloc(ddrs_plan_t plan, const ddrs_params* params,
+                                             const float* d_q_prime0, float* d_q0,
+                                             int32_t* d_err, void* stream) {
+    if (!plan) return fail(DDRS_ERR_INVALID, "plan is NULL");
+    ddrs_status st = check_params(params);
+    if (st != DDRS_OK) return st;
+    if (!d_q_prime0 || !d_q0 || !d_err) return fail(DDRS_ERR_INVALID, "NULL argument");
+    if (static_cast<size_t>(plan->P) > kNoallocMaxPositions)
+        return fail(DDRS_ERR_CAPACITY,
+                    "ddrs_hotstart_noalloc: the plan has " + std::to_string(plan->P) +
+                        " schedule positions, more than the " +
+                        std::to_string(kNoallocMaxPositions) +
+                        " its scratch holds; use ddrs_hotstart with a workspace");
+    if (!plan->on_device)
+        return fail(DDRS_ERR_INVALID,
+                    "ddrs_hotstart_noalloc: the plan's schedule is not on the device; create a "
+                    "workspace of the plan once, at set-up, before calling it");
+    unsigned char* base = nullptr;
+    st = noalloc_scratch(&base);
+    if (st != DDRS_OK) return st;
+    hipStream_t s = S(stream);
+    launch_zero(1, reinterpret_cast<float*>(d_err), s);  // fill kernel: no memset node
+    auto* ctr = reinterpret_cast<unsigned long long*>(base);
+    auto* ls = reinterpret_cast<LaunchState*>(base + 64);
+    launch_begin(ls, ctr, s);
+    TriArgs a{};
+    a.sc = plan->sc;
+    a.ch = plan->ch[0].dev;
+    a.claim = Claim{ctr, ls};
+    a.a_values = nullptr;  // A = I - N
+    a.b = d_q_prime0;
+    a.x = d_q0;
+    a.slot = reinterpret_cast<uint64_t*>(base + kNoallocHead);
+    a.clamp = 1;
+    a.lb = params->discharge_lb;
+    a.err = d_err;
+    launch_tri_fwd(a, plan->ch[0].C, s);
+    HIPCHK(hipGetLastError());
+    return DDRS_OK;
+}

@@ -227,6 +227,21 @@ ddrs_status ddrs_hotstart(ddrs_workspace_t ws, const ddrs_params* params,
 ddrs_status ddrs_hotstart_adj(ddrs_workspace_t ws, const ddrs_params* params,
                               const float* d_q0, const float* d_grad_q0,
                               float* d_grad_q_prime0, void* stream);
+/* ddrs_hotstart that allocates nothing (no counterpart in the reference): for
+ * callers that cannot call hipMalloc, such as code under HIP graph capture or
+ * a serving loop. The same kernel and results as ddrs_hotstart, with its
+ * scratch (chunk claims, launch state, one tagged word per schedule position)
+ * in a file-scope device array of the library, one per GPU, for plans of at
+ * most 4,194,304 schedule positions (32 MiB); a larger plan is rejected with
+ * DDRS_ERR_CAPACITY. The plan's schedule must already be on the device (a
+ * workspace of the plan created once, at set-up, uploads it) and on the
+ * current GPU. *d_err (caller-owned device int) is set to 0, and to 1 when
+ * the solve's inputs did not arrive within the watchdog time. Every call on a
+ * GPU shares that GPU's scratch: calls that overlap on one GPU (different
+ * streams) race; serialise them. */
+ddrs_status ddrs_hotstart_noalloc(ddrs_plan_t plan, const ddrs_params* params,
+                                  const float* d_q_prime0, float* d_q0, int32_t* d_err,
+                                  void* stream);
 
 /* ---------------------------------------------------------------------------
  * Timing of the window kernels of the last window calls on their own stream
