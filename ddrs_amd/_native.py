@@ -93,6 +93,8 @@ _SIGS = {
     "ddrs_get_trace": (c_int, [P, c_int, P, c_int, P]),
     "ddrs_build_topology": (c_int, [c_int, P, P, c_int, P, P, P, c_int, P, P, c_int, P]),
     "ddrs_compress": (c_int, [c_int, P, P, c_int, P, P, c_int, P, P, P, P, P]),
+    "ddrs_compress_ex": (c_int, [c_int, P, P, c_int, P, c_int, c_int, P, P, c_int, P, P, P, P,
+                                 P, P, P, P, P, c_int, P]),
     "ddrs_gen_tree": (c_int, [c_int, c_int, c_u64, P, P]),
     "ddrs_gen_attributes": (c_int, [c_int, c_u64, P, P, P, P]),
     "ddrs_gen_forcing": (c_int, [c_int, c_int, c_u64, c_int, P]),

@@ -4,6 +4,8 @@
 //                        rule, cycle drop and retry, isolated COMIDs appended)
 //   ddrs_compress        data/collate.rs:197-349 (sorted active set, compressed
 //                        lower-triangular COO, gauge positions)
+//   ddrs_compress_ex     the same with the compressed reach-subdivision map
+//                        (collate.rs:141-180) and outflow_idx (297-336)
 #include <algorithm>
 #include <cstdint>
 #include <string>
@@ -269,6 +271,102 @@ ddrs_status ddrs_compress(int32_t n_edges_in, const int32_t* h_rows_in, const in
     std::copy(act.begin(), act.end(), h_active);
     if (n_active) *n_active = static_cast<int32_t>(act.size());
     if (n_edges) *n_edges = static_cast<int32_t>(e.size());
+    return DDRS_OK;
+}
+
+namespace {
+
+// Owner of sub-reach row `row` in a strictly increasing parent map: the last
+// offset <= row (collate.rs:124-129).
+int parent_of_row(const int32_t* off, int n_off, int row) {
+    return static_cast<int>(std::upper_bound(off, off + n_off, row) - off) - 1;
+}
+
+}  // namespace
+
+ddrs_status ddrs_compress_ex(int32_t n_edges_in, const int32_t* h_rows_in,
+                             const int32_t* h_cols_in, int32_t n_gauges,
+                             const int32_t* h_gauges, int32_t ddr_match,
+                             int32_t n_conus_parent, const int32_t* h_conus_parent_offset,
+                             int32_t* h_active, int32_t active_cap, int32_t* n_active,
+                             int32_t* h_rows, int32_t* h_cols, int32_t* n_edges,
+                             int32_t* h_gauge, int32_t* h_parent_offset, int32_t* n_parent,
+                             int32_t* h_outflow_ptr, int32_t* h_outflow, int32_t outflow_cap,
+                             int32_t* n_outflow) {
+    if (n_conus_parent < 0 || (n_conus_parent > 0 && !h_conus_parent_offset))
+        return ifail(DDRS_ERR_INVALID, "compress: bad parent map");
+    if (n_gauges > 0 && !h_outflow_ptr) return ifail(DDRS_ERR_INVALID, "compress: NULL outflow_ptr");
+    int32_t na = 0, ne = 0;
+    ddrs_status st = ddrs_compress(n_edges_in, h_rows_in, h_cols_in, n_gauges, h_gauges, h_active,
+                                   active_cap, &na, h_rows, h_cols, &ne, h_gauge);
+    if (st != DDRS_OK) return st;
+    if (n_active) *n_active = na;
+    if (n_edges) *n_edges = ne;
+    // 4b. the reach-subdivision map in compressed rows: every parent present
+    // must be present whole, as a contiguous run of its CONUS rows
+    // (collate.rs:141-180)
+    std::vector<int32_t> poff;
+    if (n_conus_parent > 0) {
+        const int32_t* off = h_conus_parent_offset;
+        const int n_off = n_conus_parent + 1;
+        for (int k = 0; k + 1 < n_off; ++k)
+            if (!(off[k + 1] > off[k]) || off[0] != 0)
+                return ifail(DDRS_ERR_INVALID, "compress: parent map not strictly increasing from 0");
+        const int n_rows = off[n_conus_parent];
+        poff.push_back(0);
+        int i = 0;
+        while (i < na) {
+            const int row = h_active[i];
+            if (row >= n_rows)
+                return ifail(DDRS_ERR_INVALID,
+                             "compress: CONUS row " + std::to_string(row) +
+                                 " is outside the parent map (which covers " +
+                                 std::to_string(n_rows) + " rows)");
+            const int p = parent_of_row(off, n_off, row);
+            const int lo = off[p], hi = off[p + 1], m = hi - lo;
+            bool whole = na - i >= m;
+            for (int k = 0; whole && k < m; ++k) whole = h_active[i + k] == lo + k;
+            if (!whole)
+                return ifail(DDRS_ERR_INVALID,
+                             "compress: parent " + std::to_string(p) + " owns CONUS rows " +
+                                 std::to_string(lo) + ".." + std::to_string(hi) +
+                                 ", but the active set does not hold them contiguously from "
+                                 "compressed row " + std::to_string(i));
+            i += m;
+            poff.push_back(i);
+        }
+        if (static_cast<int64_t>(poff.size()) > static_cast<int64_t>(active_cap) + 1 ||
+            !h_parent_offset)
+            return ifail(DDRS_ERR_CAPACITY, "compress: parent map capacity too small");
+        std::copy(poff.begin(), poff.end(), h_parent_offset);
+    }
+    if (n_parent) *n_parent = static_cast<int32_t>(poff.empty() ? 0 : poff.size() - 1);
+    // 5. outflow_idx (collate.rs:297-336): ddr_match, the gauge's upstream
+    // columns in edge order (its own reach when it has none); otherwise the
+    // gauge's own reach, at its parent's outlet piece when there is a map
+    std::vector<int32_t> flat;
+    if (n_gauges > 0) h_outflow_ptr[0] = 0;
+    for (int g = 0; g < n_gauges; ++g) {
+        const int gc = h_gauge[g];
+        const size_t before = flat.size();
+        if (ddr_match) {
+            // edges are sorted by (row, col): the gauge's row is one run
+            const int32_t* r0 = std::lower_bound(h_rows, h_rows + ne, gc);
+            for (const int32_t* r = r0; r < h_rows + ne && *r == gc; ++r)
+                flat.push_back(h_cols[r - h_rows]);
+            if (flat.size() == before) flat.push_back(gc);
+        } else if (!poff.empty()) {
+            const int p = parent_of_row(poff.data(), static_cast<int>(poff.size()), gc);
+            flat.push_back(poff[p + 1] - 1);
+        } else {
+            flat.push_back(gc);
+        }
+        h_outflow_ptr[g + 1] = static_cast<int32_t>(flat.size());
+    }
+    if (static_cast<int64_t>(flat.size()) > outflow_cap || (!flat.empty() && !h_outflow))
+        return ifail(DDRS_ERR_CAPACITY, "compress: outflow capacity too small");
+    std::copy(flat.begin(), flat.end(), h_outflow);
+    if (n_outflow) *n_outflow = static_cast<int32_t>(flat.size());
     return DDRS_OK;
 }
 

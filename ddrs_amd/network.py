@@ -3,6 +3,8 @@ index construction (topological order, batch compression), all native."""
 from __future__ import annotations
 
 import ctypes
+from dataclasses import dataclass
+from typing import Optional
 
 import numpy as np
 import torch
@@ -99,3 +101,66 @@ def compress(rows, cols, gauges):
                                 _ptr(gauges), _ptr(active), cap, ctypes.byref(na), _ptr(orow),
                                 _ptr(ocol), ctypes.byref(ne), _ptr(og)), "ddrs_compress")
     return active[:na.value], orow[:ne.value], ocol[:ne.value], og[:gauges.shape[0]]
+
+
+@dataclass
+class Compressed:
+    """collate::compress's outputs (data/collate.rs:52-104): the sorted active
+    set (CONUS sub-reach rows, `conus_positions`), the compressed
+    lower-triangular COO, each gauge's compressed position, outflow_idx (gauge
+    g sums the discharge of reaches outflow[outflow_ptr[g]:outflow_ptr[g+1]])
+    and the compressed reach-subdivision map (None without a CONUS map)."""
+
+    active: np.ndarray
+    rows: np.ndarray
+    cols: np.ndarray
+    gauge_compressed: np.ndarray
+    outflow_ptr: np.ndarray
+    outflow: np.ndarray
+    parent_offset: Optional[np.ndarray]
+
+    def outflow_idx(self):
+        """outflow_idx as a list of arrays, one per gauge."""
+        return [self.outflow[self.outflow_ptr[g]:self.outflow_ptr[g + 1]]
+                for g in range(self.gauge_compressed.size)]
+
+    def scatter_map(self):
+        """(reaches, flat, group): the distinct reaches a batch must read out
+        (sorted), and for every outflow entry its index into them and its
+        gauge, the inputs of scatter_add_by_group (training/forward.rs:18-34)."""
+        reaches = np.unique(self.outflow).astype(np.int32)
+        flat = np.searchsorted(reaches, self.outflow).astype(np.int32)
+        group = np.repeat(np.arange(self.gauge_compressed.size, dtype=np.int32),
+                          np.diff(self.outflow_ptr))
+        return reaches, flat, group
+
+
+def compress_ex(rows, cols, gauges, ddr_match: bool = False, parent_offset=None) -> Compressed:
+    """Unioned CONUS COO + gauge rows -> a compressed batch with outflow_idx and
+    the compressed parent map (data/collate.rs:197-349). parent_offset: the
+    CONUS store's reach-subdivision map (ConusAdjacencyStore.parent_offset) or
+    None."""
+    rows = np.ascontiguousarray(rows, np.int32)
+    cols = np.ascontiguousarray(cols, np.int32)
+    gauges = np.ascontiguousarray(gauges, np.int32)
+    G = gauges.shape[0]
+    cap = 2 * rows.shape[0] + G + 1
+    active = np.zeros(cap, np.int32)
+    orow = np.zeros(max(rows.shape[0], 1), np.int32)
+    ocol = np.zeros(max(rows.shape[0], 1), np.int32)
+    og = np.zeros(max(G, 1), np.int32)
+    po = None if parent_offset is None else np.ascontiguousarray(parent_offset, np.int32)
+    npar_in = 0 if po is None else po.shape[0] - 1
+    opo = np.zeros(cap + 1, np.int32)
+    optr = np.zeros(G + 1, np.int32)
+    ocap = rows.shape[0] + G + 1
+    oflow = np.zeros(ocap, np.int32)
+    na, ne, npar, nflow = (ctypes.c_int32() for _ in range(4))
+    N.check(N.lib.ddrs_compress_ex(rows.shape[0], _ptr(rows), _ptr(cols), G, _ptr(gauges),
+                                   int(bool(ddr_match)), npar_in, _ptr(po), _ptr(active), cap,
+                                   ctypes.byref(na), _ptr(orow), _ptr(ocol), ctypes.byref(ne),
+                                   _ptr(og), _ptr(opo), ctypes.byref(npar), _ptr(optr),
+                                   _ptr(oflow), ocap, ctypes.byref(nflow)), "ddrs_compress_ex")
+    return Compressed(active[:na.value].copy(), orow[:ne.value].copy(), ocol[:ne.value].copy(),
+                      og[:G].copy(), optr, oflow[:nflow.value].copy(),
+                      None if po is None else opo[:npar.value + 1].copy())

@@ -179,3 +179,37 @@ class GagesAdjacencyStore:
 
     def __len__(self) -> int:
         return len(self.subgraphs)
+
+
+@dataclass
+class UnionedCoo:
+    """The per-batch union of gauge subgraphs (data/collate.rs:16-25): distinct
+    edges in CONUS positions sorted by (row, col), and one (staid, gage_idx,
+    gage_catchment) per gauge present in the store, in request order."""
+
+    rows: np.ndarray
+    cols: np.ndarray
+    gauges: list
+
+    def gauge_rows(self) -> np.ndarray:
+        return np.asarray([g[1] for g in self.gauges], np.int32)
+
+
+def union_subgraphs(staids: Sequence[str], gages: GagesAdjacencyStore) -> UnionedCoo:
+    """union_subgraphs (data/collate.rs:31-48): gauges missing from the store
+    are skipped; the edges of the present ones are unioned, deduplicated and
+    sorted by (row, col), the order of the reference's BTreeSet."""
+    gauges, r, c = [], [], []
+    for s in staids:
+        g = gages.get(s)
+        if g is None:
+            continue
+        gauges.append((s, int(g.gage_idx), g.gage_catchment))
+        r.append(np.asarray(g.indices_0, np.int64))
+        c.append(np.asarray(g.indices_1, np.int64))
+    if r:
+        key = np.unique(np.concatenate(r) * (1 << 32) + np.concatenate(c))
+        rows, cols = (key >> 32).astype(np.int32), (key & 0xFFFFFFFF).astype(np.int32)
+    else:
+        rows = cols = np.zeros(0, np.int32)
+    return UnionedCoo(rows, cols, gauges)

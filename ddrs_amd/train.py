@@ -139,11 +139,14 @@ class GaugedBatch:
     adjacency     the batch's compressed network (reach positions 0..N-1);
     attributes    [N, F] normalised reach attributes, the KAN head's input;
     q_prime       [T, N] hourly lateral inflow (time-major);
-    gauges        [G] distinct reach positions whose discharge is the gauge's
-                  prediction (outflow_idx of a gauge is its own reach);
+    gauges        [G] the gauges' compressed reach positions;
     observations  [G, T_days] daily observations, NaN where missing;
     gauge_obs_std [G] per-gauge standard deviation of the observations (the
-                  nse-batch sigma; None for the L1 objective)."""
+                  nse-batch sigma; None for the L1 objective);
+    outflow       the batch's outflow_idx (data/collate.rs:297-336) as
+                  Compressed.scatter_map() gives it: (reaches read out,
+                  flat indices into them, gauge of each entry). None: each
+                  gauge's prediction is its own reach."""
 
     adjacency: SparseAdjacency
     attributes: torch.Tensor
@@ -151,6 +154,11 @@ class GaugedBatch:
     gauges: np.ndarray
     observations: torch.Tensor
     gauge_obs_std: Optional[torch.Tensor]
+    outflow: Optional[tuple] = None
+
+    def readout(self) -> np.ndarray:
+        """The reaches whose discharge the routing window reads out."""
+        return self.gauges if self.outflow is None else self.outflow[0]
 
 
 @dataclass
@@ -213,15 +221,25 @@ class Trainer:
             self._statics = (length, slope, p, x)
         return self._statics
 
-    def _gauge_plan(self, K: int, T: int) -> GaugePlan:
-        if self._plan_key != (K, T):
-            idx = np.arange(K, dtype=np.int32)
-            self._plan = GaugePlan(K, idx, idx, K, T)
-            self._plan_key = (K, T)
+    def _gauge_plan(self, batch: GaugedBatch, K: int, T: int) -> GaugePlan:
+        """scatter_add_by_group of the batch's outflow map over the K reaches
+        read out (the identity when each gauge reads its own reach)."""
+        if batch.outflow is None:
+            key = (K, T, None)
+            flat = grp = np.arange(K, dtype=np.int32)
+            G = K
+        else:
+            _, flat, grp = batch.outflow
+            G = int(np.asarray(batch.gauges).size)
+            key = (K, T, G, hash(np.asarray(flat).tobytes()), hash(np.asarray(grp).tobytes()))
+        if self._plan_key != key:
+            self._plan = GaugePlan(K, flat, grp, G, T)
+            self._plan_key = key
         return self._plan
 
     def predict(self, batch: GaugedBatch, trace: Optional[dict] = None) -> torch.Tensor:
-        """Hourly discharge at the gauges, [T, G], differentiable in the head's
+        """Hourly discharge at the batch's readout reaches (its gauges, or
+        their outflow reaches), [T, K], differentiable in the head's
         weights (training/forward.rs:301-387 with the gauge readout). trace: a
         dict that receives the routed physical parameters "n", "q_spatial",
         the initial state "q0" and the gauge discharge "Qg", each keeping its
@@ -238,7 +256,7 @@ class Trainer:
             p = denormalize(out["p_spatial"], rng.p_spatial, "p_spatial" in log)
         q0 = hotstart(net, P, batch.q_prime[0].detach())
         Qg = route_window_gauged(net, P, n, qsp, p, batch.q_prime.detach(), q0.detach(),
-                                 length, slope, batch.gauges, x_storage=x, form=self.form,
+                                 length, slope, batch.readout(), x_storage=x, form=self.form,
                                  window=self.window)
         if trace is not None:
             for v in (n, qsp, Qg):
@@ -252,7 +270,7 @@ class Trainer:
         filter and the configured objective (driver.rs:126-201) of
         predict(batch); n_valid is loss_denominator. trace as in predict."""
         Qg = self.predict(batch, trace)
-        plan = self._gauge_plan(Qg.shape[1], Qg.shape[0])
+        plan = self._gauge_plan(batch, Qg.shape[1], Qg.shape[0])
         loss, n_valid, _ = gauge_batch_loss(Qg, plan, self.tau, self.warmup, batch.observations,
                                             batch.gauge_obs_std, self.eps, kind=self.loss_kind,
                                             time_major=True, **self.loss_weights)

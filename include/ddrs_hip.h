@@ -319,6 +319,27 @@ ddrs_status ddrs_compress(int32_t n_edges_in, const int32_t* h_rows_in,
                           const int32_t* h_gauges, int32_t* h_active, int32_t active_cap,
                           int32_t* n_active, int32_t* h_rows, int32_t* h_cols,
                           int32_t* n_edges, int32_t* h_gauge);
+/* ddrs_compress with the rest of collate::compress (data/collate.rs:197-349):
+ * - h_conus_parent_offset (n_conus_parent + 1 entries, n_conus_parent = 0: no
+ *   map) is the reach-subdivision map in CONUS sub-reach rows; h_parent_offset
+ *   (capacity active_cap + 1) gets it in compressed rows (*n_parent parents,
+ *   *n_parent + 1 entries). A parent present only in part is an error
+ *   (compressed_parent_offset, 141-180).
+ * - outflow_idx (297-336), as CSR: gauge g's prediction sums the discharge of
+ *   compressed reaches h_outflow[h_outflow_ptr[g] .. h_outflow_ptr[g+1]]
+ *   (h_outflow_ptr has n_gauges + 1 entries; h_outflow capacity outflow_cap).
+ *   ddr_match != 0: the gauge's upstream columns in edge order, or the gauge
+ *   itself when it has none; ddr_match == 0: the gauge's own reach, at the
+ *   last piece of its parent when a map is given. */
+ddrs_status ddrs_compress_ex(int32_t n_edges_in, const int32_t* h_rows_in,
+                             const int32_t* h_cols_in, int32_t n_gauges,
+                             const int32_t* h_gauges, int32_t ddr_match,
+                             int32_t n_conus_parent, const int32_t* h_conus_parent_offset,
+                             int32_t* h_active, int32_t active_cap, int32_t* n_active,
+                             int32_t* h_rows, int32_t* h_cols, int32_t* n_edges,
+                             int32_t* h_gauge, int32_t* h_parent_offset, int32_t* n_parent,
+                             int32_t* h_outflow_ptr, int32_t* h_outflow, int32_t outflow_cap,
+                             int32_t* n_outflow);
 
 /* ---------------------------------------------------------------------------
  * KAN parameter head (device, f32): per-reach attributes -> routing parameters.

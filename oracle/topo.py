@@ -159,11 +159,54 @@ def build_topo_and_coo(records, all_comids=None):
     return order, rows, cols, []
 
 
-def compress(edges, gauges, ddr_match=False):
-    """collate::compress (data/collate.rs:197-349) without a parent map.
+def union_subgraphs(staids, subgraphs):
+    """union_subgraphs (data/collate.rs:31-48). subgraphs: {staid: (gage_idx,
+    gage_catchment, [(row, col), ...])}; a missing STAID is skipped. Returns
+    (edges sorted and deduplicated, [(staid, gage_idx, gage_catchment)])."""
+    edges, gauges = set(), []
+    for s in staids:
+        if s not in subgraphs:
+            continue
+        gi, gc, e = subgraphs[s]
+        gauges.append((s, gi, gc))
+        edges.update((int(r), int(c)) for r, c in e)
+    return sorted(edges), gauges
+
+
+def _parent_of_row(off, row):
+    """The last offset <= row (collate.rs:124-129)."""
+    p = 0
+    while p + 1 < len(off) and off[p + 1] <= row:
+        p += 1
+    return p
+
+
+def compressed_parent_offset(active, conus_off):
+    """collate.rs:141-180: the CONUS parent map renumbered onto the active set;
+    every present parent must be present whole and contiguous."""
+    n_rows = conus_off[-1] if conus_off else 0
+    out, i = [0], 0
+    while i < len(active):
+        row = active[i]
+        if row >= n_rows:
+            raise ValueError(f"compress: CONUS row {row} is outside the parent map")
+        p = _parent_of_row(conus_off, row)
+        lo, hi = conus_off[p], conus_off[p + 1]
+        m = hi - lo
+        if len(active) - i < m or any(active[i + k] != lo + k for k in range(m)):
+            raise ValueError(f"compress: parent {p} is not held contiguously")
+        i += m
+        out.append(i)
+    return out
+
+
+def compress(edges, gauges, ddr_match=False, parent_offset=None):
+    """collate::compress (data/collate.rs:197-349).
 
     edges: iterable of (row, col) in CONUS positions; gauges: list of CONUS
-    gauge positions. Returns (active, rows, cols, gauge_compressed, outflow_idx).
+    gauge positions; parent_offset: the CONUS reach-subdivision map or None.
+    Returns (active, rows, cols, gauge_compressed, outflow_idx), plus the
+    compressed parent map when parent_offset is given.
     """
     edges = sorted(set(edges))
     active = sorted({r for r, _ in edges} | {c for _, c in edges} | set(gauges))
@@ -176,11 +219,18 @@ def compress(edges, gauges, ddr_match=False):
         if r < c:
             raise ValueError("lower-triangular violated")
     gc = [m[g] for g in gauges]
+    poff = None if parent_offset is None else compressed_parent_offset(active, list(parent_offset))
     if ddr_match:
-        outflow = []
-        for g in gc:
-            ups = [c for r, c in zip(rows, cols) if r == g]
-            outflow.append(ups if ups else [g])
+        # the reference filters the edges by row for each gauge; grouping the
+        # columns by row once, in edge order, gives the same lists
+        by_row = {}
+        for r, c in zip(rows, cols):
+            by_row.setdefault(r, []).append(c)
+        outflow = [list(by_row[g]) if g in by_row else [g] for g in gc]
+    elif poff is not None:
+        outflow = [[poff[_parent_of_row(poff, g) + 1] - 1] for g in gc]
     else:
         outflow = [[g] for g in gc]
-    return active, rows, cols, gc, outflow
+    if parent_offset is None:
+        return active, rows, cols, gc, outflow
+    return active, rows, cols, gc, outflow, poff

@@ -42,6 +42,7 @@ pytestmark = pytest.mark.gpu
 if not torch.cuda.is_available():  # collected on the CPU too, skipped there
     pytest.skip("needs a GPU", allow_module_level=True)
 
+import ddrs_amd as D  # noqa: E402
 from ddrs_amd import kan as K  # noqa: E402
 from ddrs_amd import train as TR  # noqa: E402
 
@@ -62,23 +63,29 @@ def make_head(seed):
 
 
 def device_batch(b):
-    return TR.GaugedBatch(b.adj_obj, t(b.x), t(b.qprime), b.gauges, t(b.obs), t(b.sigma))
+    return TR.GaugedBatch(b.adj_obj, t(b.x), t(b.qprime), b.gauges, t(b.obs), t(b.sigma),
+                          outflow=b.outflow)
 
 
-def make_batch(seed):
+def make_batch(seed, ddr_match=False):
     from ddrs_amd.sparse import SparseAdjacency
-    b = TC.Batch(seed=seed)
+    b = TC.Batch(seed=seed, ddr_match=ddr_match)
     b.observe(TC.oracle_params(make_head(99 + seed)), 3)
     b.adj_obj = SparseAdjacency.from_coo(b.R, b.rows, b.cols, None, b.length, b.slope)
     return b
 
 
-@pytest.mark.parametrize("window", [None, 50])
-def test_micro_batch_matches_oracle_chain(window):
-    say("training micro-batch vs oracle chain, window", window)
-    b = make_batch(0)
+@pytest.mark.parametrize("window,ddr_match", [(None, False), (50, False), (None, True)])
+def test_micro_batch_matches_oracle_chain(window, ddr_match):
+    """ddr_match: the reference's DDR-faithful convention in the physics and
+    in outflow_idx (each gauge sums its upstream reaches, collate.rs:310-326),
+    the batch built by compress_ex and the loss through scatter_add_by_group."""
+    say("training micro-batch vs oracle chain, window", window, "ddr_match", ddr_match)
+    b = make_batch(0, ddr_match)
     head = make_head(42)
-    tr = TR.Trainer(head, tau=b.tau, warmup=b.warmup, window=window)
+    params = D.Params()
+    params.ddr_match = ddr_match
+    tr = TR.Trainer(head, params=params, tau=b.tau, warmup=b.warmup, window=window)
     batch = device_batch(b)
     loss, nv = tr.loss(batch)
     loss.backward()

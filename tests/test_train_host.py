@@ -140,14 +140,19 @@ def test_nested_gauges_draws_follow_targets():
     assert np.median(np.abs(np.log(ratio))) < 0.1
 
 
-def test_oracle_training_chain_matches_fp64_finite_differences():
+@pytest.mark.parametrize("ddr_match", [False, True])
+def test_oracle_training_chain_matches_fp64_finite_differences(ddr_match):
     """Pins the chain the GPU training test compares against: its analytic
     weight gradients (KAN backward of the routing adjoint of the loss
     gradient, through the denormalize slopes) agree with float64 central
-    differences of its own loss."""
+    differences of its own loss, for each gauge's own reach and for the
+    ddr_match convention (physics, and a prediction summed over the gauge's
+    upstream reaches)."""
     from ddrs_amd import kan as K
     from tests import train_case as TC
-    b = TC.Batch(n_tree=1500, gauges=8, T=120, seed=3)
+    b = TC.Batch(n_tree=1500, gauges=8, T=120, seed=3, ddr_match=ddr_match)
+    if ddr_match:   # some gauge sums several upstream reaches
+        assert np.bincount(b.outflow[2]).max() > 1
     mk = lambda s: K.KanHeadConfig([f"a{i}" for i in range(TC.F)], TC.NAMES, s).init("cpu")
     b.observe(TC.oracle_params(mk(99)), 3)
     head = mk(42)

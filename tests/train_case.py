@@ -38,13 +38,18 @@ class Batch:
     forcing. observe() adds observations from a hidden head's routed
     discharge with 10% lognormal noise and one missing day."""
 
-    def __init__(self, n_tree=4000, gauges=16, T=240, seed=0, tau=9, warmup=2):
+    def __init__(self, n_tree=4000, gauges=16, T=240, seed=0, tau=9, warmup=2, ddr_match=False):
+        """ddr_match: the reference's DDR-faithful convention, both in the
+        routing physics and in outflow_idx (a gauge's prediction sums its
+        upstream reaches, data/collate.rs:310-326)."""
         from ddrs_amd import network as NW
         from ddrs_amd import train as TR
         rows, cols = NW.gen_tree(n_tree, 64, 500 + seed)
         g = TR.nested_gauges(n_tree, rows, cols, gauges, 5, 2000, seed=600 + seed)
         ur, uc = TR.upstream_union(n_tree, rows, cols, g)
-        active, cr, cc, cg = NW.compress(ur, uc, g)
+        comp = NW.compress_ex(ur, uc, g, ddr_match=ddr_match)
+        active, cr, cc, cg = comp.active, comp.rows, comp.cols, comp.gauge_compressed
+        self.outflow = comp.scatter_map() if ddr_match else None
         R = active.size
         at = NW.gen_attributes(n_tree, 700 + seed)
         self.R, self.T, self.tau, self.warmup = R, T, tau, warmup
@@ -54,7 +59,7 @@ class Batch:
         rng = np.random.default_rng(800 + seed)
         self.x = rng.normal(size=(R, F)).astype(np.float32)
         self.qprime = NW.gen_forcing(R, T, 900 + seed, storms=True)
-        self.P = O.make_params()
+        self.P = O.make_params(ddr_match=ddr_match)
         self.pat = O.pattern_from_coo(R, cr, cc)
         self.obs = self.sigma = None
 
@@ -74,11 +79,24 @@ class Batch:
         q0 = O.hotstart(self.P, self.pat, self.qprime[0], dtype=dtype)
         Q = O.route_fwd(self.P, self.pat, n, qsp, pp, self.length, self.slope, self.qprime,
                         q0=q0, dtype=dtype)
-        return Q[:, self.gauges], (n, qsp, pp, q0, cache)
+        return Q[:, self.readout()], (n, qsp, pp, q0, cache)
+
+    def readout(self):
+        """The reaches whose discharge forms the gauges' predictions."""
+        return self.gauges if self.outflow is None else self.outflow[0]
+
+    def scatter(self):
+        """(flat, group) of scatter_add_by_group over the readout reaches."""
+        if self.outflow is None:
+            G = self.gauges.size
+            return np.arange(G), np.arange(G)
+        return self.outflow[1], self.outflow[2]
 
     def observe(self, p_truth, k):
         Qg, _ = self.oracle_Qg(p_truth, k)
-        daily = lo.tau_trim_and_downsample(Qg.T, self.tau)
+        flat, grp = self.scatter()
+        hourly = lo.scatter_add_by_group(Qg.T, flat, grp, self.gauges.size)
+        daily = lo.tau_trim_and_downsample(hourly, self.tau)
         rng = np.random.default_rng(1)
         obs = (daily * np.exp(0.1 * rng.normal(size=daily.shape))).astype(np.float32)
         obs[3, self.warmup + 1] = np.nan
@@ -87,8 +105,8 @@ class Batch:
 
     def oracle_loss(self, p, k, eps=0.1, dtype=np.float32):
         Qg, _ = self.oracle_Qg(p, k, dtype)
-        G = self.gauges.size
-        loss, nv, _ = lo.gauge_nse_loss(Qg.T, np.arange(G), np.arange(G), G, self.tau,
+        flat, grp = self.scatter()
+        loss, nv, _ = lo.gauge_nse_loss(Qg.T, flat, grp, self.gauges.size, self.tau,
                                         self.warmup, self.obs, self.sigma, eps)
         return loss, nv
 
@@ -96,11 +114,11 @@ class Batch:
         """(loss, n_valid, {name: dL/dweight}) of the oracle chain (nq as in
         oracle_Qg)."""
         Qg, (n, qsp, pp, q0, cache) = self.oracle_Qg(p, k, dtype, nq)
-        G = self.gauges.size
-        loss, nv, g_run = lo.gauge_nse_loss(Qg.T, np.arange(G), np.arange(G), G, self.tau,
+        flat, grp = self.scatter()
+        loss, nv, g_run = lo.gauge_nse_loss(Qg.T, flat, grp, self.gauges.size, self.tau,
                                             self.warmup, self.obs, self.sigma, eps)
         gQ = np.zeros((self.T, self.R), dtype)
-        gQ[:, self.gauges] = g_run.T
+        gQ[:, self.readout()] = g_run.T
         ga = O.route_adj(self.P, self.pat, n, qsp, pp, self.length, self.slope, self.qprime, gQ,
                          q0=q0, dtype=dtype)
         gout = np.stack([ga["n"].astype(np.float64) * (N_RANGE[1] - N_RANGE[0]),
