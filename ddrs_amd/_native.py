@@ -41,6 +41,7 @@ class DdrsParams(ctypes.Structure):
 FORM_PLAIN = 0
 FORM_PERSISTENT = 1
 WS_NO_DENSE_GRAD = 1
+WS_SCHEDULE_FORCING = 2
 LOSS_L1, LOSS_NSE_BATCH, LOSS_NNSE_KGE, LOSS_KGE = 0, 1, 2, 3
 
 
@@ -61,6 +62,8 @@ _SIGS = {
     "ddrs_plan_info": (c_int, [P, P, P, P, P]),
     "ddrs_plan_get_csr": (c_int, [P, P, P, P, P, P, P, P, P]),
     "ddrs_plan_get_schedule": (c_int, [P, P, P]),
+    "ddrs_plan_positions": (c_int, [P, P]),
+    "ddrs_plan_permute_forcing": (c_int, [P, c_int, P, P, P]),
     "ddrs_workspace_create": (c_int, [P, c_int, c_int, ctypes.POINTER(P)]),
     "ddrs_workspace_create_ex": (c_int, [P, c_int, c_int, c_int, ctypes.POINTER(P)]),
     "ddrs_workspace_destroy": (None, [P]),

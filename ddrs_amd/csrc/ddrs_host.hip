@@ -92,7 +92,11 @@ struct ddrs_workspace {
     unsigned long long* claim_ctr = nullptr;  // chunk claims of persistent launches
     LaunchState* ls = nullptr;                // claim base and epoch of the current launch
     unsigned long long claims = 0;            // claims made so far
-    float* qps = nullptr;                    // [max_steps][P] q' in schedule order
+    float* qps = nullptr;                    // [max_steps][P] q' in schedule order (one row
+                                             // with DDRS_WS_SCHEDULE_FORCING)
+    const float* qwin = nullptr;             // q' rows the kernels read (qps, or the caller's
+                                             // schedule-order rows)
+    bool sched_forcing = false;              // window calls take q' in schedule order
     float* gqs = nullptr;                    // [max_steps][P] dL/dQ in schedule order
     const float* qps_src = nullptr;          // q' it was permuted from (rows qps_rows)
     int qps_rows = 0;
@@ -594,10 +598,15 @@ ddrs_status plan_to_device(ddrs_plan* pl) {
     return DDRS_OK;
 }
 
-static ddrs_status check_ws_args(ddrs_plan_t plan, int32_t window, int32_t max_steps) {
+static ddrs_status check_ws_args(ddrs_plan_t plan, int32_t window, int32_t max_steps,
+                                 int32_t flags = 0) {
     if (!plan) return fail(DDRS_ERR_INVALID, "plan is NULL");
+    if (flags & ~(DDRS_WS_NO_DENSE_GRAD | DDRS_WS_SCHEDULE_FORCING))
+        return fail(DDRS_ERR_INVALID, "unknown workspace flags");
     if (window < 1 || max_steps < 0) return fail(DDRS_ERR_INVALID, "window must be >= 1");
     if (window > 65535) return fail(DDRS_ERR_INVALID, "window must be <= 65535");
+    // the sweep addresses one step's 16-byte (P, G) pairs with 32-bit byte offsets
+    if (plan->P > (1 << 26)) return fail(DDRS_ERR_INVALID, "plans over 2^26 positions are not supported");
     return DDRS_OK;
 }
 
@@ -621,7 +630,7 @@ static void ws_buffers(ddrs_workspace* ws, int32_t flags, F&& f) {
     f(&ws->amsg, WR * 16);
     f(&ws->rec, WR * sizeof(float4));
     f(&ws->recj, WR * sizeof(float4));
-    f(&ws->qps, steps * R * 4);
+    f(&ws->qps, ((flags & DDRS_WS_SCHEDULE_FORCING) ? 1 : steps) * R * 4);
     if (!(flags & DDRS_WS_NO_DENSE_GRAD)) f(&ws->gqs, steps * R * 4);
     f(&ws->gbuf, WR * 4);
     f(&ws->slot, R * 8);
@@ -643,13 +652,35 @@ static void ws_shape(ddrs_workspace* ws, ddrs_plan_t plan, int32_t window, int32
 extern "C" {
 
 // ---------------------------------------------------------------------------
+// forcing in schedule order
+
+ddrs_status ddrs_plan_positions(ddrs_plan_t plan, int32_t* P) {
+    if (!plan) return fail(DDRS_ERR_INVALID, "plan is NULL");
+    if (!P) return fail(DDRS_ERR_INVALID, "P is NULL");
+    *P = plan->P;
+    return DDRS_OK;
+}
+
+ddrs_status ddrs_plan_permute_forcing(ddrs_plan_t plan, int32_t rows, const float* d_q_prime,
+                                      float* d_q_sched, void* stream) {
+    if (!plan) return fail(DDRS_ERR_INVALID, "plan is NULL");
+    if (rows < 0) return fail(DDRS_ERR_INVALID, "rows must be >= 0");
+    if (rows > 0 && (!d_q_prime || !d_q_sched)) return fail(DDRS_ERR_INVALID, "NULL input/output");
+    ddrs_status st = plan_to_device(plan);
+    if (st != DDRS_OK) return st;
+    launch_permute_rows(plan->sc, d_q_prime, 0, rows, d_q_sched, S(stream));
+    HIPCHK(hipGetLastError());
+    return DDRS_OK;
+}
+
+// ---------------------------------------------------------------------------
 // workspace
 
 ddrs_status ddrs_workspace_query_bytes(ddrs_plan_t plan, int32_t window, int32_t max_steps,
                                        int32_t flags, size_t* bytes) {
     if (!bytes) return fail(DDRS_ERR_INVALID, "bytes is NULL");
     *bytes = 0;
-    ddrs_status st = check_ws_args(plan, window, max_steps);
+    ddrs_status st = check_ws_args(plan, window, max_steps, flags);
     if (st != DDRS_OK) return st;
     ddrs_workspace tmp;
     ws_shape(&tmp, plan, window, max_steps);
@@ -663,7 +694,7 @@ ddrs_status ddrs_workspace_create_ex(ddrs_plan_t plan, int32_t window, int32_t m
                                      int32_t flags, ddrs_workspace_t* out) {
     if (!out) return fail(DDRS_ERR_INVALID, "out is NULL");
     *out = nullptr;
-    ddrs_status st = check_ws_args(plan, window, max_steps);
+    ddrs_status st = check_ws_args(plan, window, max_steps, flags);
     if (st != DDRS_OK) return st;
     ddrs_status pst = plan_to_device(plan);
     if (pst != DDRS_OK) return pst;
@@ -678,6 +709,8 @@ ddrs_status ddrs_workspace_create_ex(ddrs_plan_t plan, int32_t window, int32_t m
         ddrs_workspace_destroy(ws);
         return st;
     }
+    ws->sched_forcing = (flags & DDRS_WS_SCHEDULE_FORCING) != 0;
+    ws->qwin = ws->qps;
     // tags start at epoch 0 = "never written"
     hipError_t e = hipMemset(ws->hist, 0, WR * 8);
     if (e == hipSuccess) e = hipMemset(ws->amsg, 0, WR * 16);
@@ -785,7 +818,7 @@ static FwdArgs fwd_args(ddrs_workspace* ws, const ddrs_params* p) {
     a.P = to_phys(p);
     a.hist = ws->hist;
     a.ihist = ws->ihist;
-    a.qps = ws->qps;
+    a.qps = ws->qwin;
     a.st_q = ws->st_q;
     a.st_i = ws->st_i;
     a.err = ws->err;
@@ -794,12 +827,27 @@ static FwdArgs fwd_args(ddrs_workspace* ws, const ddrs_params* p) {
 
 // q' rows [0, rows) of `q` in schedule order, unless already there.
 // Returns whether it permuted.
+// With DDRS_WS_SCHEDULE_FORCING, q is in schedule order already and is read
+// in place. Returns whether the q' the kernels read changed.
 static bool permute_qprime(ddrs_workspace* ws, const float* q, int rows, hipStream_t s) {
+    if (ws->sched_forcing) {
+        const bool changed = q != ws->qwin;
+        ws->qwin = q;
+        return changed;
+    }
+    ws->qwin = ws->qps;
     if (q == ws->qps_src && rows <= ws->qps_rows) return false;
     launch_permute_rows(ws->plan->sc, q, 0, rows, ws->qps, s);
     ws->qps_src = q;
     ws->qps_rows = rows;
     return true;
+}
+
+// One row of q' (single-step calls, reach order always) into qps.
+static void permute_qprime_row(ddrs_workspace* ws, const float* q, hipStream_t s) {
+    launch_permute_rows(ws->plan->sc, q, 0, 1, ws->qps, s);
+    ws->qps_src = nullptr;
+    ws->qwin = ws->qps;
 }
 
 // Begins the next persistent launch of `grid` workgroups on stream s: a
@@ -830,7 +878,7 @@ static LinArgs lin_args(ddrs_workspace* ws, const ddrs_params* p, int t0, int ma
     L.st = ws->st;
     L.P = to_phys(p);
     L.t0 = t0;
-    L.qps = ws->qps;
+    L.qps = ws->qwin;
     L.mask_qp = mask_qp;
     L.mask_lam = mask_lam;
     L.hist = ws->hist;
@@ -926,8 +974,7 @@ ddrs_status ddrs_mc_step_fwd(ddrs_workspace_t ws, const ddrs_params* params, con
     if (!d_q_t || !d_q_prime_t || !d_q_next) return fail(DDRS_ERR_INVALID, "NULL input/output");
     hipStream_t s = S(stream);
     launch_state(ws->plan->sc, d_q_t, 0, params->discharge_lb, ws->st_q, ws->st_i, nullptr, s);
-    launch_permute_rows(ws->plan->sc, d_q_prime_t, 0, 1, ws->qps, s);
-    ws->qps_src = nullptr;
+    permute_qprime_row(ws, d_q_prime_t, s);
     FwdArgs a = fwd_args(ws, params);
     a.t0 = 0;
     a.s_base = 0;
@@ -957,8 +1004,7 @@ ddrs_status ddrs_mc_step_bwd(ddrs_workspace_t ws, const ddrs_params* params, con
     const Sched& sc = ws->plan->sc;
     launch_state(sc, d_q_t, 0, params->discharge_lb, ws->st_q, ws->st_i, nullptr, s);
     launch_hist_from_x(sc, d_x_sol, ws->hist, s);
-    launch_permute_rows(sc, d_q_prime_t, 0, 1, ws->qps, s);
-    ws->qps_src = nullptr;
+    permute_qprime_row(ws, d_q_prime_t, s);
     launch_lam_init(sc, d_grad_q_next, ws->hist, params->discharge_lb, ws->st_lam, ws->acc_n,
                     ws->acc_qsp, ws->acc_p, s);
     AdjSegment g;

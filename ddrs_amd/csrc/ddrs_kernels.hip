@@ -34,6 +34,8 @@
 
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "mc_physics.h"
 #include "ddrs_epoch.h"
 #include "ddrs_internal.h"
@@ -60,6 +62,30 @@ __device__ __forceinline__ bool word_ready(uint64_t w, uint32_t epoch) {
     return static_cast<uint32_t>(w >> 32) == epoch;
 }
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Pairs of tagged words: two words of 16 bytes, 16-byte aligned, moved by one
+// vector access with the cache policy of the agent-scope atomics (sc1, device
+// coherent). Each 8-byte half keeps its own tag, so a reader that takes a pair
+// only when both tags are current never mixes two publications, even if the
+// 16 bytes were moved as two 8-byte halves. The buffer resource covers one
+// step's row of pairs (uniform, in scalar registers); the offset is in bytes.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kCoherentPolicy = 16;  // sc1
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pair_row(const uint64_t* row, int npairs) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint64_t*>(row), 0, npairs * 16,
+                                             0x00020000);
+}
+__device__ __forceinline__ void publish_pair(__amdgpu_buffer_rsrc_t row, int i, float a, float b,
+                                             uint32_t epoch) {
+    const u32x4 v = {__float_as_uint(a), epoch, __float_as_uint(b), epoch};
+    __builtin_amdgcn_raw_buffer_store_b128(v, row, i * 16, 0, kCoherentPolicy);
+}
+__device__ __forceinline__ u32x4 poll_pair(__amdgpu_buffer_rsrc_t row, int i) {
+    return __builtin_amdgcn_raw_buffer_load_b128(row, i * 16, 0, kCoherentPolicy);
+}
+__device__ __forceinline__ bool pair_ready(const u32x4& w, uint32_t epoch) {
+    return w.y == epoch && w.w == epoch;
+}
 
 // Marks a loaded value as used here. The compiler waits for a load at the
 // first use of its register, and the wave's memory counter cannot single out
@@ -752,7 +778,10 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
     // The records go from memory straight to LDS: a register that carries a
     // load from one step to the next makes the compiler wait for everything
     // in flight (the previous step's stores included) before reusing it.
-    uint64_t pwP = 0, pwG = 0;  // P and G words of link `lane`
+    u32x4 pw = {0, 0, 0, 0};  // P and G words of link `lane`
+    // rows r of the chunk (positions r * 64 ...) holding a gauge: the others
+    // read no gauge-sparse dL/dQ
+    unsigned grows = 0;
 
     // Step-invariant data only: the per-step work of the sweep is in issue
     // and solve.
@@ -762,7 +791,11 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
         if (lane < nrep) srep[lane] = a.ch.rep_dn[rep0 + lane];
         if (a.gK) {
 #pragma unroll
-            for (int r = 0; r < RPL; ++r) pgs[r * 64 + lane] = a.gslot[p0 + r * 64 + lane];
+            for (int r = 0; r < RPL; ++r) {
+                const int sl = a.gslot[p0 + r * 64 + lane];
+                pgs[r * 64 + lane] = sl;
+                if (__any(sl >= 0)) grows |= 1u << r;
+            }
         }
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
@@ -799,9 +832,10 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
             for (int r = 0; r < RPL; ++r) {
                 const size_t p = static_cast<size_t>(p0 + r * 64 + lane);
                 load_to_lds(a.rec + row + p, &prec[r * 64], 16);
-                if (a.gqs) {
+                if (a.gqs && (!a.gK || ((grows >> r) & 1u))) {
                     // dense rows of P, or gauge rows of gK read at the position's slot
-                    // (every lane loads: slot 0 stands in where there is no gauge)
+                    // (every lane of a row with a gauge loads: slot 0 stands in
+                    // where there is no gauge)
                     const size_t idx = a.gK ? static_cast<size_t>(max(pgs[r * 64 + lane], 0)) : p;
                     load_to_lds(a.gqs + static_cast<size_t>(a.t0 + a.s_base + s) *
                                             static_cast<size_t>(a.gK ? a.gK : P) + idx,
@@ -809,46 +843,39 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
                 }
             }
         }
-        const uint64_t* mrow = a.amsg + row * 2;
         const int d = (lane < nlinks && skind[lane] < 0) ? spos[lane] : 0;
-        pwP = poll(mrow + 2 * d);
-        pwG = poll(mrow + 2 * d + 1);
+        pw = poll_pair(pair_row(a.amsg + row * 2, P), d);
     };
     auto check = [&](int s) -> bool {
-        const uint64_t* mrow = a.amsg + static_cast<size_t>(a.s_base + s) * P * 2;
+        const auto mrow = pair_row(a.amsg + static_cast<size_t>(a.s_base + s) * P * 2, P);
         bool ok = true;
         if (lane < nlinks && skind[lane] < 0) {
-            if (!word_ready(pwP, epoch) || !word_ready(pwG, epoch)) {
+            if (!pair_ready(pw, epoch)) {
                 ok = false;
             } else {
-                stP[lane] = word_value(pwP);
-                stG[lane] = word_value(pwG);
+                stP[lane] = __uint_as_float(pw.x);
+                stG[lane] = __uint_as_float(pw.z);
             }
         }
         // links beyond the first slot: all loads first, then the checks
         if (nlinks > 64) {
             constexpr int kMore = Wd::kStageDn / 64 - 1;
-            uint64_t wP[kMore], wG[kMore];
+            u32x4 w[kMore];
 #pragma unroll
             for (int k = 0; k < kMore; ++k) {
                 const int e = lane + 64 * (k + 1);
-                wP[k] = 0;
-                wG[k] = 0;
-                if (e < nlinks && skind[e] < 0) {
-                    const int d = spos[e];
-                    wP[k] = poll(mrow + 2 * d);
-                    wG[k] = poll(mrow + 2 * d + 1);
-                }
+                w[k] = u32x4{0, 0, 0, 0};
+                if (e < nlinks && skind[e] < 0) w[k] = poll_pair(mrow, spos[e]);
             }
 #pragma unroll
             for (int k = 0; k < kMore; ++k) {
                 const int e = lane + 64 * (k + 1);
                 if (e < nlinks && skind[e] < 0) {
-                    if (!word_ready(wP[k], epoch) || !word_ready(wG[k], epoch)) {
+                    if (!pair_ready(w[k], epoch)) {
                         ok = false;
                     } else {
-                        stP[e] = word_value(wP[k]);
-                        stG[e] = word_value(wG[k]);
+                        stP[e] = __uint_as_float(w[k].x);
+                        stG[e] = __uint_as_float(w[k].z);
                     }
                 }
             }
@@ -858,7 +885,7 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
         return all;
     };
 
-    // the G word of a link is published right after its P word
+    // the G word of a link is published with its P word, in one access
     auto probe = [&](int s) -> bool {
         if (nrep <= 0) return true;  // no representatives: read the links again
         const uint64_t* row = a.amsg + static_cast<size_t>(a.s_base + s) * P * 2;
@@ -869,7 +896,7 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
 
     auto solve = [&](int s, bool last, auto&& before_stores) {
         const int sg = a.s_base + s;
-        uint64_t* mrow = a.amsg + static_cast<size_t>(sg) * P * 2;
+        const auto mrow = pair_row(a.amsg + static_cast<size_t>(sg) * P * 2, P);
         // the step's records arrived in LDS before the inputs that were just
         // waited for (loads complete in order)
         {
@@ -946,8 +973,7 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
         for (int r = 0; r < RPL; ++r) {
             if (my[r] >= 0) {
                 const int j = r * 64 + lane;
-                publish(mrow + 2 * (p0 + j), ldsP[j], epoch);
-                publish(mrow + 2 * (p0 + j) + 1, ldsG[j], epoch);
+                publish_pair(mrow, p0 + j, ldsP[j], ldsG[j], epoch);
             }
         }
 #pragma unroll
@@ -1188,14 +1214,23 @@ __global__ void k_grad_q0(Sched sc, const float* lam, const float* q0, float lb,
     if (o >= 0) g[o] = (q0[o] > lb) ? lam[p] : 0.0f;
 }
 
-__global__ void k_permute_rows(Sched sc, const float* q, int t_first, int rows, float* out) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= sc.P) return;
+// Rows [t_first, t_first + rows) of q ([..][R], reach order) into schedule
+// order (out [rows][P], 0 at padding positions). Workgroups are dealt to the
+// 8 XCDs in turn (workgroup b on XCD b % 8), and each XCD takes whole rows
+// (row k on XCD k % 8, its workgroups one after another): the gathers of a
+// row then hit the one L2 that holds its lines (1.4 MB for 350k reaches)
+// instead of every XCD's L2 fetching every row.
+constexpr int kXcds = 8;
+__global__ __launch_bounds__(256) void k_permute_rows(Sched sc, const float* q, int t_first,
+                                                      int rows, int row_blocks, float* out) {
+    const int x = static_cast<int>(blockIdx.x % kXcds);
+    const int local = static_cast<int>(blockIdx.x / kXcds);
+    const int k = (local / row_blocks) * kXcds + x;
+    const int p = (local % row_blocks) * 256 + static_cast<int>(threadIdx.x);
+    if (k >= rows || p >= sc.P) return;
     const int o = sc.perm[p];
-    for (int k = blockIdx.y; k < rows; k += gridDim.y) {
-        out[static_cast<size_t>(k) * sc.P + p] =
-            (o >= 0) ? q[static_cast<size_t>(t_first + k) * sc.R + o] : 0.0f;
-    }
+    out[static_cast<size_t>(k) * sc.P + p] =
+        (o >= 0) ? q[static_cast<size_t>(t_first + k) * sc.R + o] : 0.0f;
 }
 
 __global__ void k_gauge_row0(const int* reach, int K, const float* q0, float lb, float* Qg) {
@@ -1320,10 +1355,15 @@ void launch_grad_q0(const Sched& sc, const float* lam, const float* q0, float lb
 }
 void launch_permute_rows(const Sched& sc, const float* q, int t_first, int rows, float* out,
                          hipStream_t s) {
-    if (rows <= 0) return;
-    const dim3 grid(static_cast<unsigned>((sc.P + 255) / 256),
-                    static_cast<unsigned>(rows < 1024 ? rows : 1024));
-    hipLaunchKernelGGL(k_permute_rows, grid, dim3(256), 0, s, sc, q, t_first, rows, out);
+    const int row_blocks = (sc.P + 255) / 256;
+    // launches of at most 2^30 workgroups, in whole turns of kXcds rows
+    const int per = std::max(1, static_cast<int>((1 << 30) / (static_cast<size_t>(row_blocks) * kXcds))) * kXcds;
+    for (int k0 = 0; k0 < rows; k0 += per) {
+        const int n = std::min(per, rows - k0);
+        const size_t blocks = static_cast<size_t>((n + kXcds - 1) / kXcds) * kXcds * row_blocks;
+        hipLaunchKernelGGL(k_permute_rows, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, sc,
+                           q, t_first + k0, n, row_blocks, out + static_cast<size_t>(k0) * sc.P);
+    }
 }
 void launch_zero(size_t n, float* x, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_zero_f, g1(n), dim3(256), 0, s, n, x);

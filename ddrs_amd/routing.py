@@ -178,11 +178,13 @@ def route_window(net: Network, params: Params, n, q_spatial, p_spatial, q_prime,
 class _WindowGauged(torch.autograd.Function):
     @staticmethod
     def forward(ctx, n, qsp, p, q_prime, q0, net, params, length, slope, xst, form, window,
-                gauges):
+                gauges, q_sched):
         T = int(q_prime.shape[0])
-        ws = _gauge_workspace(net, window, max(T - 1, 1), gauges)
+        sched = q_sched is not None
+        ws = _gauge_workspace(net, window, max(T - 1, 1), gauges, sched)
         cp, keep, xk = _bind(ws, params, n, qsp, p, length, slope, xst)
-        qp = _dev_f32(q_prime, T * net.n)
+        # the rows the kernels read: q' itself, or its schedule-order layout
+        qp = _dev_f32(q_sched, T * net.positions) if sched else _dev_f32(q_prime, T * net.n)
         q0d = _dev_f32(q0, net.n)
         K = int(ws.gauges.size)
         Qg = torch.empty((T, K), dtype=torch.float32, device=qp.device)
@@ -192,7 +194,7 @@ class _WindowGauged(torch.autograd.Function):
         token = next(_tokens)
         ws.fwd_token = token
         ctx.token, ctx.net, ctx.params, ctx.form, ctx.window, ctx.T = token, net, params, form, window, T
-        ctx.gauges = ws.gauges
+        ctx.gauges, ctx.sched = ws.gauges, sched
         ctx.has_x = xk is not None
         ctx.save_for_backward(*keep, qp, q0d, *([xk] if xk is not None else []))
         return Qg
@@ -203,7 +205,7 @@ class _WindowGauged(torch.autograd.Function):
         n, qsp, p, length, slope, qp, q0d = saved[:7]
         xst = saved[7] if ctx.has_x else None
         net, T = ctx.net, ctx.T
-        ws = _gauge_workspace(net, ctx.window, max(T - 1, 1), ctx.gauges)
+        ws = _gauge_workspace(net, ctx.window, max(T - 1, 1), ctx.gauges, ctx.sched)
         cp, _, _ = _bind(ws, ctx.params, n, qsp, p, length, slope, xst)
         if ws.fwd_token != ctx.token:
             N.check(N.lib.ddrs_mc_window_fwd(ws.handle, cp, T, _ptr(qp), _ptr(q0d), None,
@@ -221,25 +223,28 @@ class _WindowGauged(torch.autograd.Function):
                                                 _ptr(gq0), ctx.form, _stream()),
                 "ddrs_mc_window_adj_gauged")
         ws.check()
-        return gn, gqsp, gp, gqp, gq0, None, None, None, None, None, None, None, None
+        return gn, gqsp, gp, gqp, gq0, None, None, None, None, None, None, None, None, None
 
 
-def _gauge_workspace(net: Network, window: int, max_steps: int, gauges) -> "Workspace":
+def _gauge_workspace(net: Network, window: int, max_steps: int, gauges,
+                     schedule_forcing: bool = False) -> "Workspace":
     """A workspace of `net` without the dense dL/dQ buffer, its gauges set to
-    `gauges` (kept while unchanged). One that holds at least `window` and
-    `max_steps` is reused (a longer checkpoint window gives the same results);
-    when none does, the smaller ones are freed before the new one is made, so
-    that windows of growing length do not pile up device memory. A backward
-    whose forward ran on a freed workspace recomputes its forward."""
+    `gauges` (kept while unchanged), taking q' in reach or (schedule_forcing)
+    schedule order. One that holds at least `window` and `max_steps` is reused
+    (a longer checkpoint window gives the same results); when none does, the
+    smaller ones are freed before the new one is made, so that windows of
+    growing length do not pile up device memory. A backward whose forward ran
+    on a freed workspace recomputes its forward."""
     from .sparse import Workspace
-    window, max_steps = int(window), int(max_steps)
+    window, max_steps, sf = int(window), int(max_steps), bool(schedule_forcing)
     gauged = [k for k in net._ws if k[0] == "gauged"]
-    ws = next((net._ws[k] for k in gauged if k[1] >= window and k[2] >= max_steps), None)
+    ws = next((net._ws[k] for k in gauged
+               if k[1] >= window and k[2] >= max_steps and k[3] == sf), None)
     if ws is None:
         for k in gauged:
             net._ws.pop(k).close()
-        ws = Workspace(net, window, max_steps, dense_grad=False)
-        net._ws[("gauged", window, max_steps)] = ws
+        ws = Workspace(net, window, max_steps, dense_grad=False, schedule_forcing=sf)
+        net._ws[("gauged", window, max_steps, sf)] = ws
     g = np.ascontiguousarray(gauges, np.int32)
     if getattr(ws, "gauges", None) is None or not np.array_equal(ws.gauges, g):
         ws.set_gauges(g)
@@ -248,16 +253,22 @@ def _gauge_workspace(net: Network, window: int, max_steps: int, gauges) -> "Work
 
 def route_window_gauged(net: Network, params: Params, n, q_spatial, p_spatial, q_prime, q0,
                         length, slope, gauges, x_storage=None, form: int = N.FORM_PERSISTENT,
-                        window: Optional[int] = None) -> torch.Tensor:
+                        window: Optional[int] = None,
+                        q_sched: Optional[torch.Tensor] = None) -> torch.Tensor:
     """Q at the gauge reaches only, [T, K] with K = len(gauges) (reference
     reach ids): the same values as route_window(...)[:, gauges], computed
     without writing the full [T, R] discharge; its backward gives the same
     gradients as route_window's for a loss that reads Q only at the gauges
-    (dL/dq' and dL/dq0 only when those inputs require gradients)."""
+    (dL/dq' and dL/dq0 only when those inputs require gradients). q_sched:
+    q' already in schedule order (net.permute_forcing(q_prime), laid out once
+    per batch); the window then reads it in place and permutes nothing, with
+    bit-identical results."""
     T = int(q_prime.shape[0])
     window = max(T - 1, 1) if window is None else int(window)
+    if q_sched is not None and tuple(q_sched.shape) != (T, net.positions):
+        raise ValueError(f"q_sched must be [{T}, {net.positions}], got {tuple(q_sched.shape)}")
     return _WindowGauged.apply(n, q_spatial, p_spatial, q_prime, q0, net, params, length, slope,
-                               x_storage, form, window, gauges)
+                               x_storage, form, window, gauges, q_sched)
 
 
 @dataclass

@@ -83,6 +83,9 @@ class Network:
         info = [ctypes.c_int32() for _ in range(4)]
         N.check(N.lib.ddrs_plan_info(h, *[ctypes.byref(x) for x in info]))
         self.n, self.nnz, self.n_chunks, self.n_levels = (int(x.value) for x in info)
+        npos = ctypes.c_int32()
+        N.check(N.lib.ddrs_plan_positions(h, ctypes.byref(npos)))
+        self.positions = int(npos.value)
         self._ws = {}
 
     def __del__(self):
@@ -107,11 +110,26 @@ class Network:
         N.check(N.lib.ddrs_plan_get_schedule(self.handle, _ptr(perm), _ptr(level)))
         return perm, level
 
-    def workspace_bytes(self, window: int, max_steps: int, dense_grad: bool = True) -> int:
-        """Device bytes a Workspace(self, window, max_steps, dense_grad) would
-        hold (without its gauge buffers); allocates nothing."""
+    def permute_forcing(self, q_prime: torch.Tensor) -> torch.Tensor:
+        """q' [rows, n] (reach order, float32 on the GPU) in schedule order,
+        [rows, positions] (ddrs_plan_permute_forcing), for workspaces made with
+        schedule_forcing=True: the data side lays out a batch's forcing once."""
+        q = _dev_f32(q_prime, None)
+        if q.dim() != 2 or q.shape[1] != self.n:
+            raise ValueError(f"q_prime must be [rows, {self.n}], got {tuple(q.shape)}")
+        out = torch.empty((q.shape[0], self.positions), dtype=torch.float32, device=q.device)
+        N.check(N.lib.ddrs_plan_permute_forcing(self.handle, int(q.shape[0]), q.data_ptr(),
+                                                out.data_ptr(), _stream()),
+                "ddrs_plan_permute_forcing")
+        return out
+
+    def workspace_bytes(self, window: int, max_steps: int, dense_grad: bool = True,
+                        schedule_forcing: bool = False) -> int:
+        """Device bytes a Workspace(self, window, max_steps, dense_grad,
+        schedule_forcing) would hold (without its gauge buffers); allocates
+        nothing."""
         b = N.c_size()
-        flags = 0 if dense_grad else N.WS_NO_DENSE_GRAD
+        flags = _ws_flags(dense_grad, schedule_forcing)
         N.check(N.lib.ddrs_workspace_query_bytes(self.handle, int(window), int(max_steps), flags,
                                                  ctypes.byref(b)), "ddrs_workspace_query_bytes")
         return int(b.value)
@@ -127,15 +145,24 @@ class Network:
         return ws
 
 
+def _ws_flags(dense_grad: bool, schedule_forcing: bool) -> int:
+    return ((0 if dense_grad else N.WS_NO_DENSE_GRAD)
+            | (N.WS_SCHEDULE_FORCING if schedule_forcing else 0))
+
+
 class Workspace:
-    def __init__(self, net: Network, window: int, max_steps: int, dense_grad: bool = True):
+    def __init__(self, net: Network, window: int, max_steps: int, dense_grad: bool = True,
+                 schedule_forcing: bool = False):
         """dense_grad=False leaves out the [max_steps][n] dense dL/dQ buffer
-        (only the gauged adjoint works then)."""
+        (only the gauged adjoint works then). schedule_forcing=True: the window
+        calls take q' in schedule order, [T, positions] from
+        Network.permute_forcing, and read it in place."""
         self.net = net
         self.window = window
         self.max_steps = max_steps
+        self.schedule_forcing = bool(schedule_forcing)
         h = ctypes.c_void_p()
-        flags = 0 if dense_grad else N.WS_NO_DENSE_GRAD
+        flags = _ws_flags(dense_grad, schedule_forcing)
         N.check(N.lib.ddrs_workspace_create_ex(net.handle, window, max_steps, flags,
                                                ctypes.byref(h)), "ddrs_workspace_create_ex")
         self.handle = h

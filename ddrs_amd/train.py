@@ -19,7 +19,7 @@ torch tensor arithmetic on the device.
 from __future__ import annotations
 
 import math
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 from typing import Dict, Iterable, List, Optional
 
 import numpy as np
@@ -146,7 +146,12 @@ class GaugedBatch:
     outflow       the batch's outflow_idx (data/collate.rs:297-336) as
                   Compressed.scatter_map() gives it: (reaches read out,
                   flat indices into them, gauge of each entry). None: each
-                  gauge's prediction is its own reach."""
+                  gauge's prediction is its own reach.
+
+    The trainer lays q_prime out in the device schedule's order once per
+    batch and network (forcing_in_schedule_order) and keeps that copy with the
+    batch: a batch whose q_prime is rewritten in place must reset it
+    (q_sched = None)."""
 
     adjacency: SparseAdjacency
     attributes: torch.Tensor
@@ -155,6 +160,14 @@ class GaugedBatch:
     observations: torch.Tensor
     gauge_obs_std: Optional[torch.Tensor]
     outflow: Optional[tuple] = None
+    q_sched: Optional[tuple] = field(default=None, repr=False, compare=False)
+
+    def forcing_in_schedule_order(self, net: Network) -> torch.Tensor:
+        """q_prime in the schedule order of `net` ([T, net.positions]), made
+        on the first call for that network and kept."""
+        if self.q_sched is None or self.q_sched[0] is not net:
+            self.q_sched = (net, net.permute_forcing(self.q_prime.detach()))
+        return self.q_sched[1]
 
     def readout(self) -> np.ndarray:
         """The reaches whose discharge the routing window reads out."""
@@ -257,7 +270,8 @@ class Trainer:
         q0 = hotstart(net, P, batch.q_prime[0].detach())
         Qg = route_window_gauged(net, P, n, qsp, p, batch.q_prime.detach(), q0.detach(),
                                  length, slope, batch.readout(), x_storage=x, form=self.form,
-                                 window=self.window)
+                                 window=self.window,
+                                 q_sched=batch.forcing_in_schedule_order(net))
         if trace is not None:
             for v in (n, qsp, Qg):
                 if v.requires_grad:

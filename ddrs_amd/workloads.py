@@ -203,10 +203,13 @@ class GaugedWindow:
     producing dL/dn and dL/dq_spatial for every reach. The workspace holds no
     dense dL/dQ buffer. window: checkpoint segment (None: the whole window).
     n, q_spatial: device tensors to route with instead of the workload's
-    attributes (e.g. a KAN head's denormalised outputs)."""
+    attributes (e.g. a KAN head's denormalised outputs). schedule_forcing:
+    lay q' out in schedule order once here, as the data side of a batch
+    would, so that the window calls permute nothing (False: the calls take
+    q' in reach order and permute it themselves)."""
 
     def __init__(self, net, wl: Workload, qprime, gauges=None, window: Optional[int] = None,
-                 params=None, n=None, q_spatial=None):
+                 params=None, n=None, q_spatial=None, schedule_forcing: bool = True):
         import torch
         from . import _native as NAT
         from .config import Params
@@ -218,7 +221,10 @@ class GaugedWindow:
         self.params = params if params is not None else Params()
         self.cp = self.params.to_c()
         self.window = window or (self.T - 1)
-        self.ws = Workspace(net, self.window, self.T - 1, dense_grad=False)
+        self.ws = Workspace(net, self.window, self.T - 1, dense_grad=False,
+                            schedule_forcing=schedule_forcing)
+        # the q' rows the window calls read
+        self.qwin = net.permute_forcing(qprime) if schedule_forcing else qprime
         f = lambda x: torch.as_tensor(np.ascontiguousarray(x, np.float32), device=dev)
         at = wl.attrs
         self.n = f(at["n"]) if n is None else n.detach().float().contiguous()
@@ -250,7 +256,7 @@ class GaugedWindow:
         NAT = self.NAT
         form = NAT.FORM_PERSISTENT if form is None else form
         NAT.check(NAT.lib.ddrs_mc_window_fwd_gauged(self.ws.handle, self.cp, self.T,
-                                                    self.qprime.data_ptr(), self.q0.data_ptr(),
+                                                    self.qwin.data_ptr(), self.q0.data_ptr(),
                                                     self.Q.data_ptr(), form,
                                                     stream or self.stream()),
                   "ddrs_mc_window_fwd_gauged")
@@ -261,7 +267,7 @@ class GaugedWindow:
         NAT = self.NAT
         form = NAT.FORM_PERSISTENT if form is None else form
         NAT.check(NAT.lib.ddrs_mc_window_adj_gauged(self.ws.handle, self.cp, self.T,
-                                                    self.qprime.data_ptr(), self.q0.data_ptr(),
+                                                    self.qwin.data_ptr(), self.q0.data_ptr(),
                                                     self.gQ.data_ptr(), self.gn.data_ptr(),
                                                     self.gqsp.data_ptr(), None, None, None,
                                                     form, stream or self.stream()),
@@ -289,7 +295,7 @@ class GaugedWindow:
             out = self.torch.empty((self.T, self.R), dtype=self.torch.float32,
                                    device=self.qprime.device)
         NAT.check(NAT.lib.ddrs_mc_window_fwd(self.ws.handle, self.cp, self.T,
-                                             self.qprime.data_ptr(), self.q0.data_ptr(),
+                                             self.qwin.data_ptr(), self.q0.data_ptr(),
                                              out.data_ptr(), form, self.stream()),
                   "ddrs_mc_window_fwd")
         return out

@@ -95,6 +95,17 @@ ddrs_status ddrs_plan_get_csr(ddrs_plan_t plan, int32_t* h_crow, int32_t* h_col,
 /* Copy the schedule: h_perm[p] = reach at schedule position p, h_level[i] = level
  * (longest upstream path) of reach i. */
 ddrs_status ddrs_plan_get_schedule(ddrs_plan_t plan, int32_t* h_perm, int32_t* h_level);
+/* Number of schedule positions P (>= n: chunks are padded to whole sub-levels). */
+ddrs_status ddrs_plan_positions(ddrs_plan_t plan, int32_t* P);
+/* q' in schedule order for DDRS_WS_SCHEDULE_FORCING workspaces: d_q_sched[k][p]
+ * = d_q_prime[k][reach at position p] for k < rows (0 at padding positions);
+ * d_q_prime [rows][n] in reach order, d_q_sched [rows][P], device pointers,
+ * enqueued on `stream`. Called once per batch by the data side, so that no
+ * training or routing step permutes its forcing. No reference counterpart
+ * (the reference's forcing stays in reach order: mmc.rs:519-632 reads
+ * q_prime[t-1] per step). */
+ddrs_status ddrs_plan_permute_forcing(ddrs_plan_t plan, int32_t rows, const float* d_q_prime,
+                                      float* d_q_sched, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Workspace: device state for windows of up to `window` timesteps (the
@@ -108,8 +119,14 @@ ddrs_status ddrs_workspace_create(ddrs_plan_t plan, int32_t window, int32_t max_
 /* As ddrs_workspace_create with flags: DDRS_WS_NO_DENSE_GRAD leaves out the
  * [max_steps][n] buffer of a dense dL/dQ (only the _gauged adjoint is then
  * available; the dense one returns DDRS_ERR_INVALID). For windows whose q'
- * alone fills a large part of HBM. No reference counterpart. */
-enum { DDRS_WS_NO_DENSE_GRAD = 1 };
+ * alone fills a large part of HBM. DDRS_WS_SCHEDULE_FORCING: the window calls
+ * (ddrs_mc_window_fwd/_adj and their _gauged forms) take q' as [T][P] rows in
+ * schedule order, written by ddrs_plan_permute_forcing, and read it in place:
+ * the workspace holds no [max_steps][P] copy and the calls permute nothing.
+ * The single-step calls keep reach order. Results are bit-identical to reach
+ * order. No reference counterpart (the reference reads q' [T][n] in reach
+ * order, mmc.rs:519-632). */
+enum { DDRS_WS_NO_DENSE_GRAD = 1, DDRS_WS_SCHEDULE_FORCING = 2 };
 ddrs_status ddrs_workspace_create_ex(ddrs_plan_t plan, int32_t window, int32_t max_steps,
                                      int32_t flags, ddrs_workspace_t* out);
 void ddrs_workspace_destroy(ddrs_workspace_t ws);

@@ -403,13 +403,15 @@ def test_chunk_width_invariance(kind):
     check_discharge(kind, out[2]["Q"], c.oracle_Q(), c.oracle_Q(np.float64))
 
 
-def _window_raw(c, ws, gQ, form=NAT.FORM_PERSISTENT):
-    """Forward then adjoint window through the C ABI on one workspace."""
+def _window_raw(c, ws, gQ, form=NAT.FORM_PERSISTENT, qp=None):
+    """Forward then adjoint window through the C ABI on one workspace (qp:
+    the q' rows to pass, default c.qprime in reach order)."""
     params = to_params(c.P).to_c()
     s = torch.cuda.current_stream().cuda_stream
     R, T = c.R, c.T
     n, qsp, p, ln, sl = (t(v) for v in (c.n, c.qsp, c.p, c.length, c.slope_raw))
-    qp, q0, g = t(c.qprime), t(c.q0()), t(gQ)
+    qp = t(c.qprime) if qp is None else qp
+    q0, g = t(c.q0()), t(gQ)
     Q = torch.empty((T, R), dtype=torch.float32, device="cuda")
     outs = [torch.empty(R, device="cuda") for _ in range(4)]
     gqp = torch.empty((T, R), device="cuda")
@@ -461,6 +463,83 @@ def test_gauged_window_identical(form):
     np.testing.assert_array_equal(gqp.cpu().numpy(), dense["q_prime"])
     with pytest.raises(NAT.DdrsError):
         ws.set_gauges([1, 1])
+
+
+def test_permute_forcing_layout():
+    """ddrs_plan_permute_forcing: row k of the output holds row k of q' at the
+    schedule's positions, the reaches in the order ddrs_plan_get_schedule
+    gives, and zero at the padding positions; over several launches' worth of
+    rows (the permutation deals whole rows to the XCDs in turns of 8)."""
+    say("permute forcing layout")
+    c = Case(5000, 21, seed=51)
+    R, T, npos = c.R, c.T, c.net.positions
+    # exactly representable codes 1 + k R + i (below 2^24)
+    code = (1 + np.arange(T)[:, None] * R + np.arange(R)[None, :]).astype(np.float32)
+    out = c.net.permute_forcing(t(code)).cpu().numpy()
+    assert out.shape == (T, npos) and npos >= R
+    perm, _ = c.net.schedule()
+    for k in range(T):
+        nz = out[k] != 0
+        assert int(nz.sum()) == R
+        np.testing.assert_array_equal(out[k][nz].astype(np.int64) - 1 - k * R, perm)
+    assert c.net.permute_forcing(t(code[:0])).shape == (0, npos)
+
+
+@pytest.mark.parametrize("form", [NAT.FORM_PERSISTENT, NAT.FORM_PLAIN])
+def test_schedule_order_forcing_identical(form):
+    """Window calls on a DDRS_WS_SCHEDULE_FORCING workspace, reading q' laid
+    out once by ddrs_plan_permute_forcing, give the reach-order calls' bits:
+    the dense and gauged forward and every adjoint gradient (q' and q0
+    included), over checkpoint segments (window 13 < T-1); the workspace then
+    holds one row of q' instead of T-1; route_window_gauged with q_sched gives
+    its reach-order bits through autograd."""
+    from ddrs_amd.sparse import Workspace
+    say("schedule-order forcing", form)
+    c = Case(6000, 60, seed=43, storms=True)
+    rng = np.random.default_rng(44)
+    gauges = np.sort(np.concatenate([[c.R - 1], rng.choice(c.R - 1, 29, replace=False)]))
+    gQ = np.zeros((c.T, c.R), np.float32)
+    gQ[:, gauges] = rng.normal(0, 1, (c.T, gauges.size)).astype(np.float32)
+    ref = _window_raw(c, c.net.workspace(13, c.T - 1), gQ, form)
+    qs = c.net.permute_forcing(t(c.qprime))
+    ws = Workspace(c.net, 13, c.T - 1, schedule_forcing=True)
+    got = _window_raw(c, ws, gQ, form, qp=qs)
+    for k in ref:
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+    ws.set_gauges(gauges)
+    params = to_params(c.P).to_c()
+    s = torch.cuda.current_stream().cuda_stream
+    q0 = t(c.q0())
+    Qg = torch.empty((c.T, gauges.size), device="cuda")
+    gQg = t(gQ[:, gauges].copy())
+    outs = [torch.empty(c.R, device="cuda") for _ in range(4)]
+    gqp = torch.empty((c.T, c.R), device="cuda")
+    P = lambda x: x.data_ptr()
+    NAT.check(NAT.lib.ddrs_mc_window_fwd_gauged(ws.handle, params, c.T, P(qs), P(q0), P(Qg),
+                                                form, s))
+    NAT.check(NAT.lib.ddrs_mc_window_adj_gauged(ws.handle, params, c.T, P(qs), P(q0), P(gQg),
+                                                P(outs[0]), P(outs[1]), P(outs[2]), P(gqp),
+                                                P(outs[3]), form, s))
+    ws.check()
+    np.testing.assert_array_equal(Qg.cpu().numpy(), ref["Q"][:, gauges])
+    for k, v in zip(["n", "q_spatial", "p_spatial", "q0"], outs):
+        np.testing.assert_array_equal(v.cpu().numpy(), ref[k], err_msg=k)
+    np.testing.assert_array_equal(gqp.cpu().numpy(), ref["q_prime"])
+    ws.close()
+    dense_b = c.net.workspace_bytes(13, c.T - 1)
+    sched_b = c.net.workspace_bytes(13, c.T - 1, schedule_forcing=True)
+    assert dense_b - sched_b == (c.T - 2) * c.net.positions * 4
+    # through autograd
+    pr = to_params(c.P)
+    outs = []
+    for q_sched in (None, qs):
+        n, qsp = t(c.n).requires_grad_(), t(c.qsp).requires_grad_()
+        Qa = D.route_window_gauged(c.net, pr, n, qsp, t(c.p), t(c.qprime), q0, t(c.length),
+                                   t(c.slope_raw), gauges, form=form, window=13, q_sched=q_sched)
+        Qa.backward(gQg)
+        outs.append((Qa.detach().cpu().numpy(), n.grad.cpu().numpy(), qsp.grad.cpu().numpy()))
+    for a, b in zip(*outs):
+        np.testing.assert_array_equal(a, b)
 
 
 def test_route_window_gauged_autograd():
