@@ -13,9 +13,13 @@
 namespace ddrs {
 
 // Links of a chunk staged in LDS, per 64 reaches of chunk width: upstream
-// links (forward) and downstream links (adjoint; one per reach in a tree).
+// links (forward) and downstream links (adjoint; one per reach in a tree, so
+// a tree's chunks never close early for them). The kernels' LDS is sized so
+// that 24 one-wave workgroups (six per SIMD at 80 VGPRs) fit
+// in a CU's 160 KB: persistent launches need every chunk of the window
+// resident at once for the time skew to pipeline.
 constexpr int kStageUpBase = 192;
-constexpr int kStageDnBase = 128;
+constexpr int kStageDnBase = 64;
 // Chunk widths: 64 * RPL positions for RPL (reaches per lane) in {1, 2}.
 constexpr int kNumWidths = 2;
 // Longest chain of links inside a 64-position chunk (its sub-levels): every

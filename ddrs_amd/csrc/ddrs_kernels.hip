@@ -83,8 +83,43 @@ __device__ __forceinline__ void publish_pair(__amdgpu_buffer_rsrc_t row, int i, 
 __device__ __forceinline__ u32x4 poll_pair(__amdgpu_buffer_rsrc_t row, int i) {
     return __builtin_amdgcn_raw_buffer_load_b128(row, i * 16, 0, kCoherentPolicy);
 }
-__device__ __forceinline__ bool pair_ready(const u32x4& w, uint32_t epoch) {
-    return w.y == epoch && w.w == epoch;
+__device__ __forceinline__ bool pair_ready(const u32x4& w, uint32_t tag) {
+    return w.y == tag && w.w == tag;
+}
+
+// Waits until at most N vector memory operations of the wave are outstanding.
+// They complete in issue order, so this waits for everything issued before
+// the last N.
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
+// LDS reads of data that direct-to-LDS loads wrote, for use after wait_vm
+// has covered those loads. The compiler orders its own LDS reads after every
+// outstanding direct-to-LDS load by waiting for all vector memory operations,
+// stores included; these reads carry their own LDS wait and nothing else.
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+    return static_cast<uint32_t>(reinterpret_cast<uintptr_t>(p));
+}
+__device__ __forceinline__ u32x4 lds_read_dma16(const void* p) {
+    u32x4 v;
+    asm volatile("ds_read_b128 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=v"(v) : "v"(lds_addr(p)) : "memory");
+    return v;
+}
+__device__ __forceinline__ float lds_read_dma4(const void* p) {
+    float v;
+    asm volatile("ds_read_b32 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=v"(v) : "v"(lds_addr(p)) : "memory");
+    return v;
+}
+
+// Orders LDS within a one-wave workgroup: a wave's LDS operations execute in
+// issue order, so a compiler barrier is all it needs (__syncthreads would also
+// wait for every outstanding global store, to release them to the workgroup).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 }
 
 // Marks a loaded value as used here. The compiler waits for a load at the
@@ -106,6 +141,13 @@ __device__ __forceinline__ void load_to_lds(const void* g, void* lds, int bytes)
 #if defined(__HIP_DEVICE_COMPILE__)
     if (bytes == 16) __builtin_amdgcn_global_load_lds(g, lds, 16, 0, 0);
     else __builtin_amdgcn_global_load_lds(g, lds, 4, 0, 0);
+#endif
+}
+// The same for 16 bytes with the cache policy of the agent-scope atomics
+// (sc1, device coherent): polls of tagged pairs.
+__device__ __forceinline__ void poll_pair_to_lds(const void* g, void* lds) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_global_load_lds(g, lds, 16, 0, kCoherentPolicy);
 #endif
 }
 
@@ -166,11 +208,13 @@ __global__ void k_begin_launch(LaunchState* ls, const unsigned long long* ctr) {
 // memory counter is in order, so a wait for loads issued after the stores
 // would also wait for the stores to be acknowledged (a round trip to the
 // coherence point), while loads issued before them are waited for alone. The
-// requests also overlap the next step's coefficient arithmetic. A waiting chunk polls only the
-// representatives, after a growing pause (0.2 to 1.7 us), so that waiting
-// chunks do not flood the memory system. All five are called by the whole wave. Returns early (error flag set) when
-// the inputs of a step do not arrive within the watchdog time or another
-// workgroup has raised the flag.
+// requests also overlap the next step's coefficient arithmetic. A waiting
+// chunk polls only the representatives, after a growing pause (0.2 to 1.7 us),
+// so that waiting chunks do not flood the memory system. All five are called
+// by the whole wave, and the workgroup is that one wave: its LDS operations
+// execute in order, so the phases need no workgroup barrier. Returns early
+// (error flag set) when the inputs of a step do not arrive within the watchdog
+// time or another workgroup has raised the flag.
 template <int Dir, class Issue, class Prepare, class Check, class Probe, class Solve>
 __device__ __forceinline__ void chunk_loop(int nsteps, int* err, unsigned long long* trace,
                                            Issue issue, Prepare prepare, Check check,
@@ -182,7 +226,7 @@ __device__ __forceinline__ void chunk_loop(int nsteps, int* err, unsigned long l
         const int s = (Dir > 0) ? done : (nsteps - 1 - done);
         const uint64_t ta = trace ? wall_ticks() : 0;
         prepare(s, done == 0);
-        __syncthreads();
+        wave_lds_sync();
         if (done == 0) issue(s, false);  // the first step stages its links in prepare
         const uint64_t tb = trace ? wall_ticks() : 0;
         const bool ok = check(s);
@@ -236,10 +280,6 @@ __device__ __forceinline__ void chunk_loop(int nsteps, int* err, unsigned long l
     }
 }
 
-// Link slots per lane whose polls are issued ahead of prepare; links beyond
-// 64 * kIssueSlots in a chunk are polled when checked.
-constexpr int kIssueSlots = 1;
-
 // (v / ad) for the diagonal ad = 1 + (-c1) * 0 of I - c1 N: exactly v when
 // ad == 1, the division otherwise (c1 infinite or NaN).
 __device__ __forceinline__ float div_diag(float v, float ad) { return (ad == 1.0f) ? v : v / ad; }
@@ -251,16 +291,32 @@ struct Width {
     static constexpr int SHIFT = (RPL == 1) ? 6 : 7;
     static constexpr int kStageUp = kStageUpBase * RPL;  // staged upstream links
     static constexpr int kStageDn = kStageDnBase * RPL;  // staged downstream links
+    // forward: link slots per lane whose polls are issued ahead of prepare
+    // (every staged link), and upstream links per reach held in registers
+    static constexpr int kSlots = kStageUp / 64;
+    static constexpr int kFast = 4;
 };
 
 // A reach's first links, kept per lane so that a sub-level of the solve reads
-// only the linked values: code >= 0 is the chunk-local index of a link inside
-// the chunk, code < 0 is -1 - (its slot among the chunk's staged links).
-constexpr int kFastLinks = 2;    // forward: upstream links
+// only the linked values. Adjoint: code >= 0 is the chunk-local index of a
+// link inside the chunk, code < 0 is -1 - (its slot among the chunk's staged
+// links). Forward: one index into the chunk's values, [0, CW) the chunk's own
+// solves and CW + slot the staged links from other chunks, so that every link
+// of a sub-level is one LDS read, all issued before the first is waited for.
+// The first Width::kFast upstream links of a reach are read that way (four
+// cover all but ~0.4% of the reaches of river networks; the register budget
+// of six waves per SIMD holds no more); the links beyond them take a loop.
 constexpr int kFastLinksDn = 1;  // adjoint: downstream links (one per reach in a tree)
+// Sub-levels whose largest link count is kept per chunk (4 bits each).
+constexpr int kLinkBoundSubs = 16;
 template <int RPL>
 __device__ __forceinline__ int link_code(int c, int base, int e, int q) {
     return ((q >> Width<RPL>::SHIFT) == c) ? (q & (Width<RPL>::CW - 1)) : (-1 - (e - base));
+}
+template <int RPL>
+__device__ __forceinline__ int link_index(int c, int base, int e, int q) {
+    return ((q >> Width<RPL>::SHIFT) == c) ? (q & (Width<RPL>::CW - 1))
+                                          : (Width<RPL>::CW + (e - base));
 }
 
 // Stage the links [base, end) of chunk c into LDS: link position, weight, and
@@ -287,8 +343,12 @@ template <int PH, int RPL>
 __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
     using Wd = Width<RPL>;
     constexpr int CW = Wd::CW;
-    __shared__ float lds_x[CW];
-    __shared__ float sval[Wd::kStageUp], sadj[Wd::kStageUp];
+    // the chunk's solves (lds_x) and the staged values of links from other
+    // chunks (sval), one array so that a link is one index (link_index)
+    __shared__ float xv[CW + Wd::kStageUp];
+    float* const lds_x = xv;
+    float* const sval = xv + CW;
+    __shared__ float sadj[Wd::kStageUp];
     __shared__ int spos[Wd::kStageUp];
     __shared__ short skind[Wd::kStageUp];
     __shared__ float pb[CW], pnc1[CW], sq[CW], si[CW];
@@ -296,8 +356,8 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
     __shared__ int porig[CW];
     __shared__ int pgs[CW];  // gauge slot of the position, -1: none
     __shared__ short srep[kRepMax];
-    __shared__ int lkc[kFastLinks][CW];
-    __shared__ float lka[kFastLinks][CW];
+    __shared__ short lkc[Wd::kFast][CW];  // link_index (< CW + kStageUp)
+    __shared__ int slm[kLinkBoundSubs];
     __shared__ int claimed;
     const int lane = threadIdx.x;
     const Sched& sc = a.sc;
@@ -320,6 +380,11 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
     float qp_next[RPL];
     // whether this chunk holds a gauge of the readout (wave-uniform)
     bool hg = false;
+    // links to read at sub-level l < kLinkBoundSubs: bits [4l, 4l + 4) hold the
+    // largest link count (capped at kFast) of the sub-level's reaches; lall
+    // the largest over the chunk (wave-uniform)
+    uint64_t lbound = ~0ull;
+    int lall = Wd::kFast;
 
     // S1..S25 for window step s: coefficients and right-hand side.
     auto prepare = [&](int s, bool first) {
@@ -356,11 +421,10 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
                     rs[r] = load_reach(a.st, p);
                     qp_next[r] = a.qps[static_cast<size_t>(t) * P + p];
 #pragma unroll
-                    for (int k = 0; k < kFastLinks; ++k) {
-                        if (ea + k < eb) {
-                            lkc[k][j] = link_code<RPL>(c, base, ea + k, sc.up_pos[ea + k]);
-                            lka[k][j] = sc.up_adj[ea + k];
-                        }
+                    for (int k = 0; k < Wd::kFast; ++k) {
+                        if (ea + k < eb)
+                            lkc[k][j] = static_cast<short>(
+                                link_index<RPL>(c, base, ea + k, sc.up_pos[ea + k]));
                     }
                 }
                 const float q = sq[j];
@@ -379,44 +443,53 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
             }
         }
         unit = __all(ok_diag);
+        if (first) {
+            // per sub-level link bounds, for the solve's read count
+            if (lane < kLinkBoundSubs) slm[lane] = 0;
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                const int j = r * 64 + lane;
+                const int l = psub[j];
+                if (l >= 0 && l < kLinkBoundSubs) atomicMax(&slm[l], min(pe1[j] - pe0[j], Wd::kFast));
+            }
+            __syncthreads();
+            uint64_t m = 0;
+            int mx = 0;
+            for (int l = 0; l < kLinkBoundSubs; ++l) {
+                const int v = slm[l];
+                m |= static_cast<uint64_t>(v) << (4 * l);
+                mx = max(mx, v);
+            }
+            lbound = (static_cast<uint64_t>(uni(static_cast<int>(m >> 32))) << 32) |
+                     static_cast<uint32_t>(uni(static_cast<int>(m)));
+            // sub-levels beyond the kept ones read every register-held link
+            lall = (nsub > kLinkBoundSubs) ? Wd::kFast : uni(mx);
+        }
     };
 
-    uint64_t pw[kIssueSlots];
+    // every staged link is polled before prepare, so that the check after it
+    // waits for no round trip when the inputs were published in time
+    uint64_t pw[Wd::kSlots];
     auto issue = [&](int s, bool) {
         const uint64_t* row = a.hist + static_cast<size_t>(a.s_base + s) * P;
 #pragma unroll
-        for (int k = 0; k < kIssueSlots; ++k) {
-            const int e = lane + 64 * k;
-            pw[k] = (e < nlinks && skind[e] < 0) ? poll(row + spos[e]) : 0;
+        for (int k = 0; k < Wd::kSlots; ++k) {
+            if (64 * k < nlinks) {
+                const int e = lane + 64 * k;
+                pw[k] = (e < nlinks && skind[e] < 0) ? poll(row + spos[e]) : 0;
+            }
         }
     };
-    auto check = [&](int s) -> bool {
-        const uint64_t* row = a.hist + static_cast<size_t>(a.s_base + s) * P;
+    auto check = [&](int) -> bool {
         bool ok = true;
 #pragma unroll
-        for (int k = 0; k < kIssueSlots; ++k) {
-            const int e = lane + 64 * k;
-            if (e < nlinks && skind[e] < 0) {
-                if (!word_ready(pw[k], epoch)) ok = false;
-                else sval[e] = word_value(pw[k]);
-            }
-        }
-        // links beyond the issued ones: all loads first, then the checks, so
-        // that their round trips overlap
-        if (nlinks > 64 * kIssueSlots) {
-            constexpr int kMore = Wd::kStageUp / 64 - kIssueSlots;
-            uint64_t wx[kMore];
-#pragma unroll
-            for (int k = 0; k < kMore; ++k) {
-                const int e = lane + 64 * (kIssueSlots + k);
-                wx[k] = (e < nlinks && skind[e] < 0) ? poll(row + spos[e]) : 0;
-            }
-#pragma unroll
-            for (int k = 0; k < kMore; ++k) {
-                const int e = lane + 64 * (kIssueSlots + k);
+        for (int k = 0; k < Wd::kSlots; ++k) {
+            if (64 * k < nlinks) {
+                const int e = lane + 64 * k;
                 if (e < nlinks && skind[e] < 0) {
-                    if (!word_ready(wx[k], epoch)) ok = false;
-                    else sval[e] = word_value(wx[k]);
+                    if (!word_ready(pw[k], epoch)) ok = false;
+                    else sval[e] = word_value(pw[k]);
                 }
             }
         }
@@ -442,8 +515,14 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
 #pragma unroll
         for (int r = 0; r < RPL; ++r) touch(qp_next[r]);
         // per-lane operands of the solve, read once per step
-        int my[RPL], e0[RPL], nl[RPL], code[RPL][kFastLinks];
-        float x[RPL], inext[RPL], sxn[RPL], b[RPL], nc1[RPL], w[RPL][kFastLinks];
+        // link indices two to a register (16 bits each); the link weights are
+        // read beside the linked values from the staged weights (slot e0 + k)
+        int my[RPL], e0[RPL], nl[RPL];
+        uint32_t lix2[RPL][(Wd::kFast + 1) / 2];
+        float x[RPL], inext[RPL], sxn[RPL], b[RPL], nc1[RPL];
+        auto lix = [&](int r, int k) -> int {
+            return static_cast<int>((lix2[r][k >> 1] >> (16 * (k & 1))) & 0xffffu);
+        };
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
             const int j = r * 64 + lane;
@@ -456,45 +535,92 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
             e0[r] = pe0[j];
             nl[r] = pe1[j] - e0[r];
 #pragma unroll
-            for (int k = 0; k < kFastLinks; ++k) {
-                code[r][k] = lkc[k][j];
-                w[r][k] = lka[k][j];
+            for (int k = 0; k < (Wd::kFast + 1) / 2; ++k) {
+                // unused slots read value 0 of the chunk (always in bounds)
+                const uint32_t i0 = (2 * k < nl[r]) ? static_cast<uint32_t>(lkc[2 * k][j]) : 0u;
+                const uint32_t i1 = (2 * k + 1 < Wd::kFast && 2 * k + 1 < nl[r])
+                                        ? static_cast<uint32_t>(lkc[2 * k + 1][j]) : 0u;
+                lix2[r][k] = i0 | (i1 << 16);
             }
         }
+        // The sub-levels carry only the solve's own sum; the step's other two
+        // upstream sums (N max(x, lb) for the next right-hand side and N x for
+        // the history) are taken once after the last sub-level, over the same
+        // values in the same CSR order.
         for (int l = 0; l < nsub; ++l) {
+            const int lm = (l < kLinkBoundSubs) ? static_cast<int>((lbound >> (4 * l)) & 15u)
+                                                : Wd::kFast;
 #pragma unroll
             for (int r = 0; r < RPL; ++r) {
                 if (my[r] == l) {
                     const int j = r * 64 + lane;
-                    float sum = 0.0f, in = 0.0f, sx = 0.0f;
-                    if (nl[r] <= kFastLinks) {
+                    float sum = 0.0f;
+                    {
+                        // the first kFast links: every read first, then the sum
+                        // in CSR order
+                        float v[Wd::kFast];
+                        float wk[Wd::kFast];
 #pragma unroll
-                        for (int k = 0; k < kFastLinks; ++k) {
-                            if (k < nl[r]) {
-                                const int cd = code[r][k];
-                                const float v = (cd >= 0) ? lds_x[cd] : sval[-1 - cd];
-                                sum = sum + (nc1[r] * w[r][k]) * v;
-                                in = in + fmaxr(v, a.P.q_lb) * w[r][k];
-                                sx = sx + v * w[r][k];
+                        for (int k = 0; k < Wd::kFast; ++k) {
+                            if (k < lm) {
+                                v[k] = xv[lix(r, k)];
+                                wk[k] = sadj[(k < nl[r]) ? e0[r] + k : 0];
                             }
                         }
-                    } else {
-                        for (int e = e0[r]; e < e0[r] + nl[r]; ++e) {
-                            const int kd = skind[e];
-                            const float v = (kd >= 0) ? lds_x[kd] : sval[e];
-                            const float adj = sadj[e];
-                            sum = sum + (nc1[r] * adj) * v;
-                            in = in + fmaxr(v, a.P.q_lb) * adj;
-                            sx = sx + v * adj;
+#pragma unroll
+                        for (int k = 0; k < Wd::kFast; ++k) {
+                            if (k < lm) {
+                                const float term = (nc1[r] * wk[k]) * v[k];
+                                if (k < nl[r]) sum = sum + term;
+                            }
                         }
                     }
+                    // links beyond them (rare), continuing the order
+                    for (int e = e0[r] + Wd::kFast; e < e0[r] + nl[r]; ++e) {
+                        const int kd = skind[e];
+                        const float v = (kd >= 0) ? lds_x[kd] : sval[e];
+                        sum = sum + (nc1[r] * sadj[e]) * v;
+                    }
                     x[r] = unit ? (b[r] - sum) : div_diag(b[r] - sum, 1.0f + nc1[r] * 0.0f);
-                    inext[r] = in;
-                    sxn[r] = sx;
                     lds_x[j] = x[r];
                 }
             }
             __syncthreads();
+        }
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+            float in = 0.0f, sx = 0.0f;
+            if (my[r] >= 0) {
+                const int j = r * 64 + lane;
+                float v[Wd::kFast], wk[Wd::kFast];
+#pragma unroll
+                for (int k = 0; k < Wd::kFast; ++k) {
+                    if (k < lall) {
+                        v[k] = xv[lix(r, k)];
+                        wk[k] = sadj[(k < nl[r]) ? e0[r] + k : 0];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < Wd::kFast; ++k) {
+                    if (k < lall) {
+                        const float ti = fmaxr(v[k], a.P.q_lb) * wk[k];
+                        const float tx = v[k] * wk[k];
+                        if (k < nl[r]) {
+                            in = in + ti;
+                            sx = sx + tx;
+                        }
+                    }
+                }
+                for (int e = e0[r] + Wd::kFast; e < e0[r] + nl[r]; ++e) {
+                    const int kd = skind[e];
+                    const float vv = (kd >= 0) ? lds_x[kd] : sval[e];
+                    const float adj = sadj[e];
+                    in = in + fmaxr(vv, a.P.q_lb) * adj;
+                    sx = sx + vv * adj;
+                }
+            }
+            inext[r] = in;
+            sxn[r] = sx;
         }
         before_stores();
         // Publish after the solve: other chunks read a step's values only once
@@ -757,7 +883,13 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
     __shared__ short srep[kRepMax];
     __shared__ int lkc[kFastLinksDn][CW];
     __shared__ float lka[kFastLinksDn][CW];
+    // the polled (P, G) pairs of the staged links, loaded straight into LDS
+    __shared__ u32x4 spair[Wd::kStageDn];
     __shared__ int claimed;
+    constexpr int kSlotsDn = Wd::kStageDn / 64;
+    // vector memory operations a step issues after its polls: the pair
+    // publish and the gb store of each row of the chunk
+    constexpr int kStoresAfterPolls = 2 * RPL;
     const int lane = threadIdx.x;
     const Sched& sc = a.sc;
     const int P = sc.P;
@@ -773,12 +905,15 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
     const int nlinks = sc.dn_ptr[p0 + CW] - base;
     const int nsub = a.ch.nsubr[c];
 
-    // A step's records and its cross-chunk inputs of the first link slot are
-    // loaded by issue(s), which the previous step calls before its stores.
-    // The records go from memory straight to LDS: a register that carries a
-    // load from one step to the next makes the compiler wait for everything
-    // in flight (the previous step's stores included) before reusing it.
-    u32x4 pw = {0, 0, 0, 0};  // P and G words of link `lane`
+    // A step's records and its cross-chunk inputs are loaded by issue(s),
+    // which the previous step calls before its stores; all of them go from
+    // memory straight to LDS. The stores that follow them are exactly
+    // kStoresAfterPolls operations, so check() waits for the loads alone
+    // (wait_vm) and not for the stores to reach the coherence point; a
+    // register carrying a load across the step would make the compiler wait
+    // for everything in flight. The tags of a pair name its row's parity
+    // (row_tag), so a pair the previous step left in LDS never passes the check.
+    bool stores_after_polls = false;
     // rows r of the chunk (positions r * 64 ...) holding a gauge: the others
     // read no gauge-sparse dL/dQ
     unsigned grows = 0;
@@ -843,45 +978,38 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
                 }
             }
         }
-        const int d = (lane < nlinks && skind[lane] < 0) ? spos[lane] : 0;
-        pw = poll_pair(pair_row(a.amsg + row * 2, P), d);
-    };
-    auto check = [&](int s) -> bool {
-        const auto mrow = pair_row(a.amsg + static_cast<size_t>(a.s_base + s) * P * 2, P);
-        bool ok = true;
-        if (lane < nlinks && skind[lane] < 0) {
-            if (!pair_ready(pw, epoch)) {
-                ok = false;
-            } else {
-                stP[lane] = __uint_as_float(pw.x);
-                stG[lane] = __uint_as_float(pw.z);
+        // every lane of a used slot loads (position 0 where it has no link)
+        const uint64_t* mrow = a.amsg + row * 2;
+#pragma unroll
+        for (int k = 0; k < kSlotsDn; ++k) {
+            if (64 * k < nlinks) {
+                const int e = lane + 64 * k;
+                const int d = (e < nlinks && skind[e] < 0) ? spos[e] : 0;
+                poll_pair_to_lds(mrow + 2 * d, &spair[64 * k]);
             }
         }
-        // links beyond the first slot: all loads first, then the checks
-        if (nlinks > 64) {
-            constexpr int kMore = Wd::kStageDn / 64 - 1;
-            u32x4 w[kMore];
+        stores_after_polls = false;
+    };
+    auto check = [&](int s) -> bool {
+        if (stores_after_polls) wait_vm<kStoresAfterPolls>();
+        else wait_vm<0>();
+        const uint32_t tag = row_tag(epoch, a.s_base + s);
+        bool ok = true;
 #pragma unroll
-            for (int k = 0; k < kMore; ++k) {
-                const int e = lane + 64 * (k + 1);
-                w[k] = u32x4{0, 0, 0, 0};
-                if (e < nlinks && skind[e] < 0) w[k] = poll_pair(mrow, spos[e]);
-            }
-#pragma unroll
-            for (int k = 0; k < kMore; ++k) {
-                const int e = lane + 64 * (k + 1);
-                if (e < nlinks && skind[e] < 0) {
-                    if (!pair_ready(w[k], epoch)) {
-                        ok = false;
-                    } else {
-                        stP[e] = __uint_as_float(w[k].x);
-                        stG[e] = __uint_as_float(w[k].z);
-                    }
+        for (int k = 0; k < kSlotsDn; ++k) {
+            const int e = lane + 64 * k;
+            if (64 * k < nlinks && e < nlinks && skind[e] < 0) {
+                const u32x4 w = lds_read_dma16(&spair[e]);
+                if (!pair_ready(w, tag)) {
+                    ok = false;
+                } else {
+                    stP[e] = __uint_as_float(w.x);
+                    stG[e] = __uint_as_float(w.z);
                 }
             }
         }
         const bool all = __all(ok);
-        __syncthreads();
+        wave_lds_sync();
         return all;
     };
 
@@ -889,22 +1017,35 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
     auto probe = [&](int s) -> bool {
         if (nrep <= 0) return true;  // no representatives: read the links again
         const uint64_t* row = a.amsg + static_cast<size_t>(a.s_base + s) * P * 2;
+        const uint32_t tag = row_tag(epoch, a.s_base + s);
         bool ok = true;
-        if (lane < nrep) ok = word_ready(poll(row + 2 * spos[srep[lane]] + 1), epoch);
+        if (lane < nrep) ok = word_ready(poll(row + 2 * spos[srep[lane]] + 1), tag);
         return __all(ok);
     };
 
     auto solve = [&](int s, bool last, auto&& before_stores) {
         const int sg = a.s_base + s;
         const auto mrow = pair_row(a.amsg + static_cast<size_t>(sg) * P * 2, P);
-        // the step's records arrived in LDS before the inputs that were just
-        // waited for (loads complete in order)
+        const uint32_t tag = row_tag(epoch, sg);
+        // the step's records (and gauge dL/dQ) were loaded into LDS before the
+        // inputs check() waited for, so they have arrived (loads complete in
+        // order); read once per step
+        float4 rcv[RPL];
+        float dqv[RPL];
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+            const int j = r * 64 + lane;
+            const u32x4 q = lds_read_dma16(&prec[j]);
+            rcv[r] = make_float4(__uint_as_float(q.x), __uint_as_float(q.y), __uint_as_float(q.z),
+                                 __uint_as_float(q.w));
+            dqv[r] = (a.gqs && (!a.gK || ((grows >> r) & 1u))) ? lds_read_dma4(&pdq[j]) : 0.0f;
+        }
         {
             bool ok_diag = true;
 #pragma unroll
             for (int r = 0; r < RPL; ++r) {
                 const int j = r * 64 + lane;
-                if (pvalid[j]) ok_diag = ok_diag && (1.0f + prec[j].x * 0.0f == 1.0f);
+                if (pvalid[j]) ok_diag = ok_diag && (1.0f + rcv[r].x * 0.0f == 1.0f);
             }
             unit = __all(ok_diag);
         }
@@ -931,7 +1072,7 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
             for (int r = 0; r < RPL; ++r) {
                 if (my[r] == l) {
                     const int j = r * 64 + lane;
-                    const float4 rc = prec[j];  // {-c1, c2, dgq/dgb, mask}
+                    const float4 rc = rcv[r];  // {-c1, c2, dgq/dgb, mask}
                     float sumP = 0.0f, sumG = 0.0f;
                     if (nl[r] <= kFastLinksDn) {
 #pragma unroll
@@ -961,28 +1102,31 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
                     ldsP[j] = Pm;
                     ldsG[j] = Gm;
                     gbv[r] = gb;
-                    const float dq = (a.gqs && (!a.gK || pgs[j] >= 0)) ? pdq[j] : 0.0f;
+                    const float dq = (a.gqs && (!a.gK || pgs[j] >= 0)) ? dqv[r] : 0.0f;
                     lamv[r] = (rc.w != 0.0f) ? dq + (gb * rc.z + sumG) : 0.0f;
                 }
             }
-            __syncthreads();
+            wave_lds_sync();
         }
         before_stores();
-        // published after the solve, as in the forward
+        // Published after the solve, as in the forward. Exactly
+        // kStoresAfterPolls vector memory operations follow the polls: every
+        // lane stores (padding positions' words are never read).
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
-            if (my[r] >= 0) {
-                const int j = r * 64 + lane;
-                publish_pair(mrow, p0 + j, ldsP[j], ldsG[j], epoch);
-            }
+            const int j = r * 64 + lane;
+            publish_pair(mrow, p0 + j, ldsP[j], ldsG[j], tag);
         }
+#pragma unroll
+        for (int r = 0; r < RPL; ++r)
+            a.gbuf[static_cast<size_t>(sg) * P + p0 + r * 64 + lane] = gbv[r];
+        stores_after_polls = true;
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
             if (my[r] >= 0) {
                 const int j = r * 64 + lane;
                 slam[j] = lamv[r];
                 if (last) a.st_lam[p0 + j] = lamv[r];
-                a.gbuf[static_cast<size_t>(sg) * P + p0 + j] = gbv[r];
             }
         }
     };
