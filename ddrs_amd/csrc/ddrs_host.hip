@@ -864,6 +864,9 @@ static Claim begin_launch(ddrs_workspace* ws, int grid, hipStream_t s) {
 static void launch_fwd_ws(ddrs_workspace* ws, FwdArgs a, const ddrs_params* p, hipStream_t s) {
     const int v = variant(p);
     a.claim = begin_launch(ws, ws->grid_fwd[v], s);
+    // the deeper half of the level-major schedule runs at raised priority:
+    // measured flat between 35% and 70% of the chunks on a 350k-reach tree
+    a.prio_from = a.ch.C / 2;
     a.trace.rec = ws->trace[0];
     ws->trace_n[0] = ws->grid_fwd[v];
     launch_fwd(a, v, ws->rpl_fwd[v], ws->grid_fwd[v], s);

@@ -117,6 +117,9 @@ struct FwdArgs {
     float* st_q;           // [P] out[t] of the current step
     float* st_i;           // [P] N out[t]
     int* err;
+    // chunks at or beyond this index run their coefficients and input checks
+    // at a raised issue priority (results do not depend on it)
+    int prio_from;
 };
 
 // Adjoint, phase 1 (every reach-step of a segment independently): the step's

@@ -87,6 +87,37 @@ __device__ __forceinline__ bool pair_ready(const u32x4& w, uint32_t tag) {
     return w.y == tag && w.w == tag;
 }
 
+// One row of an array (`bytes` long, at most 2 GiB) as a buffer resource: its
+// base stays in scalar registers and a lane addresses it with a 32-bit byte
+// offset, so a persistent kernel holds no 64-bit address per lane and array
+// across its steps (the compiler keeps such addresses live across the whole
+// loop and spills them). Accesses past `bytes` read 0 and write nothing.
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void* base, size_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes),
+                                             0x00020000);
+}
+// tagged words through a row: the cache policy of the agent-scope atomics
+__device__ __forceinline__ uint64_t poll_row(__amdgpu_buffer_rsrc_t row, int i) {
+    const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(row, i * 8, 0, kCoherentPolicy);
+    return (static_cast<uint64_t>(w.y) << 32) | w.x;
+}
+__device__ __forceinline__ void publish_row(__amdgpu_buffer_rsrc_t row, int i, float v,
+                                            uint32_t epoch) {
+    const u32x2 w = {__float_as_uint(v), epoch};
+    __builtin_amdgcn_raw_buffer_store_b64(w, row, i * 8, 0, kCoherentPolicy);
+}
+__device__ __forceinline__ float load_row(__amdgpu_buffer_rsrc_t row, int i) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(row, i * 4, 0, 0));
+}
+__device__ __forceinline__ void store_row(__amdgpu_buffer_rsrc_t row, int i, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), row, i * 4, 0, 0);
+}
+__device__ __forceinline__ void store_row2(__amdgpu_buffer_rsrc_t row, int i, float a, float b) {
+    const u32x2 w = {__float_as_uint(a), __float_as_uint(b)};
+    __builtin_amdgcn_raw_buffer_store_b64(w, row, i * 8, 0, 0);
+}
+
 // Waits until at most N vector memory operations of the wave are outstanding.
 // They complete in issue order, so this waits for everything issued before
 // the last N.
@@ -120,18 +151,6 @@ __device__ __forceinline__ float lds_read_dma4(const void* p) {
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-}
-
-// Marks a loaded value as used here. The compiler waits for a load at the
-// first use of its register, and the wave's memory counter cannot single out
-// one load: once a loop whose loads it cannot count (the polling loop) or a
-// store has been issued after it, that wait becomes a wait for everything in
-// flight. Prefetched values are therefore touched where everything issued
-// before has arrived anyway, which leaves their later use free of waits.
-__device__ __forceinline__ void touch(float v) { asm volatile("" ::"v"(v)); }
-__device__ __forceinline__ void touch(int v) { asm volatile("" ::"v"(v)); }
-__device__ __forceinline__ void touch(const float4& v) {
-    asm volatile("" ::"v"(v.x), "v"(v.y), "v"(v.z), "v"(v.w));
 }
 
 // Loads `bytes` (4 or 16) per lane from g straight into LDS at lds + lane *
@@ -218,8 +237,16 @@ __global__ void k_begin_launch(LaunchState* ls, const unsigned long long* ctr) {
 template <int Dir, class Issue, class Prepare, class Check, class Probe, class Solve>
 __device__ __forceinline__ void chunk_loop(int nsteps, int* err, unsigned long long* trace,
                                            Issue issue, Prepare prepare, Check check,
-                                           Probe probe, Solve solve) {
+                                           Probe probe, Solve solve, bool raised = false) {
     const uint64_t t_claim = wall_ticks();
+    // Issue priority outside the solve: raised (2) for chunks the caller marks,
+    // e.g. deep chunks of the forward, whose steps the rest of the window
+    // waits on, while shallow chunks run ahead with slack; the solve runs at 3.
+    auto base_prio = [&]() {
+        if (raised) __builtin_amdgcn_s_setprio(2);
+        else __builtin_amdgcn_s_setprio(0);
+    };
+    base_prio();
     uint64_t t_first = 0, t_prep = 0, t_poll = 0, t_wait = 0, t_solve = 0;
     unsigned long long missed = 0;
     for (int done = 0; done < nsteps; ++done) {
@@ -258,7 +285,7 @@ __device__ __forceinline__ void chunk_loop(int nsteps, int* err, unsigned long l
             if (done + 1 < nsteps) issue((Dir > 0) ? s + 1 : s - 1, false);
         };
         solve(s, done == nsteps - 1, issue_next);
-        __builtin_amdgcn_s_setprio(0);
+        base_prio();
         if (trace) {
             const uint64_t te = wall_ticks();
             if (done == 0) t_first = te;
@@ -375,9 +402,13 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
     const int nsub = a.ch.nsub[c];
 
     // Reach statics stay in registers for the whole launch; q' of the next
-    // step is loaded one step ahead.
+    // step is loaded one step ahead, straight into LDS (sqp): a register
+    // carrying the load into the next step would make the compiler wait, in
+    // the middle of the coefficient arithmetic, for everything in flight (the
+    // previous step's stores included) to move it into place. check() waits
+    // for all loads, so the next prepare reads sqp without a wait of its own.
     Reach rs[RPL];
-    float qp_next[RPL];
+    __shared__ float sqp[CW];
     // whether this chunk holds a gauge of the readout (wave-uniform)
     bool hg = false;
     // links to read at sub-level l < kLinkBoundSubs: bits [4l, 4l + 4) hold the
@@ -390,6 +421,18 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
     auto prepare = [&](int s, bool first) {
         bool ok_diag = true;
         if (first) {
+            // The first step's loads are waited for here, inside its branch:
+            // a register still loading where the branch rejoins the later
+            // steps' path would make the compiler wait, at its uses in every
+            // step, for every operation in flight.
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                const int p = p0 + r * 64 + lane;
+                rs[r] = load_reach(a.st, p);
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    row_rsrc(a.qps + static_cast<size_t>(a.t0 + a.s_base + s) * P, P * 4ull),
+                    &sqp[r * 64], 4, p * 4, 0, 0, 0);
+            }
             stage_links<RPL>(c, base, base + nlinks, sc.up_pos, sc.up_adj, spos, sadj, skind);
             if (lane < nrep) srep[lane] = a.ch.rep_up[rep0 + lane];
             if (a.Qg) {
@@ -402,6 +445,7 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
                 }
                 hg = __any(any) != 0;
             }
+            wait_vm<0>();
         }
         const int t = a.t0 + a.s_base + s;
 #pragma unroll
@@ -418,8 +462,6 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
                     psub[j] = a.ch.sub[p];
                     sq[j] = a.st_q[p];
                     si[j] = a.st_i[p];
-                    rs[r] = load_reach(a.st, p);
-                    qp_next[r] = a.qps[static_cast<size_t>(t) * P + p];
 #pragma unroll
                     for (int k = 0; k < Wd::kFast; ++k) {
                         if (ea + k < eb)
@@ -429,9 +471,11 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
                 }
                 const float q = sq[j];
                 const float it = si[j];
-                const float qp = fmaxr(qp_next[r], a.P.q_lb);
+                const float qp = fmaxr(lds_read_dma4(&sqp[j]), a.P.q_lb);
                 if (s + 1 < a.nsteps)
-                    qp_next[r] = a.qps[static_cast<size_t>(t + 1) * P + p];
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                        row_rsrc(a.qps + static_cast<size_t>(t + 1) * P, P * 4ull), &sqp[r * 64], 4,
+                        p * 4, 0, 0, 0);
                 Coef co;
                 PowEvalInl pe;
                 mc_coef<PH>(a.P, rs[r], q, co, pe);
@@ -472,16 +516,18 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
     // waits for no round trip when the inputs were published in time
     uint64_t pw[Wd::kSlots];
     auto issue = [&](int s, bool) {
-        const uint64_t* row = a.hist + static_cast<size_t>(a.s_base + s) * P;
+        const auto row = row_rsrc(a.hist + static_cast<size_t>(a.s_base + s) * P, P * 8ull);
 #pragma unroll
         for (int k = 0; k < Wd::kSlots; ++k) {
             if (64 * k < nlinks) {
                 const int e = lane + 64 * k;
-                pw[k] = (e < nlinks && skind[e] < 0) ? poll(row + spos[e]) : 0;
+                pw[k] = (e < nlinks && skind[e] < 0) ? poll_row(row, spos[e]) : 0;
             }
         }
     };
     auto check = [&](int) -> bool {
+        // every load in flight, q' of the next step (sqp) included
+        wait_vm<0>();
         bool ok = true;
 #pragma unroll
         for (int k = 0; k < Wd::kSlots; ++k) {
@@ -500,9 +546,9 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
 
     auto probe = [&](int s) -> bool {
         if (nrep <= 0) return true;  // no representatives: read the links again
-        const uint64_t* row = a.hist + static_cast<size_t>(a.s_base + s) * P;
+        const auto row = row_rsrc(a.hist + static_cast<size_t>(a.s_base + s) * P, P * 8ull);
         bool ok = true;
-        if (lane < nrep) ok = word_ready(poll(row + spos[srep[lane]]), epoch);
+        if (lane < nrep) ok = word_ready(poll_row(row, spos[srep[lane]]), epoch);
         return __all(ok);
     };
 
@@ -510,10 +556,7 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
     auto solve = [&](int s, bool last, auto&& before_stores) {
         const int sg = a.s_base + s;
         const int t = a.t0 + sg;
-        uint64_t* row = a.hist + static_cast<size_t>(sg) * P;
-        // q' of the next step, loaded in prepare, has arrived by now (see touch())
-#pragma unroll
-        for (int r = 0; r < RPL; ++r) touch(qp_next[r]);
+        const auto row = row_rsrc(a.hist + static_cast<size_t>(sg) * P, P * 8ull);
         // per-lane operands of the solve, read once per step
         // link indices two to a register (16 bits each); the link weights are
         // read beside the linked values from the staged weights (slot e0 + k)
@@ -627,13 +670,12 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
         // its solve is done.
 #pragma unroll
         for (int r = 0; r < RPL; ++r)
-            if (my[r] >= 0) publish(row + p0 + r * 64 + lane, x[r], epoch);
+            if (my[r] >= 0) publish_row(row, p0 + r * 64 + lane, x[r], epoch);
         if (a.ihist) {
+            const auto irow = row_rsrc(a.ihist + static_cast<size_t>(sg) * P, P * 8ull);
 #pragma unroll
             for (int r = 0; r < RPL; ++r)
-                if (my[r] >= 0)
-                    a.ihist[static_cast<size_t>(sg) * P + p0 + r * 64 + lane] =
-                        make_float2(sxn[r], inext[r]);
+                if (my[r] >= 0) store_row2(irow, p0 + r * 64 + lane, sxn[r], inext[r]);
         }
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
@@ -646,18 +688,20 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
                     a.st_q[p0 + j] = qn;
                     a.st_i[p0 + j] = inext[r];
                 }
-                if (a.Q_out) a.Q_out[static_cast<size_t>(t + 1) * R + porig[j]] = qn;
+                if (a.Q_out)
+                    store_row(row_rsrc(a.Q_out + static_cast<size_t>(t + 1) * R, R * 4ull), porig[j], qn);
                 if (a.X_out) a.X_out[porig[j]] = x[r];
                 if (hg) {
                     const int sl = pgs[j];
-                    if (sl >= 0) a.Qg[static_cast<size_t>(t + 1) * a.gK + sl] = qn;
+                    if (sl >= 0)
+                        store_row(row_rsrc(a.Qg + static_cast<size_t>(t + 1) * a.gK, a.gK * 4ull), sl, qn);
                 }
             }
         }
     };
 
     chunk_loop<+1>(a.nsteps, a.err, a.trace.rec ? a.trace.rec + kTraceWords * c : nullptr, issue,
-                   prepare, check, probe, solve);
+                   prepare, check, probe, solve, c >= a.prio_from);
 }
 
 // ---------------------------------------------------------------------------
