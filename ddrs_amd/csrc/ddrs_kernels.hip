@@ -1544,8 +1544,13 @@ void launch_grad_q0(const Sched& sc, const float* lam, const float* q0, float lb
 void launch_permute_rows(const Sched& sc, const float* q, int t_first, int rows, float* out,
                          hipStream_t s) {
     const int row_blocks = (sc.P + 255) / 256;
-    // launches of at most 2^30 workgroups, in whole turns of kXcds rows
-    const int per = std::max(1, static_cast<int>((1 << 30) / (static_cast<size_t>(row_blocks) * kXcds))) * kXcds;
+    // A launch's grid is at most 2^32 - 1 work-items (the dispatch packet
+    // holds the count in 32 bits; a larger grid is cut short without an
+    // error): launches of at most 2^24 - 1 workgroups, in whole turns of
+    // kXcds rows.
+    const size_t max_blocks = (size_t{1} << 24) - 1;
+    const int per = static_cast<int>(
+        std::max<size_t>(1, max_blocks / (static_cast<size_t>(row_blocks) * kXcds)) * kXcds);
     for (int k0 = 0; k0 < rows; k0 += per) {
         const int n = std::min(per, rows - k0);
         const size_t blocks = static_cast<size_t>((n + kXcds - 1) / kXcds) * kXcds * row_blocks;

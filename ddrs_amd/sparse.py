@@ -91,7 +91,8 @@ class Network:
     def __del__(self):
         for ws in getattr(self, "_ws", {}).values():
             ws.close()
-        if getattr(self, "handle", None):
+        # at interpreter exit the library binding may already be gone
+        if getattr(self, "handle", None) and getattr(N, "lib", None) is not None:
             N.lib.ddrs_plan_destroy(self.handle)
             self.handle = None
 

@@ -485,6 +485,30 @@ def test_permute_forcing_layout():
     assert c.net.permute_forcing(t(code[:0])).shape == (0, npos)
 
 
+def test_permute_forcing_beyond_one_launch():
+    """The permutation of a forcing whose rows x positions exceed what one
+    launch of 256-lane workgroups can cover (2^32 work-items): every row is
+    written, the last included. 2^24 + 3 rows of a 200-reach network laid
+    out in 256 positions."""
+    say("permute forcing beyond one launch")
+    R = 200
+    rows, cols = NW.gen_tree(R, 256, 77)
+    net = Network(SparseAdjacency.from_coo(R, rows, cols))
+    assert net.positions == 256
+    T = (1 << 24) + 3
+    q = torch.rand((T, R), device="cuda")
+    out = net.permute_forcing(q)
+    code = (1 + torch.arange(R, device="cuda", dtype=torch.float32))[None, :]
+    at = net.permute_forcing(code)[0]
+    valid = at > 0
+    reach = (at[valid] - 1).long()
+    for k in (0, 1, 8 * 8191, (1 << 24) - 1, T - 4, T - 1):
+        assert torch.equal(out[k][valid], q[k][reach]), k
+        assert torch.all(out[k][~valid] == 0), k
+    del q, out
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("form", [NAT.FORM_PERSISTENT, NAT.FORM_PLAIN])
 def test_schedule_order_forcing_identical(form):
     """Window calls on a DDRS_WS_SCHEDULE_FORCING workspace, reading q' laid
