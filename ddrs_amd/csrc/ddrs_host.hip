@@ -134,6 +134,7 @@ struct ddrs_workspace {
     // (schedule order, -1: none), reach of each slot, a dense [R] row for the
     // last step's dL/dQ
     int gK = 0;
+    int gcap = 0;  // capacity of greach in slots
     int* gslot = nullptr;
     int* greach = nullptr;
     float* gdense = nullptr;
@@ -149,6 +150,16 @@ ddrs_status dalloc(ddrs_workspace* ws, void** p, size_t bytes) {
     ws->dev.push_back(*p);
     ws->bytes += bytes;
     return DDRS_OK;
+}
+
+// Frees one buffer of dalloc (bytes: its requested size) before the workspace
+// is destroyed. hipFree waits for the device to finish work that may use it.
+void dfree(ddrs_workspace* ws, void* p, size_t bytes) {
+    auto it = std::find(ws->dev.begin(), ws->dev.end(), p);
+    if (it == ws->dev.end()) return;
+    ws->dev.erase(it);
+    ws->bytes -= bytes;
+    hipFree(p);
 }
 
 int variant(const ddrs_params* p) { return p->ddr_match ? 1 : (p->enforce_positivity ? 2 : 0); }
@@ -1306,11 +1317,15 @@ ddrs_status ddrs_workspace_set_gauges(ddrs_workspace_t ws, int32_t K, const int3
         st = dalloc(ws, reinterpret_cast<void**>(&ws->gdense), static_cast<size_t>(R) * 4);
         if (st != DDRS_OK) return st;
     }
-    if (K > ws->gK || !ws->greach) {
-        // the old list, if any, stays allocated with the workspace
+    if (K > ws->gcap || !ws->greach) {
+        // grow the reach list; a smaller list reuses the buffer it fits in
+        if (ws->greach) dfree(ws, ws->greach, static_cast<size_t>(ws->gcap) * 4);
+        ws->greach = nullptr;
+        ws->gcap = 0;
         ddrs_status st = dalloc(ws, reinterpret_cast<void**>(&ws->greach),
                                 static_cast<size_t>(std::max(K, 1)) * 4);
         if (st != DDRS_OK) return st;
+        ws->gcap = std::max(K, 1);
     }
     std::vector<int> slot(P, -1);
     for (int p = 0; p < P; ++p)

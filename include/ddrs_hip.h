@@ -213,7 +213,10 @@ ddrs_status ddrs_mc_window_adj(ddrs_workspace_t ws, const ddrs_params* params, i
  *     returns what ddrs_mc_window_adj returns for the dense dL/dQ that is zero
  *     elsewhere (bit-identical).
  * The full discharge is then neither written by the forward nor permuted by
- * the adjoint. */
+ * the adjoint. The reach list keeps its device buffer while later lists fit
+ * in it and moves to a larger one (the old one freed) when a list outgrows
+ * it, so a HIP graph captured from _gauged calls is valid only until the next
+ * ddrs_workspace_set_gauges on that workspace. */
 ddrs_status ddrs_workspace_set_gauges(ddrs_workspace_t ws, int32_t K, const int32_t* h_reaches);
 ddrs_status ddrs_mc_window_fwd_gauged(ddrs_workspace_t ws, const ddrs_params* params, int32_t T,
                                       const float* d_q_prime, const float* d_q0, float* d_Qg_out,
