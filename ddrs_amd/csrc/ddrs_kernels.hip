@@ -417,58 +417,90 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
     uint64_t lbound = ~0ull;
     int lall = Wd::kFast;
 
-    // S1..S25 for window step s: coefficients and right-hand side.
-    auto prepare = [&](int s, bool first) {
-        bool ok_diag = true;
-        if (first) {
-            // The first step's loads are waited for here, inside its branch:
-            // a register still loading where the branch rejoins the later
-            // steps' path would make the compiler wait, at its uses in every
-            // step, for every operation in flight.
+    // The chunk's step-invariant data, before the first step: statics into
+    // registers, links, gauge slots, positions and per-sub-level link bounds
+    // into LDS, and q' of the first step (sqp). Every load is waited for here,
+    // so that no register is still loading when the step loop starts (the
+    // compiler would otherwise wait, at its uses in every step, for every
+    // operation in flight), and the pointers used only here are dead inside
+    // the loop.
+    {
+        const int t = a.t0 + a.s_base;
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+            const int p = p0 + r * 64 + lane;
+            rs[r] = load_reach(a.st, p);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                row_rsrc(a.qps + static_cast<size_t>(t) * P, P * 4ull), &sqp[r * 64], 4, p * 4,
+                0, 0, 0);
+        }
+        stage_links<RPL>(c, base, base + nlinks, sc.up_pos, sc.up_adj, spos, sadj, skind);
+        if (lane < nrep) srep[lane] = a.ch.rep_up[rep0 + lane];
+        if (a.Qg) {
+            int any = 0;
 #pragma unroll
             for (int r = 0; r < RPL; ++r) {
-                const int p = p0 + r * 64 + lane;
-                rs[r] = load_reach(a.st, p);
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                    row_rsrc(a.qps + static_cast<size_t>(a.t0 + a.s_base + s) * P, P * 4ull),
-                    &sqp[r * 64], 4, p * 4, 0, 0, 0);
+                const int sl = a.gslot[p0 + r * 64 + lane];
+                pgs[r * 64 + lane] = sl;
+                any |= sl >= 0;
             }
-            stage_links<RPL>(c, base, base + nlinks, sc.up_pos, sc.up_adj, spos, sadj, skind);
-            if (lane < nrep) srep[lane] = a.ch.rep_up[rep0 + lane];
-            if (a.Qg) {
-                int any = 0;
-#pragma unroll
-                for (int r = 0; r < RPL; ++r) {
-                    const int sl = a.gslot[p0 + r * 64 + lane];
-                    pgs[r * 64 + lane] = sl;
-                    any |= sl >= 0;
-                }
-                hg = __any(any) != 0;
-            }
-            wait_vm<0>();
+            hg = __any(any) != 0;
         }
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+            const int j = r * 64 + lane;
+            const int p = p0 + j;
+            const int orig = sc.perm[p];
+            porig[j] = orig;
+            psub[j] = -1;
+            if (orig >= 0) {
+                const int ea = sc.up_ptr[p], eb = sc.up_ptr[p + 1];
+                pe0[j] = static_cast<short>(ea - base);
+                pe1[j] = static_cast<short>(eb - base);
+                psub[j] = a.ch.sub[p];
+                sq[j] = a.st_q[p];
+                si[j] = a.st_i[p];
+#pragma unroll
+                for (int k = 0; k < Wd::kFast; ++k) {
+                    if (ea + k < eb)
+                        lkc[k][j] = static_cast<short>(
+                            link_index<RPL>(c, base, ea + k, sc.up_pos[ea + k]));
+                }
+            }
+        }
+        wait_vm<0>();
+        // per sub-level link bounds, for the solve's read count
+        if (lane < kLinkBoundSubs) slm[lane] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+            const int j = r * 64 + lane;
+            const int l = psub[j];
+            if (l >= 0 && l < kLinkBoundSubs) atomicMax(&slm[l], min(pe1[j] - pe0[j], Wd::kFast));
+        }
+        __syncthreads();
+        uint64_t m = 0;
+        int mx = 0;
+        for (int l = 0; l < kLinkBoundSubs; ++l) {
+            const int v = slm[l];
+            m |= static_cast<uint64_t>(v) << (4 * l);
+            mx = max(mx, v);
+        }
+        lbound = (static_cast<uint64_t>(uni(static_cast<int>(m >> 32))) << 32) |
+                 static_cast<uint32_t>(uni(static_cast<int>(m)));
+        // sub-levels beyond the kept ones read every register-held link
+        lall = (nsub > kLinkBoundSubs) ? Wd::kFast : uni(mx);
+    }
+
+    // S1..S25 for window step s: coefficients and right-hand side.
+    auto prepare = [&](int s, bool) {
+        bool ok_diag = true;
         const int t = a.t0 + a.s_base + s;
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
             const int j = r * 64 + lane;
             const int p = p0 + j;
-            if (first) porig[j] = sc.perm[p];
-            const int orig = porig[j];
-            if (orig >= 0) {
-                if (first) {
-                    const int ea = sc.up_ptr[p], eb = sc.up_ptr[p + 1];
-                    pe0[j] = static_cast<short>(ea - base);
-                    pe1[j] = static_cast<short>(eb - base);
-                    psub[j] = a.ch.sub[p];
-                    sq[j] = a.st_q[p];
-                    si[j] = a.st_i[p];
-#pragma unroll
-                    for (int k = 0; k < Wd::kFast; ++k) {
-                        if (ea + k < eb)
-                            lkc[k][j] = static_cast<short>(
-                                link_index<RPL>(c, base, ea + k, sc.up_pos[ea + k]));
-                    }
-                }
+            if (porig[j] >= 0) {
                 const float q = sq[j];
                 const float it = si[j];
                 const float qp = fmaxr(lds_read_dma4(&sqp[j]), a.P.q_lb);
@@ -482,34 +514,9 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
                 pb[j] = ((co.c2 * it) + (co.c3 * q)) + (co.c4 * qp);             // S25
                 pnc1[j] = -co.c1;
                 ok_diag = ok_diag && (1.0f + (-co.c1) * 0.0f == 1.0f);
-            } else if (first) {
-                psub[j] = -1;
             }
         }
         unit = __all(ok_diag);
-        if (first) {
-            // per sub-level link bounds, for the solve's read count
-            if (lane < kLinkBoundSubs) slm[lane] = 0;
-            __syncthreads();
-#pragma unroll
-            for (int r = 0; r < RPL; ++r) {
-                const int j = r * 64 + lane;
-                const int l = psub[j];
-                if (l >= 0 && l < kLinkBoundSubs) atomicMax(&slm[l], min(pe1[j] - pe0[j], Wd::kFast));
-            }
-            __syncthreads();
-            uint64_t m = 0;
-            int mx = 0;
-            for (int l = 0; l < kLinkBoundSubs; ++l) {
-                const int v = slm[l];
-                m |= static_cast<uint64_t>(v) << (4 * l);
-                mx = max(mx, v);
-            }
-            lbound = (static_cast<uint64_t>(uni(static_cast<int>(m >> 32))) << 32) |
-                     static_cast<uint32_t>(uni(static_cast<int>(m)));
-            // sub-levels beyond the kept ones read every register-held link
-            lall = (nsub > kLinkBoundSubs) ? Wd::kFast : uni(mx);
-        }
     };
 
     // every staged link is polled before prepare, so that the check after it
@@ -634,7 +641,6 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
         for (int r = 0; r < RPL; ++r) {
             float in = 0.0f, sx = 0.0f;
             if (my[r] >= 0) {
-                const int j = r * 64 + lane;
                 float v[Wd::kFast], wk[Wd::kFast];
 #pragma unroll
                 for (int k = 0; k < Wd::kFast; ++k) {
