@@ -173,6 +173,7 @@ __device__ __forceinline__ void poll_pair_to_lds(const void* g, void* lds) {
 // 100 MHz wall clock
 __device__ __forceinline__ uint64_t wall_ticks() { return __builtin_amdgcn_s_memrealtime(); }
 constexpr uint64_t kWatchdogTicks = 200000000ull;  // 2 s at 100 MHz
+constexpr uint64_t kErrCheckTicks = 2000ull;        // 20 us at 100 MHz
 
 // ---------------------------------------------------------------------------
 // One chunk per workgroup, for all steps of the launch.
@@ -228,7 +229,7 @@ __global__ void k_begin_launch(LaunchState* ls, const unsigned long long* ctr) {
 // would also wait for the stores to be acknowledged (a round trip to the
 // coherence point), while loads issued before them are waited for alone. The
 // requests also overlap the next step's coefficient arithmetic. A waiting
-// chunk polls only the representatives, after a growing pause (0.2 to 1.7 us),
+// chunk polls only the representatives, after a growing pause (0.2 to 0.8 us),
 // so that waiting chunks do not flood the memory system. All five are called
 // by the whole wave, and the workgroup is that one wave: its LDS operations
 // execute in order, so the phases need no workgroup barrier. Returns early
@@ -264,10 +265,16 @@ __device__ __forceinline__ void chunk_loop(int nsteps, int* err, unsigned long l
             for (;;) {
                 ++missed;
                 for (int k = 0; k < nap; ++k) __builtin_amdgcn_s_sleep(8);
-                nap = (nap < 8) ? 2 * nap : 8;
+                nap = (nap < 4) ? 2 * nap : 4;
+                // The error flag is a device-coherent read, a full memory round
+                // trip: a wait reads it only once it has lasted kErrCheckTicks,
+                // so that the usual short waits spend their round trips on
+                // their inputs alone.
+                const uint64_t waited = wall_ticks() - t0;
                 const bool give_up =
-                    (wall_ticks() - t0 > kWatchdogTicks) ||
-                    __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+                    waited > kErrCheckTicks &&
+                    (waited > kWatchdogTicks ||
+                     __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0);
                 if (give_up) {
                     if (threadIdx.x == 0) atomicExch(err, 1);
                     return;
