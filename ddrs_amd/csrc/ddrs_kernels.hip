@@ -850,7 +850,7 @@ __device__ __forceinline__ void lin_item(const LinArgs& a, int sb, int p) {
         const float qp = fmaxr(qp_raw, lb);
         Coef co;
         PowEval pe;
-        mc_coef<PH>(a.P, rc, q, co, pe);
+        mc_coef<PH, PowEval, true>(a.P, rc, q, co, pe);
         const LocalGrads g = mc_local_bwd<PH, true>(a.P, co, rc, q, qp, it, 1.0f, sx, 0.0f);
         const size_t o = static_cast<size_t>(sg) * P + p;
         a.rec[o] = make_float4(-co.c1, co.c2, g.gq, (!a.mask_lam || q > lb) ? 1.0f : 0.0f);

@@ -71,6 +71,38 @@ __device__ __forceinline__ void make_reach(float n, float qsp, float psp, float 
     out[2] = make_float4(psp * sqrt_s + 1e-8f, (1.0f / f53) * 3.0f, 1.0f / n, f53 * f53);
 }
 
+// Correctly rounded quotients a / b of floats sharing one divisor b, from a
+// reciprocal refined in double: y0 = rcp(b) (1 ulp), two Newton steps give
+// r = 1/b within 2^-53 relative, and (double)a * r is within 2^-52 of a / b.
+// The quotient of two floats lies at least 2^-49 (relative) from any fp32
+// rounding midpoint, so rounding the double once gives the correctly rounded
+// (IEEE) quotient: the same bits as a / b. That holds for a divisor of
+// magnitude in [2^-100, 2^100] and a result that is zero or in the normal
+// range (or overflows); div_ok() tells, per quotient, and the caller divides
+// the IEEE way where it does not hold.
+struct SharedDiv {
+    double r;
+    bool ok_b;
+    __device__ __forceinline__ explicit SharedDiv(float b) {
+        const float ab = fabsf(b);
+        ok_b = ab >= 0x1p-100f && ab <= 0x1p100f;  // false for NaN
+        const double bd = static_cast<double>(b);
+        const double y0 = static_cast<double>(__builtin_amdgcn_rcpf(b));
+        const double e0 = fma(-bd, y0, 1.0);
+        const double y1 = fma(e0, y0, y0);
+        const double e1 = fma(-bd, y1, 1.0);
+        r = fma(e1, y1, y1);
+    }
+    __device__ __forceinline__ float div(float a) const {
+        return static_cast<float>(static_cast<double>(a) * r);
+    }
+    __device__ __forceinline__ float inv() const { return static_cast<float>(r); }
+    // q is zero, normal, infinite (overflow): not subnormal and not NaN
+    static __device__ __forceinline__ bool div_ok(float q) {
+        return fabsf(q) >= 0x1p-126f || q == 0.0f;
+    }
+};
+
 struct Coef {
     float ratio, ln_ratio, depth, ln_depth, tw, ss_raw, ss, bw_raw, bw, hyd_r;
     float v_un, v_cl, cel, k_musk, x_eff, denom, c1, c2, c3, c4;
@@ -128,7 +160,9 @@ struct PowEvalInl {
 
 // S1..S23 for out[t] = q. ln_ratio and ln_depth are log_f32 of ratio and depth
 // (needed by B6 and B7), taken from the same ln evaluation as the powers.
-template <int PH, class Pow>
+// SD: the four quotients over S20's denominator from one shared reciprocal
+// (SharedDiv, bit-identical to the divisions); off, each is an IEEE division.
+template <int PH, class Pow, bool SD = false>
 __device__ __forceinline__ void mc_coef(const Phys& P, const Reach& r, float q, Coef& o,
                                         Pow& pw) {
     float numerator = (q * r.n) * r.qe1;                                // S2
@@ -176,10 +210,30 @@ __device__ __forceinline__ void mc_coef(const Phys& P, const Reach& r, float q, 
     float two_kx = two_k * o.x_eff;
     float two_k_1mx = two_k * one_minus_x;
     o.denom = two_k_1mx + P.dt;
-    o.c1 = ((-two_kx) + P.dt) / o.denom;
-    o.c2 = (two_kx + P.dt) / o.denom;
-    o.c3 = (two_k_1mx - P.dt) / o.denom;
-    o.c4 = (1.0f / o.denom) * (2.0f * P.dt);
+    const float n1 = (-two_kx) + P.dt;
+    const float n2 = two_kx + P.dt;
+    const float n3 = two_k_1mx - P.dt;
+    float c1, c2, c3, inv;
+    bool ieee = true;
+    if (SD) {
+        const SharedDiv sd(o.denom);
+        c1 = sd.div(n1);
+        c2 = sd.div(n2);
+        c3 = sd.div(n3);
+        inv = sd.inv();
+        ieee = !(sd.ok_b && SharedDiv::div_ok(c1) && SharedDiv::div_ok(c2) &&
+                 SharedDiv::div_ok(c3) && SharedDiv::div_ok(inv));
+    }
+    if (ieee) {
+        c1 = n1 / o.denom;
+        c2 = n2 / o.denom;
+        c3 = n3 / o.denom;
+        inv = 1.0f / o.denom;
+    }
+    o.c1 = c1;
+    o.c2 = c2;
+    o.c3 = c3;
+    o.c4 = inv * (2.0f * P.dt);
 }
 
 struct LocalGrads {

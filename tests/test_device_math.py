@@ -101,3 +101,83 @@ def test_device_pow_log_match_libm():
     # either, and x*x disagrees with it no more often than pow_f32 does
     print(f"side slope square vs powf(x,2): x*x {sq_libm}, pow_f32 {sq_pow} differ")
     assert sq_libm <= 1.1 * sq_pow + 50
+
+
+SHARED_DIV_SRC = r'''
+#include <cmath>
+#include <cstdio>
+#include <cstdint>
+static uint64_t st = 0x9E3779B97F4A7C15ull;
+static uint64_t nx() { st ^= st << 13; st ^= st >> 7; st ^= st << 17; return st; }
+static double u() { return (nx() >> 11) * (1.0 / 9007199254740992.0); }
+/* The steps of SharedDiv (mc_physics.h) with a given first reciprocal y0: two
+ * Newton steps in double, then one product per quotient and one rounding. */
+static double refine(float b, float y0f) {
+  const double bd = b, y0 = y0f;
+  const double e0 = std::fma(-bd, y0, 1.0);
+  const double y1 = std::fma(e0, y0, y0);
+  const double e1 = std::fma(-bd, y1, 1.0);
+  return std::fma(e1, y1, y1);
+}
+static bool ok_q(float q) { return std::fabs(q) >= 0x1p-126f || q == 0.0f; }
+int main() {
+  long n = 0, bad = 0, skipped = 0, near = 0;
+  for (long k = 0; k < 4000000; ++k) {
+    float a, b;
+    const int m = k % 4;
+    if (m == 0) {        /* wide random magnitudes */
+      a = (float)(std::exp(-40.0 + 80.0 * u()) * (u() < 0.5 ? -1 : 1));
+      b = (float)(std::exp(-60.0 + 120.0 * u()) * (u() < 0.5 ? -1 : 1));
+    } else if (m == 1) { /* the routing's ranges: numerators near dt, denominators >= dt */
+      a = (float)(3600.0 * (2.0 * u() - 0.5));
+      b = (float)(3600.0 + std::exp(std::log(1.0) + 20.0 * u()));
+    } else {             /* quotients next to an fp32 rounding midpoint: A/B ~ M/2^25 */
+      const uint32_t M = (uint32_t)((nx() >> 40) | (1u << 24)) | 1u; /* odd, 25 bits */
+      const uint32_t B = (uint32_t)((nx() >> 41) | (1u << 23));      /* 24 bits */
+      const double A = std::nearbyint((double)M * (double)B / 33554432.0);
+      if (A >= 16777216.0 || A < 8388608.0) { --k; continue; }
+      a = std::ldexp((float)A, (int)(nx() % 40) - 20);
+      b = std::ldexp((float)B, (int)(nx() % 40) - 20);
+      ++near;
+    }
+    const float ieee = a / b, inv = 1.0f / b;
+    const float ab = std::fabs(b);
+    if (!(ab >= 0x1p-100f && ab <= 0x1p100f)) { ++skipped; continue; }
+    /* every first reciprocal within one ulp of 1/b */
+    const float y0s[3] = {std::nextafter(inv, -INFINITY), inv, std::nextafter(inv, INFINITY)};
+    for (float y0 : y0s) {
+      const double r = refine(b, y0);
+      const float q = (float)((double)a * r), qi = (float)r;
+      if (!ok_q(q) || !ok_q(qi)) { ++skipped; continue; }
+      ++n;
+      if (q != ieee || qi != inv) {
+        if (bad < 5) printf("# %a / %a: %a vs %a\n", a, b, q, ieee);
+        ++bad;
+      }
+    }
+  }
+  printf("%ld %ld %ld %ld\n", n, bad, skipped, near);
+  return 0;
+}
+'''
+
+
+def test_shared_divisor_quotients_are_ieee():
+    """SharedDiv (mc_physics.h), which the linearisation uses for the four
+    quotients over S20's denominator: from any first reciprocal within one ulp
+    of 1/b (the hardware reciprocal's bound), two Newton steps in double and
+    one product give the IEEE fp32 quotient bit for bit, because the quotient
+    of two floats lies at least 2^-49 from a rounding midpoint. Checked on
+    random and routing-range operands and on quotients built next to a
+    midpoint (the hardest case)."""
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "t.cpp")
+        exe = os.path.join(d, "t")
+        open(src, "w").write(SHARED_DIV_SRC)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", src, "-o", exe, "-lm"],
+                       check=True)
+        raw = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    print(raw)
+    n, bad, skipped, near = (int(v) for v in raw.split("\n")[-2].split())
+    assert n > 10_000_000 and near > 1_000_000
+    assert bad == 0
