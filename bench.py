@@ -825,11 +825,31 @@ def cpu_baseline(wl, steps):
     O.route_adj(P, pat, at["n"], at["q_spatial"], p, at["length"], at["slope"], qp, gQ, q0=q0)
     dt = time.perf_counter() - t0
     log(f"[bench] cpu oracle: {dt:.1f}s for {R}x{steps}")
-    return {"value": R * steps / dt, "unit": "reach-timesteps/s", "cores": 1, "kind": "port",
-            "sample": f"same {R}-reach network and forcing, {steps} timesteps; oracle "
-                      f"route_adj (forward, per-step recompute and adjoint), fp32, one thread "
-                      f"({dt:.1f} s)",
-            "cpu_model": _cpu_model(), "host_cores": os.cpu_count()}
+    out = {"value": R * steps / dt, "unit": "reach-timesteps/s", "cores": 1, "kind": "port",
+           "sample": f"same {R}-reach network and forcing, {steps} timesteps; oracle "
+                     f"route_adj (forward, per-step recompute and adjoint), fp32, one thread "
+                     f"({dt:.1f} s)",
+           "cpu_model": _cpu_model(), "host_cores": os.cpu_count()}
+    # the same sample with the per-reach loops of each step on every core the
+    # job may use (the triangular solves and the scatters stay serial, as in
+    # the reference); the reference's own CPU path is single-threaded
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    if threads > 1:
+        O.set_threads(threads)
+        try:
+            t0 = time.perf_counter()
+            O.route_adj(P, pat, at["n"], at["q_spatial"], p, at["length"], at["slope"], qp, gQ,
+                        q0=q0)
+            dtp = time.perf_counter() - t0
+        finally:
+            O.set_threads(1)
+        log(f"[bench] cpu oracle, {threads} threads: {dtp:.1f}s")
+        out["parallel"] = {"value": R * steps / dtp, "unit": "reach-timesteps/s",
+                           "cores": threads, "kind": "port",
+                           "sample": "the same sample with each step's per-reach loops "
+                                     f"(coefficients, local backward) on {threads} OpenMP "
+                                     f"threads; solves and scatters serial ({dtp:.1f} s)"}
+    return out
 
 
 def _cpu_model():

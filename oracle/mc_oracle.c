@@ -9,6 +9,12 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Threads of the per-reach loops of orc_step_fwd / orc_step_bwd (1: serial,
+ * the default). Each reach's values there are computed independently, so the
+ * results do not depend on it; only the CPU baseline's parallel leg sets it. */
+int orc_threads = 1;
+void orc_set_threads(int n) { orc_threads = (n > 1) ? n : 1; }
+
 /* powf_scalar(2.0) of the reference (S13, S17) is the exact square rounded
  * once, written x * x. (gcc emits x * x for powf(x, 2.0f) as well; a run-time
  * glibc powf(x, 2) is not correctly rounded on ~0.07% of x in [0.5, 50].) */

@@ -59,6 +59,11 @@ enum {
     ORC_NSAVE
 };
 
+/* Threads of the per-reach loops of the step functions (1, the default:
+ * serial; results do not depend on it). For the CPU baseline's parallel leg. */
+extern int orc_threads;
+void orc_set_threads(int n);
+
 /* ---- CsrPattern construction (sparse/mod.rs:173-304). Integer work, one
  * variant. Outputs sized n+1 / nnz_off+n as in the reference. Returns 0, or
  * -1 if an entry is not strictly lower triangular. */

@@ -57,6 +57,12 @@ def lib() -> ctypes.CDLL:
     return _LIBS[name]
 
 
+def set_threads(n: int) -> None:
+    """Threads of the per-reach loops of the oracle's step functions (1: serial,
+    the default). Results do not depend on it."""
+    lib().orc_set_threads(int(n))
+
+
 def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
 

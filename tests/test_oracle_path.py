@@ -148,3 +148,28 @@ def test_pattern_build_is_thread_safe():
             for a, b in zip(serial, par):
                 for f in ("crow", "col", "row_for_nnz", "adj_values", "trans_col", "trans_to_orig"):
                     assert np.array_equal(getattr(a, f), getattr(b, f)), f
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_threaded_oracle_is_bit_identical(dtype):
+    """The oracle's per-reach loops on several threads (the CPU baseline's
+    parallel leg) give the serial oracle's bits: each reach's values are
+    computed on their own, and the solves and scatters stay serial."""
+    R, T = 3000, 30
+    rows, cols = NW.gen_tree(R, 64, 1301)
+    at = NW.gen_attributes(R, 2301)
+    qp = NW.gen_forcing(R, T, 3301, storms=True)
+    P = O.make_params()
+    pat = O.pattern_from_coo(R, rows, cols)
+    p = np.full(R, 21.0, np.float32)
+    gQ = np.random.default_rng(5).normal(0, 1, (T, R)).astype(np.float32)
+    runs = []
+    for threads in (1, 4):
+        O.set_threads(threads)
+        try:
+            runs.append(O.route_adj(P, pat, at["n"], at["q_spatial"], p, at["length"],
+                                    at["slope"], qp, gQ, dtype=dtype))
+        finally:
+            O.set_threads(1)
+    for k in runs[0]:
+        np.testing.assert_array_equal(runs[1][k], runs[0][k], err_msg=k)
