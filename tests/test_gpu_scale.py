@@ -41,7 +41,8 @@ reciprocal) and within 1e-3 max|g| of the f32 oracle, or twice the spread
 between the two oracles where libm's rounding moves that one further, with an
 FP64 error at most 3x the f32 oracle's own. Beside the norm bar, the element-wise
 relative error on the well-conditioned entries (|g| >= 1e-2 max|g|) is
-reported. The finite differences (oracle/scale_check.py: the step chosen per
+reported, and against the device-math oracle it is held within 5e-3 (a
+systematic relative error on small but clean gradients fails it). The finite differences (oracle/scale_check.py: the step chosen per
 reach from five, by the agreement of adjacent steps): on every reach they
 resolve (two steps agreeing within 1e-3; at least 40 of 64) the f64 adjoint
 within 2e-3 of them, which pins the differences, and each GPU gradient within
@@ -116,6 +117,11 @@ def assert_grad(label, e):
     assert e["norm_err_vs_f32_oracle"] <= max(
         1e-3, 2.0 * e["f32_oracle_vs_device_math_oracle_norm"] + 1e-4), (label, e)
     assert e["norm_err_vs_device_math_oracle"] <= 1e-4, (label, e)
+    # element-wise on the well-conditioned entries (|g| >= 1e-2 max|g|): the
+    # adjoint's gradient-only divisions use the hardware reciprocal, and the
+    # per-step terms of an entry sum over thousands of steps (measured at most
+    # 1.1e-3 over every compared reach of c2-c5)
+    assert e["elem_max_rel_err_vs_device_math_oracle"] <= 5e-3, (label, e)
     assert e["norm_err_vs_fp64"] <= 3.0 * e["f32_oracle_norm_err_vs_fp64"] + 1e-6, (label, e)
 
 
