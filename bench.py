@@ -359,7 +359,7 @@ def kernel_source_hash() -> str:
     """sha256 (16 hex digits) of the window kernels' sources."""
     import hashlib
     h = hashlib.sha256()
-    for f in ("ddrs_kernels.hip", "mc_physics.h", "mc_math.h", "ddrs_internal.h"):
+    for f in ("ddrs_kernels.hip", "mc_physics.h", "mc_math.h", "mc_ln_table.h", "ddrs_internal.h"):
         with open(os.path.join(ROOT, "ddrs_amd", "csrc", f), "rb") as fh:
             h.update(fh.read())
     return h.hexdigest()[:16]

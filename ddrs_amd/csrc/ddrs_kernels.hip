@@ -392,6 +392,7 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
     __shared__ short srep[kRepMax];
     __shared__ short lkc[Wd::kFast][CW];  // link_index (< CW + kStageUp)
     __shared__ int slm[kLinkBoundSubs];
+    __shared__ LnEnt sln[kLnEntries];  // the logarithm table of the powers
     __shared__ int claimed;
     const int lane = threadIdx.x;
     const Sched& sc = a.sc;
@@ -475,6 +476,7 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
                 }
             }
         }
+        if (lane < kLnEntries) sln[lane] = kLnTab[lane];
         wait_vm<0>();
         // per sub-level link bounds, for the solve's read count
         if (lane < kLinkBoundSubs) slm[lane] = 0;
@@ -516,7 +518,7 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
                         row_rsrc(a.qps + static_cast<size_t>(t + 1) * P, P * 4ull), &sqp[r * 64], 4,
                         p * 4, 0, 0, 0);
                 Coef co;
-                PowEvalInl pe;
+                PowEvalInl pe(sln);
                 mc_coef<PH>(a.P, rs[r], q, co, pe);
                 pb[j] = ((co.c2 * it) + (co.c3 * q)) + (co.c4 * qp);             // S25
                 pnc1[j] = -co.c1;
@@ -743,7 +745,7 @@ constexpr uint64_t kClaimStallTicks = 5000;  // 50 us at 100 MHz
 // writes the history may still run; history words are read with coherent
 // loads and checked for its epoch (awaited when missing).
 template <int PH, bool Coh>
-__device__ __forceinline__ void lin_item(const LinArgs& a, int sb, int p) {
+__device__ __forceinline__ void lin_item(const LinArgs& a, int sb, int p, const LnEnt* lnt) {
     const Sched& sc = a.sc;
     const int P = sc.P;
     if (p >= P) return;
@@ -849,8 +851,8 @@ __device__ __forceinline__ void lin_item(const LinArgs& a, int sb, int p) {
         const float qp_raw = a.qps[static_cast<size_t>(t) * P + p];
         const float qp = fmaxr(qp_raw, lb);
         Coef co;
-        PowEval pe;
-        mc_coef<PH, PowEval, true>(a.P, rc, q, co, pe);
+        PowEvalInl pe(lnt);
+        mc_coef<PH, PowEvalInl, true>(a.P, rc, q, co, pe);
         const LocalGrads g = mc_local_bwd<PH, true>(a.P, co, rc, q, qp, it, 1.0f, sx, 0.0f);
         const size_t o = static_cast<size_t>(sg) * P + p;
         a.rec[o] = make_float4(-co.c1, co.c2, g.gq, (!a.mask_lam || q > lb) ? 1.0f : 0.0f);
@@ -862,9 +864,12 @@ __device__ __forceinline__ void lin_item(const LinArgs& a, int sb, int p) {
 
 // One workgroup per item: grid x = block of kLinBlock positions, y = step block.
 template <int PH>
-__global__ __launch_bounds__(kLinBlock) void k_adj_lin(LinArgs a) {
+__global__ __launch_bounds__(kLinBlock, 6) void k_adj_lin(LinArgs a) {
+    __shared__ LnEnt sln[kLnEntries];  // the logarithm table of the powers
+    if (threadIdx.x < kLnEntries) sln[threadIdx.x] = kLnTab[threadIdx.x];
+    __syncthreads();
     lin_item<PH, false>(a, static_cast<int>(blockIdx.y),
-                        static_cast<int>(blockIdx.x) * kLinBlock + static_cast<int>(threadIdx.x));
+                        static_cast<int>(blockIdx.x) * kLinBlock + static_cast<int>(threadIdx.x), sln);
 }
 
 // One-wave workgroups taking items of kLinItem positions from the counter, in
@@ -872,6 +877,9 @@ __global__ __launch_bounds__(kLinBlock) void k_adj_lin(LinArgs a) {
 template <int PH, bool Coh>
 __global__ __launch_bounds__(kLinItem) void k_adj_lin_items(LinArgs a) {
     const int npb = (a.sc.P + kLinItem - 1) / kLinItem;
+    __shared__ LnEnt sln[kLnEntries];  // the logarithm table of the powers
+    if (threadIdx.x < kLnEntries) sln[threadIdx.x] = kLnTab[threadIdx.x];
+    __syncthreads();
     if constexpr (Coh) {
         // wait for the forward's claims while they advance
         const auto* ctr = const_cast<unsigned long long*>(a.fwd_claim_ctr);
@@ -900,7 +908,7 @@ __global__ __launch_bounds__(kLinItem) void k_adj_lin_items(LinArgs a) {
         k = __shfl(k, 0);
         if (k < 0) break;
         const uint64_t t_take = a.trace ? wall_ticks() : 0;
-        lin_item<PH, Coh>(a, k / npb, (k % npb) * kLinItem + static_cast<int>(threadIdx.x));
+        lin_item<PH, Coh>(a, k / npb, (k % npb) * kLinItem + static_cast<int>(threadIdx.x), sln);
         if (a.trace) {
             __builtin_amdgcn_wave_barrier();
             if (threadIdx.x == 0) {

@@ -138,22 +138,25 @@ struct PowEval {
 
 // The same evaluations inlined: no call, so the caller's outstanding memory
 // operations are not waited for at each power (a call waits for all of them).
+// T: the logarithm table, kLnTab or a copy of it in LDS.
 struct PowEvalInl {
     PowVals v;
+    const LnEnt* T;
+    __device__ explicit PowEvalInl(const LnEnt* tab) : T(tab) {}
     __device__ __forceinline__ PowLn p5(float x, float y) {
-        const PowLn r = pow_ln_f32_inl(x, y);
+        const PowLn r = pow_ln_f32_inl(x, y, T);
         v.pw5 = r.pw;
         v.ln5 = r.ln;
         return r;
     }
     __device__ __forceinline__ PowLn p7(float x, float y) {
-        const PowLn r = pow_ln_f32_inl(x, y);
+        const PowLn r = pow_ln_f32_inl(x, y, T);
         v.pw7 = r.pw;
         v.ln7 = r.ln;
         return r;
     }
     __device__ __forceinline__ float p15(float x) {
-        v.pw15 = pow_f32_inl(x, 2.0f / 3.0f);
+        v.pw15 = pow_f32_inl(x, 2.0f / 3.0f, T);
         return v.pw15;
     }
 };

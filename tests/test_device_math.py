@@ -181,3 +181,82 @@ def test_shared_divisor_quotients_are_ieee():
     n, bad, skipped, near = (int(v) for v in raw.split("\n")[-2].split())
     assert n > 10_000_000 and near > 1_000_000
     assert bad == 0
+
+
+LN_EDGE_SRC = r'''
+#include <cmath>
+#include <cstdio>
+#include <cstdint>
+#include <cstring>
+#include "mc_math.h"
+static float fb(uint32_t u) { float x; memcpy(&x, &u, 4); return x; }
+static long n = 0, bad = 0;
+static double worst = 0.0;  /* of ln_tab's double result, relative to logl */
+static void chk(float x) {
+  if (!(x > 0.0f) || !std::isfinite(x)) return;
+  if (x >= 0x1p-126f && x != 1.0f) {
+    const long double ex = logl((long double)x);
+    const double rel = (double)fabsl(((long double)ddrs::ln_tab(x, ddrs::kLnTab) - ex) / ex);
+    if (rel > worst) worst = rel;
+  }
+  ++n;
+  const float a = ddrs::log_f32(x), c = (float)logl((long double)x);
+  if (a != c) { if (bad < 5) printf("# log %a: %a vs %a\n", x, a, c); ++bad; }
+  /* pow through the same logarithm: y = 0.3 and 2/3 against powl */
+  const float ys[2] = {0.3f, 2.0f / 3.0f};
+  for (float y : ys) {
+    ++n;
+    const float p = ddrs::pow_f32(x, y), q = (float)powl((long double)x, (long double)y);
+    if (p != q) { if (bad < 5) printf("# pow %a %a: %a vs %a\n", x, y, p, q); ++bad; }
+  }
+}
+int main() {
+  /* 1 +- k ulp: the entry that holds 1.0 keeps relative accuracy */
+  for (uint32_t k = 0; k < 200000; ++k) { chk(fb(0x3f800000u + k)); chk(fb(0x3f800000u - k)); }
+  /* every boundary of the 32 table intervals, +-64 ulp, in 24 binades */
+  for (int e = -12; e < 12; ++e)
+    for (uint32_t i = 0; i <= 32; ++i) {
+      const uint32_t b = ddrs::kLnOff + (i << (23 - ddrs::kLnBits)) + ((uint32_t)e << 23);
+      for (int d = -64; d <= 64; ++d) chk(fb(b + (uint32_t)d));
+    }
+  /* subnormal operands (series logarithm) and the smallest normals */
+  for (uint32_t u = 1; u < 0x01000000u; u += 4099) chk(fb(u));
+  /* the whole positive range, 2^23 steps */
+  for (uint32_t u = 0x00800000u; u < 0x7f800000u; u += 251) chk(fb(u));
+  printf("%.3e\n%ld %ld\n", worst, n, bad);
+  return 0;
+}
+'''
+
+
+def test_ln_table_is_generated_and_edges_round_correctly():
+    """The logarithm's reduction table (mc_ln_table.h) is what gen_ln_table.py
+    writes, and log_f32 / pow_f32 stay correctly rounded where a table method is
+    most exposed: x within 200,000 ulp of 1 (relative accuracy of ln near its
+    zero), both sides of every table interval boundary in 24 binades,
+    subnormal x and a sweep of the whole normal range. Against long-double
+    logl/powl; the double logarithm itself within 2^-50 relative (a wrong
+    polynomial term or table entry shows there before it shows in fp32)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "gen_ln_table", os.path.join(ROOT, "ddrs_amd", "csrc", "gen_ln_table.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    with open(os.path.join(ROOT, "ddrs_amd", "csrc", "mc_ln_table.h")) as f:
+        assert f.read() == gen.render()
+    invc, logc = zip(*gen.entries())
+    assert invc[18] == 1.0 and logc[18] == 0.0
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "t.cpp")
+        exe = os.path.join(d, "t")
+        open(src, "w").write(LN_EDGE_SRC)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I",
+                        os.path.join(ROOT, "ddrs_amd", "csrc"), src, "-o", exe, "-lm"], check=True)
+        raw = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    print(raw)
+    lines = raw.split("\n")
+    worst = float(lines[-3])
+    n, bad = (int(v) for v in lines[-2].split())
+    assert n > 1_000_000
+    assert bad <= 2  # a long double rounded again to fp32 can miss a near-tie
+    assert worst <= 2.0 ** -50
