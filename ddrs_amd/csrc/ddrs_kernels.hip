@@ -983,10 +983,10 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
     // read no gauge-sparse dL/dQ
     unsigned grows = 0;
 
-    // Step-invariant data only: the per-step work of the sweep is in issue
-    // and solve.
-    auto prepare = [&](int, bool first) {
-        if (!first) return;
+    // Step-invariant data, before the step loop (the per-step work of the
+    // sweep is in issue and solve): the pointers used only here are dead
+    // inside the loop.
+    {
         stage_links<RPL>(c, base, base + nlinks, sc.dn_pos, sc.dn_adj, spos, sadj, skind);
         if (lane < nrep) srep[lane] = a.ch.rep_dn[rep0 + lane];
         if (a.gK) {
@@ -1019,7 +1019,9 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
                 psub[j] = -1;
             }
         }
-    };
+        wave_lds_sync();
+    }
+    auto prepare = [&](int, bool) {};
 
     // Every lane loads, whether or not it needs the value (an empty position
     // reads its unused record slot, a lane without a cross-chunk link the
@@ -1103,7 +1105,9 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
             const u32x4 q = lds_read_dma16(&prec[j]);
             rcv[r] = make_float4(__uint_as_float(q.x), __uint_as_float(q.y), __uint_as_float(q.z),
                                  __uint_as_float(q.w));
-            dqv[r] = (a.gqs && (!a.gK || ((grows >> r) & 1u))) ? lds_read_dma4(&pdq[j]) : 0.0f;
+            // direct dL/d out[t] of the position (zero where it has no gauge)
+            const bool hasdq = a.gqs && (!a.gK || (((grows >> r) & 1u) && pgs[j] >= 0));
+            dqv[r] = hasdq ? lds_read_dma4(&pdq[j]) : 0.0f;
         }
         {
             bool ok_diag = true;
@@ -1116,13 +1120,14 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
         }
         // per-lane operands of the solve, read once per step
         int my[RPL], e0[RPL], nl[RPL], code[RPL][kFastLinksDn];
-        float gbv[RPL], lamv[RPL], gx[RPL], w[RPL][kFastLinksDn];
+        float gbv[RPL], lamv[RPL], gsum[RPL], gx[RPL], w[RPL][kFastLinksDn];
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
             const int j = r * 64 + lane;
             my[r] = psub[j];
             gbv[r] = 0.0f;
             lamv[r] = 0.0f;
+            gsum[r] = 0.0f;
             gx[r] = slam[j];
             e0[r] = pe0[j];
             nl[r] = pe1[j] - e0[r];
@@ -1167,12 +1172,16 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
                     ldsP[j] = Pm;
                     ldsG[j] = Gm;
                     gbv[r] = gb;
-                    const float dq = (a.gqs && (!a.gK || pgs[j] >= 0)) ? dqv[r] : 0.0f;
-                    lamv[r] = (rc.w != 0.0f) ? dq + (gb * rc.z + sumG) : 0.0f;
+                    gsum[r] = sumG;
                 }
             }
             wave_lds_sync();
         }
+        // B24 for every position of the chunk at once, after the sub-levels
+#pragma unroll
+        for (int r = 0; r < RPL; ++r)
+            if (my[r] >= 0)
+                lamv[r] = (rcv[r].w != 0.0f) ? dqv[r] + (gbv[r] * rcv[r].z + gsum[r]) : 0.0f;
         before_stores();
         // Published after the solve, as in the forward. Exactly
         // kStoresAfterPolls vector memory operations follow the polls: every
