@@ -424,6 +424,9 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
     // the largest over the chunk (wave-uniform)
     uint64_t lbound = ~0ull;
     int lall = Wd::kFast;
+    // link slots k this lane polls (bit k: staged link lane + 64 k is from
+    // another chunk), so that no step reads the staging tables to decide
+    unsigned pollm = 0;
 
     // The chunk's step-invariant data, before the first step: statics into
     // registers, links, gauge slots, positions and per-sub-level link bounds
@@ -499,6 +502,11 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
                  static_cast<uint32_t>(uni(static_cast<int>(m)));
         // sub-levels beyond the kept ones read every register-held link
         lall = (nsub > kLinkBoundSubs) ? Wd::kFast : uni(mx);
+#pragma unroll
+        for (int k = 0; k < Wd::kSlots; ++k) {
+            const int e = lane + 64 * k;
+            if (e < nlinks && skind[e] < 0) pollm |= 1u << k;
+        }
     }
 
     // S1..S25 for window step s: coefficients and right-hand side.
@@ -537,7 +545,7 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
         for (int k = 0; k < Wd::kSlots; ++k) {
             if (64 * k < nlinks) {
                 const int e = lane + 64 * k;
-                pw[k] = (e < nlinks && skind[e] < 0) ? poll_row(row, spos[e]) : 0;
+                pw[k] = ((pollm >> k) & 1u) ? poll_row(row, spos[e]) : 0;
             }
         }
     };
@@ -549,7 +557,7 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
         for (int k = 0; k < Wd::kSlots; ++k) {
             if (64 * k < nlinks) {
                 const int e = lane + 64 * k;
-                if (e < nlinks && skind[e] < 0) {
+                if ((pollm >> k) & 1u) {
                     if (!word_ready(pw[k], epoch)) ok = false;
                     else sval[e] = word_value(pw[k]);
                 }
@@ -983,6 +991,10 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
     // read no gauge-sparse dL/dQ
     unsigned grows = 0;
 
+    // link slots k this lane polls (bit k: staged link lane + 64 k is from
+    // another chunk) and the positions it polls (0 where it has no link)
+    unsigned pollm = 0;
+    int dpos[kSlotsDn];
     // Step-invariant data, before the step loop (the per-step work of the
     // sweep is in issue and solve): the pointers used only here are dead
     // inside the loop.
@@ -1020,6 +1032,13 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
             }
         }
         wave_lds_sync();
+#pragma unroll
+        for (int k = 0; k < kSlotsDn; ++k) {
+            const int e = lane + 64 * k;
+            const bool x = e < nlinks && skind[e] < 0;
+            pollm |= x ? 1u << k : 0u;
+            dpos[k] = x ? spos[e] : 0;
+        }
     }
     auto prepare = [&](int, bool) {};
 
@@ -1049,11 +1068,7 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
         const uint64_t* mrow = a.amsg + row * 2;
 #pragma unroll
         for (int k = 0; k < kSlotsDn; ++k) {
-            if (64 * k < nlinks) {
-                const int e = lane + 64 * k;
-                const int d = (e < nlinks && skind[e] < 0) ? spos[e] : 0;
-                poll_pair_to_lds(mrow + 2 * d, &spair[64 * k]);
-            }
+            if (64 * k < nlinks) poll_pair_to_lds(mrow + 2 * dpos[k], &spair[64 * k]);
         }
         stores_after_polls = false;
     };
@@ -1065,7 +1080,7 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
 #pragma unroll
         for (int k = 0; k < kSlotsDn; ++k) {
             const int e = lane + 64 * k;
-            if (64 * k < nlinks && e < nlinks && skind[e] < 0) {
+            if (64 * k < nlinks && ((pollm >> k) & 1u)) {
                 const u32x4 w = lds_read_dma16(&spair[e]);
                 if (!pair_ready(w, tag)) {
                     ok = false;
