@@ -427,6 +427,7 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
     // link slots k this lane polls (bit k: staged link lane + 64 k is from
     // another chunk), so that no step reads the staging tables to decide
     unsigned pollm = 0;
+    int rpos = 0;  // the position this lane probes (lane < nrep)
 
     // The chunk's step-invariant data, before the first step: statics into
     // registers, links, gauge slots, positions and per-sub-level link bounds
@@ -507,6 +508,7 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
             const int e = lane + 64 * k;
             if (e < nlinks && skind[e] < 0) pollm |= 1u << k;
         }
+        if (lane < nrep) rpos = spos[srep[lane]];
     }
 
     // S1..S25 for window step s: coefficients and right-hand side.
@@ -572,7 +574,7 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
         if (nrep <= 0) return true;  // no representatives: read the links again
         const auto row = row_rsrc(a.hist + static_cast<size_t>(a.s_base + s) * P, P * 8ull);
         bool ok = true;
-        if (lane < nrep) ok = word_ready(poll_row(row, spos[srep[lane]]), epoch);
+        if (lane < nrep) ok = word_ready(poll_row(row, rpos), epoch);
         return __all(ok);
     };
 
@@ -654,6 +656,12 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
             }
             __syncthreads();
         }
+        before_stores();
+        // Publish as soon as the solve is done (other chunks read a step's
+        // values only then), before the step's two upstream sums.
+#pragma unroll
+        for (int r = 0; r < RPL; ++r)
+            if (my[r] >= 0) publish_row(row, p0 + r * 64 + lane, x[r], epoch);
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
             float in = 0.0f, sx = 0.0f;
@@ -688,12 +696,6 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
             inext[r] = in;
             sxn[r] = sx;
         }
-        before_stores();
-        // Publish after the solve: other chunks read a step's values only once
-        // its solve is done.
-#pragma unroll
-        for (int r = 0; r < RPL; ++r)
-            if (my[r] >= 0) publish_row(row, p0 + r * 64 + lane, x[r], epoch);
         if (a.ihist) {
             const auto irow = row_rsrc(a.ihist + static_cast<size_t>(sg) * P, P * 8ull);
 #pragma unroll
@@ -995,6 +997,13 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
     // another chunk) and the positions it polls (0 where it has no link)
     unsigned pollm = 0;
     int dpos[kSlotsDn];
+    int rpos = 0;  // the position this lane probes (lane < nrep)
+    // per-lane operands of the solve, the same for every step: the position's
+    // sub-level (-1: padding), its downstream links (CSR offset, count, first
+    // kFastLinksDn codes and weights); and the gradient into x carried from
+    // one step to the next (masked dL/d out[t+1])
+    int my[RPL], e0[RPL], nl[RPL], code[RPL][kFastLinksDn];
+    float w[RPL][kFastLinksDn], gx[RPL];
     // Step-invariant data, before the step loop (the per-step work of the
     // sweep is in issue and solve): the pointers used only here are dead
     // inside the loop.
@@ -1038,6 +1047,20 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
             const bool x = e < nlinks && skind[e] < 0;
             pollm |= x ? 1u << k : 0u;
             dpos[k] = x ? spos[e] : 0;
+        }
+        if (lane < nrep) rpos = spos[srep[lane]];
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+            const int j = r * 64 + lane;
+            my[r] = psub[j];
+            e0[r] = (my[r] >= 0) ? pe0[j] : 0;
+            nl[r] = (my[r] >= 0) ? pe1[j] - e0[r] : 0;
+            gx[r] = (my[r] >= 0) ? slam[j] : 0.0f;
+#pragma unroll
+            for (int k = 0; k < kFastLinksDn; ++k) {
+                code[r][k] = (k < nl[r]) ? lkc[k][j] : 0;
+                w[r][k] = (k < nl[r]) ? lka[k][j] : 0.0f;
+            }
         }
     }
     auto prepare = [&](int, bool) {};
@@ -1101,7 +1124,7 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
         const uint64_t* row = a.amsg + static_cast<size_t>(a.s_base + s) * P * 2;
         const uint32_t tag = row_tag(epoch, a.s_base + s);
         bool ok = true;
-        if (lane < nrep) ok = word_ready(poll(row + 2 * spos[srep[lane]] + 1), tag);
+        if (lane < nrep) ok = word_ready(poll(row + 2 * rpos + 1), tag);
         return __all(ok);
     };
 
@@ -1127,29 +1150,26 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
         {
             bool ok_diag = true;
 #pragma unroll
-            for (int r = 0; r < RPL; ++r) {
-                const int j = r * 64 + lane;
-                if (pvalid[j]) ok_diag = ok_diag && (1.0f + rcv[r].x * 0.0f == 1.0f);
-            }
+            for (int r = 0; r < RPL; ++r)
+                if (my[r] >= 0) ok_diag = ok_diag && (1.0f + rcv[r].x * 0.0f == 1.0f);
             unit = __all(ok_diag);
         }
-        // per-lane operands of the solve, read once per step
-        int my[RPL], e0[RPL], nl[RPL], code[RPL][kFastLinksDn];
-        float gbv[RPL], lamv[RPL], gsum[RPL], gx[RPL], w[RPL][kFastLinksDn];
+        float gbv[RPL], lamv[RPL], gsum[RPL], pmv[RPL], gmv[RPL];
+        // the staged (other chunks') values of the fast links, read before the
+        // sub-levels: only links inside the chunk are read within them
+        float xP[RPL][kFastLinksDn], xG[RPL][kFastLinksDn];
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
-            const int j = r * 64 + lane;
-            my[r] = psub[j];
             gbv[r] = 0.0f;
             lamv[r] = 0.0f;
             gsum[r] = 0.0f;
-            gx[r] = slam[j];
-            e0[r] = pe0[j];
-            nl[r] = pe1[j] - e0[r];
+            pmv[r] = 0.0f;
+            gmv[r] = 0.0f;
 #pragma unroll
             for (int k = 0; k < kFastLinksDn; ++k) {
-                code[r][k] = lkc[k][j];
-                w[r][k] = lka[k][j];
+                const int cd = code[r][k];
+                xP[r][k] = (k < nl[r] && cd < 0) ? stP[-1 - cd] : 0.0f;
+                xG[r][k] = (k < nl[r] && cd < 0) ? stG[-1 - cd] : 0.0f;
             }
         }
         for (int l = 0; l < nsub; ++l) {
@@ -1164,8 +1184,8 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
                         for (int k = 0; k < kFastLinksDn; ++k) {
                             if (k < nl[r]) {
                                 const int cd = code[r][k];
-                                const float Pd = (cd >= 0) ? ldsP[cd] : stP[-1 - cd];
-                                const float Gd = (cd >= 0) ? ldsG[cd] : stG[-1 - cd];
+                                const float Pd = (cd >= 0) ? ldsP[cd] : xP[r][k];
+                                const float Gd = (cd >= 0) ? ldsG[cd] : xG[r][k];
                                 sumP = sumP + w[r][k] * Pd;              // B27
                                 sumG = sumG + Gd * w[r][k];              // B24 N^T gi
                             }
@@ -1186,17 +1206,14 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
                     const float Gm = rc.y * gb;
                     ldsP[j] = Pm;
                     ldsG[j] = Gm;
+                    pmv[r] = Pm;
+                    gmv[r] = Gm;
                     gbv[r] = gb;
                     gsum[r] = sumG;
                 }
             }
             wave_lds_sync();
         }
-        // B24 for every position of the chunk at once, after the sub-levels
-#pragma unroll
-        for (int r = 0; r < RPL; ++r)
-            if (my[r] >= 0)
-                lamv[r] = (rcv[r].w != 0.0f) ? dqv[r] + (gbv[r] * rcv[r].z + gsum[r]) : 0.0f;
         before_stores();
         // Published after the solve, as in the forward. Exactly
         // kStoresAfterPolls vector memory operations follow the polls: every
@@ -1204,18 +1221,22 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
             const int j = r * 64 + lane;
-            publish_pair(mrow, p0 + j, ldsP[j], ldsG[j], tag);
+            publish_pair(mrow, p0 + j, pmv[r], gmv[r], tag);
         }
 #pragma unroll
         for (int r = 0; r < RPL; ++r)
             a.gbuf[static_cast<size_t>(sg) * P + p0 + r * 64 + lane] = gbv[r];
         stores_after_polls = true;
+        // B24 for every position of the chunk at once, after the publish
+#pragma unroll
+        for (int r = 0; r < RPL; ++r)
+            if (my[r] >= 0)
+                lamv[r] = (rcv[r].w != 0.0f) ? dqv[r] + (gbv[r] * rcv[r].z + gsum[r]) : 0.0f;
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
             if (my[r] >= 0) {
-                const int j = r * 64 + lane;
-                slam[j] = lamv[r];
-                if (last) a.st_lam[p0 + j] = lamv[r];
+                gx[r] = lamv[r];
+                if (last) a.st_lam[p0 + r * 64 + lane] = lamv[r];
             }
         }
     };
