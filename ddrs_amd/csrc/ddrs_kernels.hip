@@ -529,7 +529,7 @@ __global__ __launch_bounds__(64, 6) void k_fwd_window(FwdArgs a) {
                         p * 4, 0, 0, 0);
                 Coef co;
                 PowEvalInl pe(sln);
-                mc_coef<PH>(a.P, rs[r], q, co, pe);
+                mc_coef<PH, PowEvalInl, true>(a.P, rs[r], q, co, pe);
                 pb[j] = ((co.c2 * it) + (co.c3 * q)) + (co.c4 * qp);             // S25
                 pnc1[j] = -co.c1;
                 ok_diag = ok_diag && (1.0f + (-co.c1) * 0.0f == 1.0f);
