@@ -100,7 +100,8 @@ struct ddrs_workspace {
     float* gqs = nullptr;                    // [max_steps][P] dL/dQ in schedule order
     const float* qps_src = nullptr;          // q' it was permuted from (rows qps_rows)
     int qps_rows = 0;
-    float4 *rec = nullptr, *recj = nullptr;  // [window][P]
+    float4* rec = nullptr;  // [window][P]
+    float* recj = nullptr;  // four planes of [window][P]
     float* gbuf = nullptr;                   // [window][P]
     int* err = nullptr;
     uint32_t epoch = 0;
@@ -640,7 +641,7 @@ static void ws_buffers(ddrs_workspace* ws, int32_t flags, F&& f) {
     f(&ws->ihist, WR * sizeof(float2));
     f(&ws->amsg, WR * 16);
     f(&ws->rec, WR * sizeof(float4));
-    f(&ws->recj, WR * sizeof(float4));
+    f(&ws->recj, WR * 4 * sizeof(float));
     f(&ws->qps, ((flags & DDRS_WS_SCHEDULE_FORCING) ? 1 : steps) * R * 4);
     if (!(flags & DDRS_WS_NO_DENSE_GRAD)) f(&ws->gqs, steps * R * 4);
     f(&ws->gbuf, WR * 4);
@@ -900,6 +901,7 @@ static LinArgs lin_args(ddrs_workspace* ws, const ddrs_params* p, int t0, int ma
     L.ck_i = ck_i;
     L.rec = ws->rec;
     L.recj = ws->recj;
+    L.plane = static_cast<size_t>(ws->window) * ws->plan->sc.P;
     L.err = ws->err;
     return L;
 }
@@ -947,6 +949,7 @@ static void run_adj_segment(ddrs_workspace* ws, const ddrs_params* p, const AdjS
     F.perm = sc.perm;
     F.t0 = g.t0;
     F.recj = ws->recj;
+    F.plane = static_cast<size_t>(ws->window) * sc.P;
     F.gbuf = ws->gbuf;
     F.acc_n = ws->acc_n;
     F.acc_qsp = ws->acc_qsp;

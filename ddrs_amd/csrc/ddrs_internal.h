@@ -139,7 +139,11 @@ struct LinArgs {
     const float* ck_q;     // [P] out[t0] (schedule order)
     const float* ck_i;     // [P] N out[t0]
     float4* rec;           // [window][P] {-c1, c2, d gq / d gb, mask of dL/d out[t]}
-    float4* recj;          // [window][P] d(gn, gqsp, gp, gq') / d gb
+    // d(gn, gqsp, gp, gq') / d gb of each reach-step, in four planes of
+    // `plane` floats ([window][P] each): the finishing phase reads the fourth
+    // only when dL/dq' is asked for
+    float* recj;
+    size_t plane;
     // With items == null the grid holds one workgroup per 256 positions and 32
     // steps. Otherwise one-wave workgroups take items (64 positions, 32 steps)
     // in step-block-major order from the counter *items (zero before the first
@@ -188,7 +192,8 @@ struct FinArgs {
     int R, P;
     const int* perm;
     int t0, s_base, nsteps;
-    const float4* recj;
+    const float* recj;     // four planes (LinArgs::recj)
+    size_t plane;
     const float* gbuf;
     float* acc_n;
     float* acc_qsp;

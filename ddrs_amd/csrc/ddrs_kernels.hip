@@ -866,7 +866,10 @@ __device__ __forceinline__ void lin_item(const LinArgs& a, int sb, int p, const 
         const LocalGrads g = mc_local_bwd<PH, true>(a.P, co, rc, q, qp, it, 1.0f, sx, 0.0f);
         const size_t o = static_cast<size_t>(sg) * P + p;
         a.rec[o] = make_float4(-co.c1, co.c2, g.gq, (!a.mask_lam || q > lb) ? 1.0f : 0.0f);
-        a.recj[o] = make_float4(g.gn, g.gqsp, g.gp, (!a.mask_qp || qp_raw > lb) ? g.gqp : 0.0f);
+        a.recj[o] = g.gn;
+        a.recj[a.plane + o] = g.gqsp;
+        a.recj[2 * a.plane + o] = g.gp;
+        a.recj[3 * a.plane + o] = (!a.mask_qp || qp_raw > lb) ? g.gqp : 0.0f;
         q = fmaxr(xs, lb);
         it = inext;
     }
@@ -1258,7 +1261,8 @@ __global__ __launch_bounds__(256) void k_adj_fin(FinArgs a) {
     for (int s = a.nsteps - 1; s >= 0; --s) {
         const size_t o = static_cast<size_t>(a.s_base + s) * a.P + p;
         const float gb = a.gbuf[o];
-        const float4 j = a.recj[o];
+        const float4 j = make_float4(a.recj[o], a.recj[a.plane + o], a.recj[2 * a.plane + o],
+                                     a.gqp_out ? a.recj[3 * a.plane + o] : 0.0f);
         an = an + gb * j.x;
         aq = aq + gb * j.y;
         ap = ap + gb * j.z;
