@@ -1007,6 +1007,7 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
     // one step to the next (masked dL/d out[t+1])
     int my[RPL], e0[RPL], nl[RPL], code[RPL][kFastLinksDn];
     float w[RPL][kFastLinksDn], gx[RPL];
+    bool tree = false;  // no position of the chunk has two downstream links
     // Step-invariant data, before the step loop (the per-step work of the
     // sweep is in issue and solve): the pointers used only here are dead
     // inside the loop.
@@ -1065,6 +1066,10 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
                 w[r][k] = (k < nl[r]) ? lka[k][j] : 0.0f;
             }
         }
+        bool one = true;
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) one = one && nl[r] <= 1;
+        tree = __all(one);
     }
     auto prepare = [&](int, bool) {};
 
@@ -1173,9 +1178,50 @@ __global__ __launch_bounds__(64, 6) void k_adj_sweep(SweepArgs a) {
                 const int cd = code[r][k];
                 xP[r][k] = (k < nl[r] && cd < 0) ? stP[-1 - cd] : 0.0f;
                 xG[r][k] = (k < nl[r] && cd < 0) ? stG[-1 - cd] : 0.0f;
+                // the reads are complete here: within the sub-levels no wait
+                // for them holds back a sub-level's read behind the previous
+                // sub-level's write
+                asm volatile("" : "+v"(xP[r][k]), "+v"(xG[r][k]));
             }
         }
-        for (int l = 0; l < nsub; ++l) {
+        if (tree) {
+            // every position has at most one downstream link (a tree): the
+            // sub-levels carry only P; G is gathered after them, below
+            for (int l = 0; l < nsub; ++l) {
+#pragma unroll
+                for (int r = 0; r < RPL; ++r) {
+                    if (my[r] == l) {
+                        const int j = r * 64 + lane;
+                        const int cd = code[r][0];
+                        // an in-bounds read for every lane (its own slot where
+                        // its link leaves the chunk)
+                        const float Pin = ldsP[(cd >= 0) ? cd : j];
+                        const float Pd = (cd >= 0) ? Pin : xP[r][0];
+                        float sumP = 0.0f;
+                        if (nl[r] > 0) sumP = sumP + w[r][0] * Pd;       // B27
+                        const float gb = unit ? (gx[r] - sumP)
+                                              : div_diag(gx[r] - sumP, 1.0f + rcv[r].x * 0.0f);
+                        const float Pm = rcv[r].x * gb;
+                        ldsP[j] = Pm;
+                        pmv[r] = Pm;
+                        gmv[r] = rcv[r].y * gb;
+                        gbv[r] = gb;
+                    }
+                }
+                wave_lds_sync();
+            }
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) ldsG[r * 64 + lane] = gmv[r];
+            wave_lds_sync();
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                if (my[r] >= 0 && nl[r] > 0) {
+                    const int cd = code[r][0];
+                    const float Gd = (cd >= 0) ? ldsG[cd] : xG[r][0];
+                    gsum[r] = 0.0f + Gd * w[r][0];                      // B24 N^T gi
+                }
+            }
+        } else for (int l = 0; l < nsub; ++l) {
 #pragma unroll
             for (int r = 0; r < RPL; ++r) {
                 if (my[r] == l) {

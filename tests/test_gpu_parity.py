@@ -325,11 +325,14 @@ def test_gradients_vs_fp64_finite_differences():
             assert abs(g - fd) <= 1e-3 * abs(fd) + 1e-5 * scale, (key, i, g, fd)
 
 
-@pytest.mark.parametrize("kind", ["forest", "long_chain", "weights", "star", "ragged"])
+@pytest.mark.parametrize("kind", ["forest", "long_chain", "weights", "star", "ragged", "braided"])
 def test_structures(kind):
     """Edge structures: a forest of uneven basins, a 3,000-reach chain (one
     sub-level per reach inside every chunk), non-unit edge weights, a reach
-    with 40 upstream reaches, and sizes that leave a ragged last chunk."""
+    with 40 upstream reaches, sizes that leave a ragged last chunk, and a tree
+    with 120 extra edges, so that some reaches drain into two or three
+    downstream reaches (the sweep's general sub-level loop and the links
+    beyond the register-held ones)."""
     say("structure", kind)
     rng = np.random.default_rng(12)
     vals = None
@@ -352,6 +355,21 @@ def test_structures(kind):
     elif kind == "star":
         R = 41
         rows, cols = np.full(40, 40, np.int32), np.arange(40, dtype=np.int32)
+    elif kind == "braided":
+        R = 1500
+        rows, cols = random_tree(R, 24, rng)
+        have = set(zip(rows.tolist(), cols.tolist()))
+        extra = []
+        while len(extra) < 120:
+            cl = int(rng.integers(0, R - 2))
+            r = int(rng.integers(cl + 1, min(cl + 40, R - 1) + 1))
+            if (r, cl) not in have:
+                have.add((r, cl))
+                extra.append((r, cl))
+        rows = np.concatenate([rows, np.array([e[0] for e in extra], np.int32)])
+        cols = np.concatenate([cols, np.array([e[1] for e in extra], np.int32)])
+        assert np.bincount(cols, minlength=R).max() >= 2
+        vals = rng.uniform(0.3, 0.7, rows.shape[0]).astype(np.float32)
     else:
         R = 129
         rows, cols = random_tree(R, 8, rng)
