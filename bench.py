@@ -69,9 +69,9 @@ ALGO_BYTES = {"fwd": 40, "adjoint": 72}
 #     records written 16 + 16;
 #   sweep: records read 16, (P, G) published 16 and read 16 by the upstream
 #     chunk, gb written 4; a dense dL/dQ would add 4 read;
-#   finish: records read 16, gb read 4 (this bench's loss asks for no dL/dq',
-#     which would add 4 written).
-KERNEL_BYTES = {"fwd": 28, "lin": 52, "sweep": 52, "fin": 20}
+#   finish: three record planes read 12, gb read 4 (this bench's loss asks
+#     for no dL/dq', which would add the fourth plane, 4 read, and 4 written).
+KERNEL_BYTES = {"fwd": 28, "lin": 52, "sweep": 52, "fin": 16}
 KERNEL_NAMES = {"fwd": "k_fwd_window (persistent forward)", "lin": "k_adj_lin (linearise)",
                 "sweep": "k_adj_sweep (persistent reverse sweep)", "fin": "k_adj_fin"}
 def log(*a):
