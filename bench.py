@@ -794,9 +794,9 @@ def train_bench(args, torch):
                    "schedule": schedule, "form": "persistent", "parallelism": "single GPU"},
         "roofline": roofline_of(launch_ms, rs, None, routing_kernels_ms=routing_ms,
                                 other_ms=ms_per_step - routing_ms,
-                                note="other_ms: the q' permutation, KAN head forward and "
-                                     "backward, cold start, loss, optimizer, host "
-                                     "synchronisation of the loss"),
+                                note="other_ms: KAN head forward and backward, cold start, "
+                                     "loss, optimizer, host synchronisation of the loss "
+                                     "(q' is put in schedule order once per batch)"),
         "losses": losses,
         "hbm": {"device_used_GiB": (total_b - free_b) / 2**30,
                 "device_total_GiB": total_b / 2**30, "workspace_GiB": ws.nbytes() / 2**30},
